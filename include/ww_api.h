@@ -1,0 +1,178 @@
+/*
+ * ww_api.h -- C ABI of libww, the MI355X (gfx950) search path for Wieferich and
+ * Wall-Sun-Sun primes.
+ *
+ * This is the drop-in boundary that replaces the reference's device layer: the
+ * simpleCL wrapper (simpleCL.h:63-133), the raw OpenCL buffer/queue calls of main.cpp
+ * and the five kernel entry points with their argument contracts (main.cpp:1596-1629).
+ * Plain C types only; every entry point returns 0 on success or a negative WW_E* code,
+ * with a message available from ww_last_error() (the reference prints and exits on
+ * every device error, simpleCL.c:30-348; the drop-in host maps a nonzero return to
+ * that behaviour).
+ *
+ * Semantics kept from the reference:
+ *   - ranges are half-open [low, high) over unsigned 64-bit integers;
+ *   - the PRP list holds every n in the range that is odd, has no prime factor in
+ *     3..113 and is a base-2 strong probable prime (getsegprimes.cl:228-301);
+ *   - each list entry p adds low32(s1) of s = 2^((p-1)/2) mod p^2 = s0 + p*s1 and
+ *     low32(c21) of F_{p-(p/5)} mod p^2 = c20 + p*c21 to a 64-bit checksum
+ *     (wieferich.cl:315, wallsunsun.cl:352);
+ *   - result records carry result code 1093 (Wieferich), 1 (Wall-Sun-Sun prime) or
+ *     2 (Wall-Sun-Sun special instance) with the rem/quot conventions of
+ *     wieferich.cl:247-293 and wallsunsun.cl:325-349.
+ * Changed on purpose: the list stores p only (p^-1 mod 2^64 and 2^64 mod p^2 are
+ * recomputed in registers), and the Wieferich and Wall-Sun-Sun kernels are one
+ * launch, so there is no g_one round trip.
+ *
+ * Thread safety: calls on different contexts are independent. Calls on one context
+ * are serialised by the caller (the reference drives one in-order queue from one
+ * host thread, main.cpp:1683-1795).
+ */
+#ifndef WW_API_H
+#define WW_API_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WW_OK 0
+#define WW_E_ARG (-1)      /* invalid argument */
+#define WW_E_HIP (-2)      /* HIP runtime error */
+#define WW_E_NODEV (-3)    /* no usable GPU */
+#define WW_E_CAPACITY (-4) /* request exceeds a buffer capacity */
+
+/* Smallest START the search accepts (main.cpp:1265-1269 clamps to it). */
+#define WW_MIN_START 127ull
+/* Result-buffer size of the reference (numresults, main.cpp:1119). */
+#define WW_DEFAULT_RESULTS 15000u
+
+/* One result record (the four arrays gpuSpecialPrime/gpuResult/gpuRem/gpuQuot of
+   main.cpp:1458-1493, as one struct). */
+typedef struct {
+    uint64_t p;
+    int32_t result; /* 1093 Wieferich, 1 WSS prime, 2 WSS special instance */
+    int32_t rem;
+    int32_t quot;
+    int32_t pad;
+} ww_result;
+
+/* Device counters read back at a checkpoint (main.cpp:793-796). */
+typedef struct {
+    uint64_t checksum;        /* gpuChecksum */
+    uint64_t totalprimecount; /* d_totalprimecount: PRPs tested since the last clear */
+    uint32_t primecount;      /* d_segprimecount[0]: current list length */
+    uint32_t primecount_max;  /* d_segprimecount[1]: largest list length tested */
+    uint32_t resultcount;     /* gpuResultCount (may exceed the result capacity) */
+    uint32_t reserved;
+} ww_counters;
+
+typedef struct {
+    char name[256];
+    char arch[64];
+    int compute_units;
+    int clock_khz;
+    uint64_t global_mem;
+    int warp_size;
+} ww_device_info_t;
+
+typedef struct ww_ctx ww_ctx;
+
+/* Message of the last failing call on this thread. */
+const char *ww_last_error(void);
+
+/* Number of visible GPUs (0 when none). */
+int ww_device_count(int *count);
+
+/* Device properties (replaces the clGetDeviceInfo queries, main.cpp:1325-1423). */
+int ww_device_info(int device, ww_device_info_t *info);
+
+/* Open a context on `device` with a PRP list of `list_capacity` entries and
+   `result_capacity` result records. Replaces clCreateContext / queue creation and the
+   buffer allocations of main.cpp:1310-1323, 1446-1493, 1563-1581. Counters start
+   cleared. */
+int ww_open(int device, uint64_t list_capacity, uint32_t result_capacity, ww_ctx **out);
+
+/* Release the context and its device memory (cleanup, main.cpp:69-90). */
+int ww_close(ww_ctx *ctx);
+
+/* Upper bound on the list length for one range [low, high): an explicit
+   Brun-Titchmarsh bound 2w/ln(w) on primes in a window of width w, plus the base-2
+   strong pseudoprimes it could hold. Replaces the primesieve-based sizing
+   (main.cpp:1087-1091). */
+uint64_t ww_list_bound(uint64_t low, uint64_t high);
+
+/* clearresult (clearresult.cl:36-50): zero checksum, resultcount, totalprimecount
+   and primecount_max. Stream-ordered. */
+int ww_clear_results(ww_ctx *ctx);
+
+/* clearn (clearn.cl:12-21): empty the PRP list. Stream-ordered. */
+int ww_clear_prp_count(ww_ctx *ctx);
+
+/* getsegprimes (getsegprimes.cl:228-301, launched at main.cpp:1719-1727): append the
+   PRPs of [low, high) to the list. The reference also passes a mod-30 wheel index
+   computed by findWheelOffset (main.cpp:528-587); this sieve needs none.
+   Requires WW_MIN_START <= low < high and high - low <= 2^34. Stream-ordered. */
+int ww_getsegprimes(ww_ctx *ctx, uint64_t low, uint64_t high);
+
+/* wieferich + wallsunsun (wieferich.cl:196-328, wallsunsun.cl:221-354, launched at
+   main.cpp:1738-1741): test every list entry, add to the checksum, append result
+   records, add the list length to totalprimecount and update primecount_max.
+   Stream-ordered. */
+int ww_wieferich_wallsunsun(ww_ctx *ctx);
+
+/* One whole range as the reference's chunk loop does it (main.cpp:1683-1741):
+   for each chunk of at most `chunk` integers: clearn, getsegprimes,
+   wieferich+wallsunsun. chunk = 0 picks the context's default. Stream-ordered; the
+   host does not wait between chunks. */
+int ww_search_range(ww_ctx *ctx, uint64_t low, uint64_t high, uint64_t chunk);
+
+/* Wait for all queued work (sclFinish / waitOnEvent, simpleCL.c, main.cpp:329-377). */
+int ww_finish(ww_ctx *ctx);
+
+/* Blocking read of the counters (main.cpp:793-796). */
+int ww_read_counters(ww_ctx *ctx, ww_counters *out);
+
+/* Blocking read of up to `cap` result records (main.cpp:837-840); *n receives the
+   number copied. */
+int ww_read_results(ww_ctx *ctx, ww_result *out, uint32_t cap, uint32_t *n);
+
+/* Blocking read of the PRP list (checkPRPS, main.cpp:460-470); *n receives the
+   list length (entries beyond cap are not copied). */
+int ww_read_prps(ww_ctx *ctx, uint64_t *out, uint64_t cap, uint64_t *n);
+
+/* Kernel milliseconds accumulated since the last call (and reset): the sieve/PRP
+   kernel and the Wieferich/Wall-Sun-Sun kernel, from HIP events on the context's
+   stream (ProfilesclEnqueueKernel, simpleCL.c:511-534). Waits for queued work. */
+int ww_kernel_times(ww_ctx *ctx, double *ms_sieve, double *ms_tests, uint64_t *launches);
+
+/* Per-value access to the device arithmetic, for verification. ww_residues writes
+   out[4*i..4*i+3] = (s0, s1, c20, c21) for p[i] (each p odd, >= 127, prime to 5);
+   ww_prp_flags writes 1 where n[i] (odd, > 1) is a base-2 strong probable prime.
+   Host buffers in and out; blocking. */
+int ww_residues(ww_ctx *ctx, const uint64_t *p, uint64_t n, uint64_t *out);
+int ww_prp_flags(ww_ctx *ctx, const uint64_t *n, uint64_t count, uint8_t *out);
+
+/* Host-side checks the drop-in uses to validate results (CPU, no device needed):
+   isPrime (main.cpp:301-324) and isPRP (main.cpp:245-291). */
+int ww_cpu_is_prime(uint64_t n);
+int ww_cpu_is_prp(uint64_t n);
+
+/* The reference CPU path over [low, high) (main.cpp:648-697), on the host with the
+   same arithmetic: the startup self-test's comparison target (testKernels). out =
+   {prp count, checksum, cpu result count}. */
+int ww_cpu_range(uint64_t low, uint64_t high, uint64_t out[3]);
+
+/* Measurement support, no counterpart in the reference: the issue rate of
+   independent 32x32->64 multiplies on every CU of `device` (kind 0: v_mad_u64_u32,
+   kind 1: v_mul_hi_u32), in operations per second. Prices the roofline of the
+   path, which is bound by these multiplies. */
+int ww_peak_mul32(int device, int kind, double *ops_per_s);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,165 @@
+"""Parity of the HIP path (through the C ABI) with the CPU oracle.
+
+Bit-exact everywhere: every quantity on this path is an unsigned integer. Small and
+ragged ranges are compared entry by entry; the known-answer ranges of the reference
+(tests.txt:6-25) and the full-size configurations are checked end to end.
+"""
+import random
+
+import pytest
+
+import oracle
+import wwlib
+
+pytestmark = pytest.mark.gpu
+
+P64_MAX = 18446744073709551557  # largest prime below 2^64
+T3_P = 2276306935816523
+T4_P = 338772621946054253
+
+
+def test_device_is_gfx950():
+    info = wwlib.device_info(0)
+    print("device:", info.name.decode(), info.arch.decode(), info.compute_units, "CUs")
+    assert info.arch.decode().startswith("gfx950")
+
+
+def _edge_prps():
+    ps = [1093, 3511, 495791, 42799, 49141, 88357, 90751, 104653, 1373653, 25326001,
+          3215031751, T3_P, T4_P, P64_MAX]
+    for lo in (127, (1 << 32) - 4000, 1 << 32, (1 << 40) + 1, 1 << 50, (1 << 62) - 6000,
+               1 << 63, (1 << 64) - 6000):
+        ps += oracle.prp_list(lo, min(lo + 6000, (1 << 64) - 1))[:40]
+    rng = random.Random(7)
+    for bits in range(8, 65):
+        for _ in range(6):
+            lo = rng.getrandbits(bits) | (1 << (bits - 1)) | 1
+            lo = max(lo, 127)
+            ps += oracle.prp_list(lo, min(lo + 4000, (1 << 64) - 1))[:3]
+    return sorted(set(p for p in ps if p >= 127))
+
+
+def test_residues_vs_oracle(gctx):
+    ps = _edge_prps()
+    got = gctx.residues(ps)
+    bad = [(p, g, oracle.wieferich(p), oracle.wallsunsun(p)) for p, g in zip(ps, got)
+           if g[:2] != oracle.wieferich(p) or g[2:] != oracle.wallsunsun(p)]
+    assert not bad, bad[:5]
+    print(f"{len(ps)} PRPs: residues bit-exact")
+
+
+def test_prp_flags_vs_oracle(gctx):
+    rng = random.Random(11)
+    ns = [42799, 49141, 88357, 90751, 104653, 1373653, 25326001, 3215031751,  # spsp(2)
+          49981, 83333, 129889,                                                # Fermat-only
+          2047, 3277, 4033, 4681, 8321, 3, 5, 7, 9, 15, 127, P64_MAX, (1 << 64) - 1]
+    for bits in range(3, 65):
+        ns += [rng.getrandbits(bits) | (1 << (bits - 1)) | 1 for _ in range(200)]
+    ns = [n for n in ns if n > 1]
+    got = gctx.prp_flags(ns)
+    want = [oracle.strong_prp(2, n) for n in ns]
+    bad = [(n, g, w) for n, g, w in zip(ns, got, want) if g != w]
+    assert not bad, bad[:10]
+
+
+RANGES = [
+    (127, 2_000_000),                          # 1093, 3511, every special instance p < 1001
+    (127, 128), (127, 131), (128, 129), (1000, 1001), (1001, 1003),
+    (2_000_000, 2_000_061), (5_000_001, 5_050_000),
+    ((1 << 32) - (1 << 20), (1 << 32) + (1 << 20)),  # two-branch m2p_one boundary
+    (1 << 50, (1 << 50) + (1 << 22)),          # cfg2 slice
+    (T3_P - 1000, T3_P + 1000),                # Wieferich special instance (-1 -3 p)
+    (T4_P, T4_P + 5000),                       # WSS special instance at START (inclusive)
+    ((1 << 62) - 12345, (1 << 62) + 54321),
+    (1 << 63, (1 << 63) + (1 << 22)),          # cfg3 slice
+    ((1 << 64) - (1 << 22), (1 << 64) - 1),    # top of the domain
+    ((1 << 64) - 100, (1 << 64) - 1),
+]
+
+
+@pytest.mark.parametrize("lo,hi", RANGES)
+def test_range_vs_oracle(gctx, lo, hi):
+    got = gctx.run_range(lo, hi)
+    prp, cks, recs = oracle.range_results(lo, hi)
+    assert got.prp_count == prp
+    assert got.checksum == cks, f"{got.checksum:016X} != {cks:016X}"
+    assert got.results == recs
+
+
+def test_ragged_ranges_vs_oracle(gctx):
+    rng = random.Random(3)
+    for _ in range(40):
+        bits = rng.randint(8, 64)
+        lo = max(127, rng.getrandbits(bits))
+        w = rng.choice([1, 2, 3, 59, 60, 61, 127, 128, 129, 8191, 8192, 8193, 32767, 32768,
+                        32769, 100_000])
+        hi = min(lo + w, (1 << 64) - 1)
+        if hi <= lo:
+            continue
+        got = gctx.run_range(lo, hi)
+        prp, cks, recs = oracle.range_results(lo, hi)
+        assert (got.prp_count, got.checksum, got.results) == (prp, cks, recs), (lo, hi)
+
+
+@pytest.mark.parametrize("lo,hi", [(127, 3_000_000), (1 << 63, (1 << 63) + (1 << 21)),
+                                   ((1 << 64) - (1 << 21), (1 << 64) - 1)])
+def test_prp_list_vs_oracle_and_sieve(gctx, lo, hi):
+    gctx.clear_results()
+    gctx.clear_prp_count()
+    gctx.getsegprimes(lo, hi)
+    gctx.finish()
+    got = sorted(gctx.prps())
+    assert got == oracle.prp_list(lo, hi)
+    # after CPU primality filtering the list is exactly the primes of the range
+    # (checkPRPS, main.cpp:474-515, with a sieve of Eratosthenes for primesieve)
+    primes = [p for p in got if wwlib.cpu_is_prime(p)]
+    if hi < (1 << 40):
+        assert primes == oracle.sieve_primes(lo, hi)
+    else:
+        assert primes == [p for p in got if oracle.is_prime(p)]
+
+
+def test_chunking_invariance(gctx):
+    lo, hi = 10**13, 10**13 + 3 * (1 << 22) + 12345
+    ref = gctx.run_range(lo, hi)
+    for chunk in (1 << 20, 999_983, 1 << 23):
+        got = gctx.run_range(lo, hi, chunk)
+        assert (got.prp_count, got.checksum, got.results) == (ref.prp_count, ref.checksum,
+                                                              ref.results)
+
+
+# Known answers of the reference (tests.txt): START END -> results file lines.
+KAT = [
+    ("T1", 18446744013709551615, 18446744073709551615, [], 0x509DE6503B2A6E8E),
+    ("T2", 127, 60_000_000_000, None, 0x8B77D2DA053842DD),  # START 1 clamps to 127
+    ("T3", 2276300000000000, 2276360000000000, [(T3_P, 1093, -1, -3)], 0x6522CBC4CA2E4CFF),
+    ("T4", 338772621946054253, 338772681946054253, [(T4_P, 2, 0, 91)], 0x589A69274A6AB5A8),
+]
+
+
+@pytest.mark.parametrize("name,lo,hi,lines,cks", KAT, ids=[k[0] for k in KAT])
+def test_known_answers(gctx, name, lo, hi, lines, cks):
+    got = gctx.run_range(lo, hi, 1 << 32)
+    print(f"{name}: {got.prp_count} PRPs, checksum {got.checksum:016X}")
+    assert got.checksum == cks, f"{got.checksum:016X} != {cks:016X}"
+    primes_only = [r for r in got.results if wwlib.cpu_is_prime(r[0])]
+    if lines is not None:
+        assert primes_only == lines
+
+
+def test_cfg3_full_size_properties(gctx):
+    """[2^63, 2^63 + 2^33): checksum additivity over halves, and 16 sampled 2^22-wide
+    slices bit-exact against the oracle."""
+    lo, hi = 1 << 63, (1 << 63) + (1 << 33)
+    whole = gctx.run_range(lo, hi, 1 << 32)
+    mid = lo + (1 << 32) + 777
+    a = gctx.run_range(lo, mid, 1 << 32)
+    b = gctx.run_range(mid, hi, 1 << 32)
+    assert whole.prp_count == a.prp_count + b.prp_count
+    assert whole.checksum == (a.checksum + b.checksum) % (1 << 64)
+    print(f"cfg3: {whole.prp_count} PRPs, checksum {whole.checksum:016X}")
+    for k in range(16):
+        s = lo + k * (1 << 29)
+        got = gctx.run_range(s, s + (1 << 22))
+        prp, cks, _, _ = oracle.range_summary(s, s + (1 << 22), threads=16)
+        assert (got.prp_count, got.checksum) == (prp, cks), k

@@ -1,0 +1,391 @@
+/*
+ * ww_api.cpp -- the C ABI of include/ww_api.h over HIP.
+ *
+ * One context = one device, one stream, the device counters, the PRP list and the
+ * result records (the cl_mem set of progData, main.cpp:51-66). Every HIP error is
+ * turned into a negative return code and a thread-local message; nothing here falls
+ * back to the CPU: without a usable GPU every device entry point fails.
+ */
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "../../include/ww_api.h"
+#include "ww_arith.h"
+#include "ww_kernels.h"
+
+using ww::DevState;
+
+struct ww_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    DevState *d_state = nullptr;
+    uint64_t *d_list = nullptr;
+    uint64_t list_cap = 0;
+    ww_result *d_results = nullptr;
+    uint32_t result_cap = 0;
+    int test_grid = 0;
+    uint64_t default_chunk = 0;
+    /* event pairs of launches whose durations have not been collected yet */
+    std::vector<hipEvent_t> ev_sieve, ev_tests;
+    double ms_sieve = 0, ms_tests = 0;
+    uint64_t launches = 0;
+};
+
+namespace {
+
+thread_local char g_err[512] = "";
+
+int fail(int code, const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                    \
+    do {                                                                                 \
+        hipError_t e_ = (expr);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            return fail(WW_E_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_),  \
+                        __FILE__, __LINE__);                                             \
+    } while (0)
+
+int use(ww_ctx *c)
+{
+    if (!c) return fail(WW_E_ARG, "null context");
+    HIP_TRY(hipSetDevice(c->device));
+    return WW_OK;
+}
+
+/* Record a start/stop event pair around one launch for ww_kernel_times. */
+int event_pair(ww_ctx *c, std::vector<hipEvent_t> &v, hipEvent_t *a, hipEvent_t *b)
+{
+    HIP_TRY(hipEventCreate(a));
+    HIP_TRY(hipEventCreate(b));
+    v.push_back(*a);
+    v.push_back(*b);
+    return WW_OK;
+}
+
+int collect(std::vector<hipEvent_t> &v, double *acc)
+{
+    for (size_t i = 0; i + 1 < v.size(); i += 2) {
+        float ms = 0;
+        HIP_TRY(hipEventSynchronize(v[i + 1]));
+        HIP_TRY(hipEventElapsedTime(&ms, v[i], v[i + 1]));
+        *acc += ms;
+        (void)hipEventDestroy(v[i]);
+        (void)hipEventDestroy(v[i + 1]);
+    }
+    v.clear();
+    return WW_OK;
+}
+
+/* Pending event pairs are collected once this many accumulate, so a long run holds a
+   bounded number of events. */
+constexpr size_t kMaxPendingEvents = 512;
+
+} // namespace
+
+extern "C" {
+
+const char *ww_last_error(void) { return g_err; }
+
+int ww_device_count(int *count)
+{
+    if (!count) return fail(WW_E_ARG, "null count");
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) n = 0;
+    *count = n;
+    return WW_OK;
+}
+
+int ww_device_info(int device, ww_device_info_t *info)
+{
+    if (!info) return fail(WW_E_ARG, "null info");
+    hipDeviceProp_t p;
+    HIP_TRY(hipGetDeviceProperties(&p, device));
+    memset(info, 0, sizeof *info);
+    snprintf(info->name, sizeof info->name, "%s", p.name);
+    snprintf(info->arch, sizeof info->arch, "%s", p.gcnArchName);
+    info->compute_units = p.multiProcessorCount;
+    info->clock_khz = p.clockRate;
+    info->global_mem = (uint64_t)p.totalGlobalMem;
+    info->warp_size = p.warpSize;
+    return WW_OK;
+}
+
+uint64_t ww_list_bound(uint64_t low, uint64_t high)
+{
+    if (high <= low) return 0;
+    double w = (double)(high - low);
+    /* Montgomery-Vaughan: pi(x+w) - pi(x) <= 2w/ln w for w > 1; small windows are
+       bounded by the count of odd numbers instead. Base-2 strong pseudoprimes below
+       2^64 number about 3.1e7 in all; a window gets at most its share plus slack. */
+    double primes = w < 64 ? w : 2.0 * w / std::log(w);
+    double odd = w / 2 + 1;
+    double b = std::fmin(primes, odd) * 1.02 + 4096;
+    return (uint64_t)b;
+}
+
+int ww_open(int device, uint64_t list_capacity, uint32_t result_capacity, ww_ctx **out)
+{
+    if (!out) return fail(WW_E_ARG, "null out");
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0)
+        return fail(WW_E_NODEV, "no GPU visible to HIP");
+    if (device < 0 || device >= n) return fail(WW_E_ARG, "device %d out of range", device);
+    if (list_capacity == 0 || result_capacity == 0)
+        return fail(WW_E_ARG, "capacities must be nonzero");
+    ww_ctx *c = new ww_ctx;
+    c->device = device;
+    c->list_cap = list_capacity;
+    c->result_cap = result_capacity;
+    auto bail = [&](int rc) {
+        ww_close(c);
+        return rc;
+    };
+    if (hipSetDevice(device) != hipSuccess) return bail(fail(WW_E_HIP, "hipSetDevice"));
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess)
+        return bail(fail(WW_E_HIP, "hipGetDeviceProperties"));
+    c->test_grid = prop.multiProcessorCount * 8;
+    c->default_chunk = 1ull << 32;
+    hipError_t e;
+    if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess)
+        return bail(fail(WW_E_HIP, "hipStreamCreate: %s", hipGetErrorString(e)));
+    if ((e = hipMalloc(&c->d_state, sizeof(DevState))) != hipSuccess)
+        return bail(fail(WW_E_HIP, "hipMalloc(state): %s", hipGetErrorString(e)));
+    if ((e = hipMalloc(&c->d_list, list_capacity * sizeof(uint64_t))) != hipSuccess)
+        return bail(fail(WW_E_CAPACITY, "hipMalloc(list of %llu): %s",
+                         (unsigned long long)list_capacity, hipGetErrorString(e)));
+    if ((e = hipMalloc(&c->d_results, (size_t)result_capacity * sizeof(ww_result))) != hipSuccess)
+        return bail(fail(WW_E_HIP, "hipMalloc(results): %s", hipGetErrorString(e)));
+    if ((e = hipMemsetAsync(c->d_state, 0, sizeof(DevState), c->stream)) != hipSuccess)
+        return bail(fail(WW_E_HIP, "hipMemsetAsync: %s", hipGetErrorString(e)));
+    if ((e = hipStreamSynchronize(c->stream)) != hipSuccess)
+        return bail(fail(WW_E_HIP, "hipStreamSynchronize: %s", hipGetErrorString(e)));
+    *out = c;
+    return WW_OK;
+}
+
+int ww_close(ww_ctx *c)
+{
+    if (!c) return WW_OK;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (hipEvent_t e : c->ev_sieve) (void)hipEventDestroy(e);
+    for (hipEvent_t e : c->ev_tests) (void)hipEventDestroy(e);
+    if (c->d_results) (void)hipFree(c->d_results);
+    if (c->d_list) (void)hipFree(c->d_list);
+    if (c->d_state) (void)hipFree(c->d_state);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return WW_OK;
+}
+
+int ww_clear_results(ww_ctx *c)
+{
+    int rc = use(c);
+    if (rc) return rc;
+    /* checksum, totalprimecount, then primecount_max + resultcount (the list length
+       primecount is cleared by ww_clear_prp_count only) */
+    HIP_TRY(hipMemsetAsync(&c->d_state->checksum, 0, 16, c->stream));
+    HIP_TRY(hipMemsetAsync(&c->d_state->primecount_max, 0, 8, c->stream));
+    return WW_OK;
+}
+
+int ww_clear_prp_count(ww_ctx *c)
+{
+    int rc = use(c);
+    if (rc) return rc;
+    HIP_TRY(hipMemsetAsync(&c->d_state->primecount, 0, 4, c->stream));
+    return WW_OK;
+}
+
+int ww_getsegprimes(ww_ctx *c, uint64_t low, uint64_t high)
+{
+    int rc = use(c);
+    if (rc) return rc;
+    if (low < WW_MIN_START) return fail(WW_E_ARG, "low %llu below %llu", (unsigned long long)low, WW_MIN_START);
+    if (high <= low) return fail(WW_E_ARG, "empty range");
+    if (high - low > (1ull << 34)) return fail(WW_E_ARG, "range wider than 2^34");
+    hipEvent_t a, b;
+    if ((rc = event_pair(c, c->ev_sieve, &a, &b))) return rc;
+    HIP_TRY(hipEventRecord(a, c->stream));
+    HIP_TRY(ww::launch_sieve_prp(low, high, c->d_list, c->list_cap, c->d_state, c->stream));
+    HIP_TRY(hipEventRecord(b, c->stream));
+    c->launches++;
+    if (c->ev_sieve.size() >= kMaxPendingEvents) return collect(c->ev_sieve, &c->ms_sieve);
+    return WW_OK;
+}
+
+int ww_wieferich_wallsunsun(ww_ctx *c)
+{
+    int rc = use(c);
+    if (rc) return rc;
+    hipEvent_t a, b;
+    if ((rc = event_pair(c, c->ev_tests, &a, &b))) return rc;
+    HIP_TRY(hipEventRecord(a, c->stream));
+    HIP_TRY(ww::launch_ww_tests(c->d_list, c->list_cap, c->d_state, c->d_results, c->result_cap,
+                                c->test_grid, c->stream));
+    HIP_TRY(hipEventRecord(b, c->stream));
+    if (c->ev_tests.size() >= kMaxPendingEvents) return collect(c->ev_tests, &c->ms_tests);
+    return WW_OK;
+}
+
+int ww_search_range(ww_ctx *c, uint64_t low, uint64_t high, uint64_t chunk)
+{
+    int rc = use(c);
+    if (rc) return rc;
+    if (low < WW_MIN_START) return fail(WW_E_ARG, "low below %llu", WW_MIN_START);
+    if (high <= low) return fail(WW_E_ARG, "empty range");
+    if (chunk == 0) chunk = c->default_chunk;
+    if (chunk > (1ull << 34)) chunk = 1ull << 34;
+    for (uint64_t s = low; s < high;) {
+        uint64_t e = s + chunk;
+        if (e > high || e < s) e = high;
+        if (ww_list_bound(s, e) > c->list_cap)
+            return fail(WW_E_CAPACITY, "chunk [%llu,%llu) may exceed the list capacity %llu",
+                        (unsigned long long)s, (unsigned long long)e, (unsigned long long)c->list_cap);
+        if ((rc = ww_clear_prp_count(c))) return rc;
+        if ((rc = ww_getsegprimes(c, s, e))) return rc;
+        if ((rc = ww_wieferich_wallsunsun(c))) return rc;
+        s = e;
+    }
+    return WW_OK;
+}
+
+int ww_finish(ww_ctx *c)
+{
+    int rc = use(c);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return WW_OK;
+}
+
+int ww_read_counters(ww_ctx *c, ww_counters *out)
+{
+    int rc = use(c);
+    if (rc) return rc;
+    if (!out) return fail(WW_E_ARG, "null out");
+    DevState s;
+    HIP_TRY(hipMemcpyAsync(&s, c->d_state, sizeof s, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    out->checksum = s.checksum;
+    out->totalprimecount = s.totalprimecount;
+    out->primecount = s.primecount;
+    out->primecount_max = s.primecount_max;
+    out->resultcount = s.resultcount;
+    out->reserved = 0;
+    return WW_OK;
+}
+
+int ww_read_results(ww_ctx *c, ww_result *out, uint32_t cap, uint32_t *n)
+{
+    int rc = use(c);
+    if (rc) return rc;
+    ww_counters k;
+    if ((rc = ww_read_counters(c, &k))) return rc;
+    uint32_t avail = k.resultcount < c->result_cap ? k.resultcount : c->result_cap;
+    uint32_t m = avail < cap ? avail : cap;
+    if (m) {
+        if (!out) return fail(WW_E_ARG, "null out");
+        HIP_TRY(hipMemcpyAsync(out, c->d_results, (size_t)m * sizeof(ww_result),
+                               hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+    }
+    if (n) *n = m;
+    return WW_OK;
+}
+
+int ww_read_prps(ww_ctx *c, uint64_t *out, uint64_t cap, uint64_t *n)
+{
+    int rc = use(c);
+    if (rc) return rc;
+    ww_counters k;
+    if ((rc = ww_read_counters(c, &k))) return rc;
+    uint64_t len = k.primecount;
+    uint64_t m = len < c->list_cap ? len : c->list_cap;
+    if (m > cap) m = cap;
+    if (m) {
+        if (!out) return fail(WW_E_ARG, "null out");
+        HIP_TRY(hipMemcpyAsync(out, c->d_list, m * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                               c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+    }
+    if (n) *n = len;
+    return WW_OK;
+}
+
+int ww_kernel_times(ww_ctx *c, double *ms_sieve, double *ms_tests, uint64_t *launches)
+{
+    int rc = use(c);
+    if (rc) return rc;
+    if ((rc = collect(c->ev_sieve, &c->ms_sieve))) return rc;
+    if ((rc = collect(c->ev_tests, &c->ms_tests))) return rc;
+    if (ms_sieve) *ms_sieve = c->ms_sieve;
+    if (ms_tests) *ms_tests = c->ms_tests;
+    if (launches) *launches = c->launches;
+    c->ms_sieve = c->ms_tests = 0;
+    c->launches = 0;
+    return WW_OK;
+}
+
+int ww_residues(ww_ctx *c, const uint64_t *p, uint64_t n, uint64_t *out)
+{
+    int rc = use(c);
+    if (rc) return rc;
+    if (n == 0) return WW_OK;
+    if (!p || !out) return fail(WW_E_ARG, "null buffer");
+    for (uint64_t i = 0; i < n; ++i)
+        if (p[i] < WW_MIN_START || (p[i] & 1) == 0 || p[i] % 5 == 0)
+            return fail(WW_E_ARG, "p[%llu] = %llu is not odd, >= 127 and prime to 5",
+                        (unsigned long long)i, (unsigned long long)p[i]);
+    uint64_t *dp = nullptr, *dout = nullptr;
+    HIP_TRY(hipMalloc(&dp, n * sizeof(uint64_t)));
+    hipError_t e = hipMalloc(&dout, 4 * n * sizeof(uint64_t));
+    if (e == hipSuccess) e = hipMemcpyAsync(dp, p, n * 8, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = ww::launch_residues(dp, n, dout, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(out, dout, 4 * n * 8, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(dp);
+    if (dout) (void)hipFree(dout);
+    if (e != hipSuccess) return fail(WW_E_HIP, "ww_residues: %s", hipGetErrorString(e));
+    return WW_OK;
+}
+
+int ww_prp_flags(ww_ctx *c, const uint64_t *n, uint64_t count, uint8_t *out)
+{
+    int rc = use(c);
+    if (rc) return rc;
+    if (count == 0) return WW_OK;
+    if (!n || !out) return fail(WW_E_ARG, "null buffer");
+    for (uint64_t i = 0; i < count; ++i)
+        if (n[i] < 3 || (n[i] & 1) == 0)
+            return fail(WW_E_ARG, "n[%llu] = %llu is not odd and > 1", (unsigned long long)i,
+                        (unsigned long long)n[i]);
+    uint64_t *dn = nullptr;
+    uint8_t *dout = nullptr;
+    HIP_TRY(hipMalloc(&dn, count * sizeof(uint64_t)));
+    hipError_t e = hipMalloc(&dout, count);
+    if (e == hipSuccess) e = hipMemcpyAsync(dn, n, count * 8, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = ww::launch_prp_flags(dn, count, dout, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(out, dout, count, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(dn);
+    if (dout) (void)hipFree(dout);
+    if (e != hipSuccess) return fail(WW_E_HIP, "ww_prp_flags: %s", hipGetErrorString(e));
+    return WW_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,396 @@
+/*
+ * ww_arith.h -- exact integer arithmetic of the search path, written for CDNA4 VGPRs.
+ *
+ * Compiles as device code (hipcc) and as host code (g++ or hipcc host pass), so the
+ * very same routines run the host-side self-test and result validation that the
+ * reference performs on the CPU (main.cpp:590-738, 768-956).
+ *
+ * Layout: 64-bit values are pairs of 32-bit VGPRs. Every 32x32->64 partial product is
+ * written as (u64)a*b + c with 32-bit a, b so the backend emits v_mad_u64_u32
+ * (one MUL32 each); 64-bit carries become v_add_co_u32 / v_addc_co_u32 pairs.
+ *
+ * Residues mod p^2 are two base-p digits (d0 + p*d1, 0 <= di < p) in Montgomery form
+ * with factor 2^64 -- the representation of the reference (wwcpu.cpp:104-250;
+ * Dorais & Klyve, J. Integer Seq. 14 (2011)). All outputs are exact, so any exact
+ * evaluation order yields the reference's values; the Wall-Sun-Sun ladder here uses
+ * Lucas V_n (one square + one product per exponent bit) instead of the reference's
+ * Fibonacci pair (two squares + one product, wallsunsun.cl:274-294).
+ */
+#ifndef WW_ARITH_H
+#define WW_ARITH_H
+
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define WW_HD __host__ __device__ __forceinline__
+#else
+#define WW_HD static inline
+#endif
+
+namespace ww {
+
+typedef uint64_t u64;
+typedef uint32_t u32;
+
+struct u128 {
+    u64 lo, hi;
+};
+
+/* Residue mod p^2: d0 + p*d1, both digits in [0, p). */
+struct m2 {
+    u64 d0, d1;
+};
+
+WW_HD u64 mad32(u32 a, u32 b, u64 c) { return (u64)a * b + c; }
+
+WW_HD int clz64(u64 x) { return __builtin_clzll(x); }
+WW_HD int ctz64(u64 x) { return __builtin_ctzll(x); }
+
+/* 64x64 -> 128, four MUL32. No partial sum can overflow 64 bits:
+   (2^32-1)^2 + 2*(2^32-1) = 2^64-1. */
+WW_HD u128 mul_wide(u64 a, u64 b)
+{
+    u32 a0 = (u32)a, a1 = (u32)(a >> 32), b0 = (u32)b, b1 = (u32)(b >> 32);
+    u64 t0 = (u64)a0 * b0;
+    u64 t1 = mad32(a0, b1, t0 >> 32);
+    u64 t2 = mad32(a1, b0, (u32)t1);
+    u64 t3 = mad32(a1, b1, (t1 >> 32) + (t2 >> 32));
+    u128 r;
+    r.lo = (u64)(u32)t0 | (t2 << 32);
+    r.hi = t3;
+    return r;
+}
+
+/* High 64 bits of a*b. */
+WW_HD u64 mul_hi(u64 a, u64 b)
+{
+    u32 a0 = (u32)a, a1 = (u32)(a >> 32), b0 = (u32)b, b1 = (u32)(b >> 32);
+    u64 t0 = (u64)a0 * b0;
+    u64 t1 = mad32(a0, b1, t0 >> 32);
+    u64 t2 = mad32(a1, b0, (u32)t1);
+    return mad32(a1, b1, (t1 >> 32) + (t2 >> 32));
+}
+
+/* a^2 -> 128: the cross product a0*a1 is formed once and doubled by feeding it
+   to two accumulating MADs, so the result needs no 65-bit intermediate. */
+WW_HD u128 sqr_wide(u64 a)
+{
+    u32 a0 = (u32)a, a1 = (u32)(a >> 32);
+    u64 t0 = (u64)a0 * a0;
+    u64 t1 = mad32(a0, a1, t0 >> 32);     /* a0a1 + hi(a0^2)        */
+    u64 t2 = mad32(a0, a1, (u32)t1);      /* a0a1 + lo(t1)          */
+    u64 t3 = mad32(a1, a1, (t1 >> 32) + (t2 >> 32));
+    u128 r;
+    r.lo = (u64)(u32)t0 | (t2 << 32);
+    r.hi = t3;
+    return r;
+}
+
+/* ---------------------------------------------------------------------------
+ * mod n (64-bit) helpers
+ * ------------------------------------------------------------------------- */
+
+/* x + y mod p for x, y < p (prime.h:34-43). */
+WW_HD u64 add_mod(u64 x, u64 y, u64 p)
+{
+    u64 c = (x >= p - y) ? p : 0;
+    return x + y - c;
+}
+
+/* x - y mod p for x < p, y <= p. */
+WW_HD u64 sub_mod(u64 x, u64 y, u64 p)
+{
+    u64 r = x - y;
+    return x < y ? r + p : r;
+}
+
+/* p^-1 mod 2^64 for odd p: Newton from the 5-bit seed (3p) xor 2, three 32-bit steps
+   then one 64-bit step (same value as invert(), prime.h:7-12). */
+WW_HD u64 inv64(u64 p)
+{
+    u32 p32 = (u32)p;
+    u32 x = (3u * p32) ^ 2u;
+    x *= 2u - p32 * x;
+    x *= 2u - p32 * x;
+    x *= 2u - p32 * x;
+    u64 X = x;
+    X *= 2u - p * X;
+    return X;
+}
+
+/* floor(2^64 / n) and 2^64 mod n for odd n >= 2^32, without a 64-bit division: a
+   double estimate of the quotient is within one of the truth and is corrected by the
+   exact signed remainder 2^64 - Qe*n. */
+WW_HD void pow64_divmod(u64 n, u64 *quot, u64 *rem)
+{
+    if (n >> 63) { /* 2^63 < n < 2^64 */
+        *quot = 1;
+        *rem = 0 - n;
+        return;
+    }
+    double qd = 18446744073709551616.0 / (double)n;
+    u64 qe = (u64)qd; /* qd < 2^32 + 1 */
+    u128 pr = mul_wide(qe, n);
+    u64 lo = 0 - pr.lo;
+    u64 hi = 1 - pr.hi - (pr.lo != 0 ? 1 : 0); /* (2^64 - qe*n) >> 64, two's complement */
+    if ((int64_t)hi < 0) {                      /* qe one too large */
+        *quot = qe - 1;
+        *rem = lo + n;
+    } else if (lo >= n) {                       /* qe one too small */
+        *quot = qe + 1;
+        *rem = lo - n;
+    } else {
+        *quot = qe;
+        *rem = lo;
+    }
+}
+
+/* 2^64 mod n (Montgomery 1, montInit prime.h:49 / getsegprimes.cl:107). */
+WW_HD u64 mont_one(u64 n)
+{
+    if (n >> 32) {
+        u64 qq, r;
+        pow64_divmod(n, &qq, &r);
+        return r;
+    }
+    return (0 - n) % n;
+}
+
+/* Montgomery square a^2/2^64 mod n with q = n^-1 mod 2^64 (prime.h:15-31). */
+WW_HD u64 mont_sqr(u64 a, u64 n, u64 q)
+{
+    u128 t = sqr_wide(a);
+    u64 m = t.lo * q;
+    u64 mp = mul_hi(m, n);
+    u64 r = t.hi - mp;
+    return t.hi < mp ? r + n : r;
+}
+
+/* Montgomery product a*b/2^64 mod n (prime.h:15-31). */
+WW_HD u64 mont_mul(u64 a, u64 b, u64 n, u64 q)
+{
+    u128 t = mul_wide(a, b);
+    u64 m = t.lo * q;
+    u64 mp = mul_hi(m, n);
+    u64 r = t.hi - mp;
+    return t.hi < mp ? r + n : r;
+}
+
+/* Base-2 strong probable-prime test of odd n > 1 (getsegprimes.cl:118-164,
+   prime.h:66-112). one = 2^64 mod n. */
+WW_HD bool spsp2(u64 n, u64 q, u64 one)
+{
+    u64 nm1 = n - 1;
+    int t = ctz64(nm1);
+    u64 d = nm1 >> t;
+    u64 mone = n - one;
+    u64 a = add_mod(one, one, n); /* Montgomery 2: the leading bit of d */
+    for (int i = 62 - clz64(d); i >= 0; --i) {
+        a = mont_sqr(a, n, q);
+        u64 dbl = add_mod(a, a, n);
+        a = ((d >> i) & 1) ? dbl : a;
+    }
+    bool ok = (a == one) | (a == mone);
+    for (int s = 1; s < t && !ok; ++s) {
+        a = mont_sqr(a, n, q);
+        ok = (a == mone);
+    }
+    return ok;
+}
+
+/* ---------------------------------------------------------------------------
+ * mod p^2 residues (two base-p digits, Montgomery factor 2^64)
+ * ------------------------------------------------------------------------- */
+
+/* 2^64 mod p^2 as digits (wwcpu.cpp:109-125). */
+WW_HD m2 m2_one(u64 p)
+{
+    m2 r;
+    if (p >> 32) {
+        pow64_divmod(p, &r.d1, &r.d0); /* 2^64 = d0 + p*d1 with d1 = floor(2^64/p) */
+    } else {
+        u64 p2 = p * p;
+        u64 rr = (0 - p2) % p2;
+        r.d0 = rr % p;
+        r.d1 = rr / p;
+    }
+    return r;
+}
+
+/* x + y mod p^2 (wwcpu.cpp:241-250). */
+WW_HD m2 m2_add(m2 x, m2 y, u64 p)
+{
+    bool c = x.d0 >= p - y.d0;
+    m2 r;
+    r.d0 = x.d0 + y.d0 - (c ? p : 0);
+    r.d1 = add_mod(x.d1 + (c ? 1 : 0), y.d1, p);
+    return r;
+}
+
+/* x - y mod p^2. */
+WW_HD m2 m2_sub(m2 x, m2 y, u64 p)
+{
+    bool b = x.d0 < y.d0;
+    m2 r;
+    r.d0 = x.d0 - y.d0 + (b ? p : 0);
+    r.d1 = sub_mod(x.d1, y.d1 + (b ? 1 : 0), p);
+    return r;
+}
+
+/* -x mod p^2. */
+WW_HD m2 m2_neg(m2 x, u64 p)
+{
+    m2 z;
+    z.d0 = 0;
+    z.d1 = 0;
+    return m2_sub(z, x, p);
+}
+
+/* Montgomery reduction tail shared by square and product: t = x0*y0 (128 bits) and
+   the cross term c (129 bits: c_lo, c_hi, c_top) become the digits of
+   (x*y)/2^64 mod p^2 (wwcpu.cpp:143-166, 213-236). */
+WW_HD m2 m2_redc(u128 t, u64 c_lo, u64 c_hi, u64 c_top, u64 p, u64 q)
+{
+    u64 u0 = t.lo * q;
+    u64 v1 = mul_hi(p, u0);
+    /* tp = c + u0 */
+    u64 tlo = c_lo + u0;
+    u64 cy = tlo < u0 ? 1 : 0;
+    u64 thi = c_hi + cy;
+    u64 top = c_top | ((thi < cy) ? 1 : 0);
+    u64 t1p = (top | (thi >= p ? 1 : 0)) ? thi - p : thi; /* tp >> 64 < 2p */
+    u64 up0 = tlo * q;
+    u64 v1p = mul_hi(p, up0);
+    bool b = t.hi < v1;
+    m2 r;
+    r.d0 = t.hi - v1 + (b ? p : 0);
+    r.d1 = sub_mod(t1p, v1p + (b ? 1 : 0), p);
+    return r;
+}
+
+/* x^2 / 2^64 mod p^2 (wwcpu.cpp:140-167). */
+WW_HD m2 m2_sqr(m2 x, u64 p, u64 q)
+{
+    u128 t = sqr_wide(x.d0);
+    u128 c = mul_wide(x.d0, x.d1);
+    u64 top = c.hi >> 63;
+    u64 hi2 = (c.hi << 1) | (c.lo >> 63);
+    return m2_redc(t, c.lo << 1, hi2, top, p, q);
+}
+
+/* x*y / 2^64 mod p^2 (wwcpu.cpp:210-237). */
+WW_HD m2 m2_mul(m2 x, m2 y, u64 p, u64 q)
+{
+    u128 t = mul_wide(x.d0, y.d0);
+    u128 a = mul_wide(x.d0, y.d1);
+    u128 b = mul_wide(x.d1, y.d0);
+    u64 lo = a.lo + b.lo;
+    u64 c1 = lo < a.lo ? 1 : 0;
+    u64 hi = a.hi + b.hi;
+    u64 top = hi < a.hi ? 1 : 0;
+    hi += c1;
+    top |= (hi < c1) ? 1 : 0;
+    return m2_redc(t, lo, hi, top, p, q);
+}
+
+/* Montgomery form back to ordinary digits (REDC, wwcpu.cpp:189-206). */
+WW_HD m2 m2_get(m2 x, u64 p, u64 q)
+{
+    u64 u0 = q * x.d0;
+    u64 v1 = mul_hi(p, u0);
+    u64 tp = x.d1 + u0;
+    u64 up0 = q * tp;
+    u64 t1p = tp < x.d1 ? 1 : 0;
+    u64 v1p = mul_hi(p, up0);
+    bool b = v1 != 0;
+    m2 r;
+    r.d0 = b ? p - v1 : 0;
+    r.d1 = sub_mod(t1p, v1p + (b ? 1 : 0), p);
+    return r;
+}
+
+/* x mod 5 using 2^32 = 1 (mod 5): fold the high word twice; the second fold cannot
+   carry because the first sum is at most 2^33 - 2. */
+WW_HD u32 mod5(u64 x)
+{
+    u64 s = (x & 0xffffffffu) + (x >> 32);
+    u32 t = (u32)s + (u32)(s >> 32);
+    return t % 5u;
+}
+
+/* x / 5 mod p for x < p: (x + j*p) / 5 with j = -x * p^-1 mod 5, computed as an exact
+   division by multiplication with 5^-1 mod 2^64. Returns j through *jout. */
+WW_HD u64 div5_digit(u64 x, u64 p, u32 pinv5, u32 *jout)
+{
+    u32 j = ((5u - mod5(x)) * pinv5) % 5u;
+    *jout = j;
+    /* x + j*p < 5*2^64; the quotient is < p < 2^64 so its low 64 bits suffice */
+    return (x + (u64)j * p) * 0xCCCCCCCCCCCCCCCDull;
+}
+
+/* x / 5 mod p^2 in digits: y0 = x0/5 (mod p) with 5*y0 = x0 + j*p, then
+   y1 = (x1 - j)/5 (mod p). */
+WW_HD m2 m2_div5(m2 x, u64 p)
+{
+    u32 k = mod5(p);
+    u32 pinv5 = (k * k * k) % 5u; /* k^-1 = k^3 mod 5 */
+    u32 j0, j1;
+    m2 r;
+    r.d0 = div5_digit(x.d0, p, pinv5, &j0);
+    u64 x1 = sub_mod(x.d1, (u64)j0, p);
+    r.d1 = div5_digit(x1, p, pinv5, &j1);
+    return r;
+}
+
+/* 2^((p-1)/2) mod p^2 as digits (s0, s1) (wieferich.cl:219-245, wwcpu.cpp:253-284). */
+WW_HD m2 wieferich_residue(u64 p, u64 q, m2 one)
+{
+    u64 e = p >> 1;
+    m2 a = m2_add(one, one, p); /* 2: the leading bit of e */
+    for (int i = 62 - clz64(e); i >= 0; --i) {
+        a = m2_sqr(a, p, q);
+        m2 d = m2_add(a, a, p);
+        bool bit = (e >> i) & 1;
+        a.d0 = bit ? d.d0 : a.d0;
+        a.d1 = bit ? d.d1 : a.d1;
+    }
+    return m2_get(a, p, q);
+}
+
+/* F_{p-(p/5)} mod p^2 as digits (c20, c21) (wallsunsun.cl:242-300, wwcpu.cpp:311-372),
+   by the Lucas ladder on (V_k, V_{k+1}) with P = 1, Q = -1:
+     V_2j = V_j^2 - 2(-1)^j,   V_2k+1 = V_k V_k+1 - (-1)^k,
+   and 5 F_m = 2 V_{m+1} - V_m at the end. */
+WW_HD m2 wallsunsun_residue(u64 p, u64 q, m2 one)
+{
+    u32 pm5 = mod5(p);
+    u64 m = (pm5 == 2 || pm5 == 3) ? p + 1 : p - 1;
+    m2 two = m2_add(one, one, p);
+    m2 mone = m2_neg(one, p), mtwo = m2_neg(two, p);
+    m2 vk = one;                   /* V_1 = 1 */
+    m2 vk1 = m2_add(two, one, p);  /* V_2 = 3 */
+    bool kodd = true;
+    for (int i = 62 - clz64(m); i >= 0; --i) {
+        bool bit = (m >> i) & 1;
+        m2 pr = m2_mul(vk, vk1, p, q);
+        pr = m2_add(pr, kodd ? one : mone, p);            /* V_{2k+1} */
+        m2 sin;
+        sin.d0 = bit ? vk1.d0 : vk.d0;
+        sin.d1 = bit ? vk1.d1 : vk.d1;
+        bool jodd = bit ? !kodd : kodd;
+        m2 sq = m2_sqr(sin, p, q);
+        sq = m2_add(sq, jodd ? two : mtwo, p);             /* V_{2j} */
+        vk.d0 = bit ? pr.d0 : sq.d0;
+        vk.d1 = bit ? pr.d1 : sq.d1;
+        vk1.d0 = bit ? sq.d0 : pr.d0;
+        vk1.d1 = bit ? sq.d1 : pr.d1;
+        kodd = bit;
+    }
+    m2 f5 = m2_sub(m2_add(vk1, vk1, p), vk, p);
+    return m2_get(m2_div5(f5, p), p, q);
+}
+
+} // namespace ww
+
+#endif

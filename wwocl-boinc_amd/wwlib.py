@@ -1,0 +1,268 @@
+"""ctypes binding of libww.so, the C ABI declared in include/ww_api.h.
+
+The host side of the search path for Python callers (tests, bench.py). It loads the
+in-tree library built by build.py next to this file and fails loudly when it is
+missing: there is no CPU fallback for the device entry points.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libww.so")
+HOST_EXE = os.path.join(HERE, "bin", "wwocl")
+
+WW_OK = 0
+WW_E_ARG = -1
+WW_E_HIP = -2
+WW_E_NODEV = -3
+WW_E_CAPACITY = -4
+WW_MIN_START = 127
+WW_DEFAULT_RESULTS = 15000
+
+U64 = ctypes.c_uint64
+U32 = ctypes.c_uint32
+
+
+class Result(ctypes.Structure):
+    """ww_result: one result record (main.cpp:1458-1493 as one struct)."""
+
+    _fields_ = [("p", U64), ("result", ctypes.c_int32), ("rem", ctypes.c_int32),
+                ("quot", ctypes.c_int32), ("pad", ctypes.c_int32)]
+
+    def key(self):
+        return (self.p, self.result, self.rem, self.quot)
+
+
+class Counters(ctypes.Structure):
+    """ww_counters: the device counters read at a checkpoint (main.cpp:793-796)."""
+
+    _fields_ = [("checksum", U64), ("totalprimecount", U64), ("primecount", U32),
+                ("primecount_max", U32), ("resultcount", U32), ("reserved", U32)]
+
+
+class DeviceInfo(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char * 256), ("arch", ctypes.c_char * 64),
+                ("compute_units", ctypes.c_int), ("clock_khz", ctypes.c_int),
+                ("global_mem", U64), ("warp_size", ctypes.c_int)]
+
+
+class WWError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"libww error {code}: {msg}")
+        self.code = code
+
+
+# (name, restype, argtypes) of every entry point of include/ww_api.h
+_SIGNATURES = [
+    ("ww_last_error", ctypes.c_char_p, []),
+    ("ww_device_count", ctypes.c_int, [ctypes.POINTER(ctypes.c_int)]),
+    ("ww_device_info", ctypes.c_int, [ctypes.c_int, ctypes.POINTER(DeviceInfo)]),
+    ("ww_open", ctypes.c_int, [ctypes.c_int, U64, U32, ctypes.POINTER(ctypes.c_void_p)]),
+    ("ww_close", ctypes.c_int, [ctypes.c_void_p]),
+    ("ww_list_bound", U64, [U64, U64]),
+    ("ww_clear_results", ctypes.c_int, [ctypes.c_void_p]),
+    ("ww_clear_prp_count", ctypes.c_int, [ctypes.c_void_p]),
+    ("ww_getsegprimes", ctypes.c_int, [ctypes.c_void_p, U64, U64]),
+    ("ww_wieferich_wallsunsun", ctypes.c_int, [ctypes.c_void_p]),
+    ("ww_search_range", ctypes.c_int, [ctypes.c_void_p, U64, U64, U64]),
+    ("ww_finish", ctypes.c_int, [ctypes.c_void_p]),
+    ("ww_read_counters", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Counters)]),
+    ("ww_read_results", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Result), U32,
+                                       ctypes.POINTER(U32)]),
+    ("ww_read_prps", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(U64), U64,
+                                    ctypes.POINTER(U64)]),
+    ("ww_kernel_times", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double),
+                                       ctypes.POINTER(ctypes.c_double), ctypes.POINTER(U64)]),
+    ("ww_residues", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(U64), U64,
+                                   ctypes.POINTER(U64)]),
+    ("ww_prp_flags", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(U64), U64,
+                                    ctypes.POINTER(ctypes.c_uint8)]),
+    ("ww_peak_mul32", ctypes.c_int, [ctypes.c_int, ctypes.c_int,
+                                     ctypes.POINTER(ctypes.c_double)]),
+    ("ww_cpu_is_prime", ctypes.c_int, [U64]),
+    ("ww_cpu_is_prp", ctypes.c_int, [U64]),
+    ("ww_cpu_range", ctypes.c_int, [U64, U64, ctypes.POINTER(U64)]),
+]
+
+EXPORTED = [s[0] for s in _SIGNATURES]
+
+_lib = None
+
+
+def load(path: str = LIB_PATH) -> ctypes.CDLL:
+    """Load libww.so (once). Raises FileNotFoundError when it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise FileNotFoundError(
+            f"{path} is missing: run `python -c 'import __graft_entry__ as g; g.build()'`")
+    lib = ctypes.CDLL(path)
+    for name, res, args in _SIGNATURES:
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _check(rc: int):
+    if rc != WW_OK:
+        msg = load().ww_last_error()
+        raise WWError(rc, msg.decode() if msg else "")
+
+
+def device_count() -> int:
+    n = ctypes.c_int(0)
+    _check(load().ww_device_count(ctypes.byref(n)))
+    return n.value
+
+
+def device_info(device: int = 0) -> DeviceInfo:
+    info = DeviceInfo()
+    _check(load().ww_device_info(device, ctypes.byref(info)))
+    return info
+
+
+def peak_mul32(device: int = 0, kind: int = 0) -> float:
+    """Measured issue rate (ops/s) of v_mad_u64_u32 (kind 0) or v_mul_hi_u32 (kind 1)."""
+    v = ctypes.c_double(0)
+    _check(load().ww_peak_mul32(device, kind, ctypes.byref(v)))
+    return v.value
+
+
+def list_bound(low: int, high: int) -> int:
+    return load().ww_list_bound(low, high)
+
+
+def cpu_is_prime(n: int) -> bool:
+    return bool(load().ww_cpu_is_prime(n))
+
+
+def cpu_is_prp(n: int) -> bool:
+    return bool(load().ww_cpu_is_prp(n))
+
+
+def cpu_range(low: int, high: int):
+    out = (U64 * 3)()
+    _check(load().ww_cpu_range(low, high, out))
+    return tuple(out)
+
+
+@dataclass
+class RangeResult:
+    prp_count: int
+    checksum: int
+    results: list  # sorted (p, result, rem, quot), Wieferich before Wall-Sun-Sun
+
+
+def sort_results(recs):
+    """Order of the reference's results file: by p, Wieferich (1093) first for equal p
+    (stable bubble sort over records appended W-kernel-first, main.cpp:843-854)."""
+    return sorted(recs, key=lambda r: (r[0], 0 if r[1] == 1093 else 1))
+
+
+class Context:
+    """One device context: stream, counters, PRP list and result records."""
+
+    def __init__(self, device: int = 0, list_capacity: int = 1 << 24,
+                 result_capacity: int = WW_DEFAULT_RESULTS):
+        self.lib = load()
+        self.h = ctypes.c_void_p()
+        _check(self.lib.ww_open(device, list_capacity, result_capacity, ctypes.byref(self.h)))
+        self.list_capacity = list_capacity
+        self.result_capacity = result_capacity
+
+    def close(self):
+        if self.h:
+            self.lib.ww_close(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # reference kernel-level entry points
+    def clear_results(self):
+        _check(self.lib.ww_clear_results(self.h))
+
+    def clear_prp_count(self):
+        _check(self.lib.ww_clear_prp_count(self.h))
+
+    def getsegprimes(self, low: int, high: int):
+        _check(self.lib.ww_getsegprimes(self.h, low, high))
+
+    def wieferich_wallsunsun(self):
+        _check(self.lib.ww_wieferich_wallsunsun(self.h))
+
+    def search_range(self, low: int, high: int, chunk: int = 0):
+        _check(self.lib.ww_search_range(self.h, low, high, chunk))
+
+    def finish(self):
+        _check(self.lib.ww_finish(self.h))
+
+    def counters(self) -> Counters:
+        c = Counters()
+        _check(self.lib.ww_read_counters(self.h, ctypes.byref(c)))
+        return c
+
+    def results(self):
+        cap = self.result_capacity
+        buf = (Result * cap)()
+        n = U32(0)
+        _check(self.lib.ww_read_results(self.h, buf, cap, ctypes.byref(n)))
+        return [buf[i].key() for i in range(n.value)]
+
+    def prps(self):
+        n = U64(0)
+        _check(self.lib.ww_read_prps(self.h, None, 0, ctypes.byref(n)))
+        if n.value == 0:
+            return []
+        buf = (U64 * n.value)()
+        _check(self.lib.ww_read_prps(self.h, buf, n.value, ctypes.byref(n)))
+        return list(buf)
+
+    def kernel_times(self):
+        a, b, n = ctypes.c_double(), ctypes.c_double(), U64()
+        _check(self.lib.ww_kernel_times(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(n)))
+        return a.value, b.value, n.value
+
+    def residues(self, ps):
+        n = len(ps)
+        if n == 0:
+            return []
+        pin = (U64 * n)(*ps)
+        out = (U64 * (4 * n))()
+        _check(self.lib.ww_residues(self.h, pin, n, out))
+        return [tuple(out[4 * i:4 * i + 4]) for i in range(n)]
+
+    def prp_flags(self, ns):
+        n = len(ns)
+        if n == 0:
+            return []
+        nin = (U64 * n)(*ns)
+        out = (ctypes.c_uint8 * n)()
+        _check(self.lib.ww_prp_flags(self.h, nin, n, out))
+        return [bool(v) for v in out]
+
+    # convenience: one whole range, counters cleared first
+    def run_range(self, low: int, high: int, chunk: int = 0) -> RangeResult:
+        self.clear_results()
+        self.clear_prp_count()
+        self.search_range(low, high, chunk)
+        self.finish()
+        c = self.counters()
+        if c.resultcount > self.result_capacity:
+            raise WWError(WW_E_CAPACITY, f"{c.resultcount} results overflowed the buffer")
+        return RangeResult(c.totalprimecount, c.checksum, sort_results(self.results()))
