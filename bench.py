@@ -23,6 +23,10 @@ LO = 1 << 63
 HI = (1 << 63) + (1 << 33)
 CHUNK = 1 << 32
 BITS = 64  # every p in the range has bit length 64
+# HBM bytes per list entry moved by k_ww_tests: FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 per
+# launch (factors from a 1 GiB copy kernel) over the entries of the launch, measured with
+# rocprofv3 --pmc on MI355X for this workload. Algorithmic: the 8-byte list read.
+TRAFFIC_BYTES_PER_ENTRY = 8.00
 
 
 def mul32_path(b: int) -> int:
@@ -64,6 +68,8 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="also launch one plain copy kernel of BYTES (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -73,11 +79,14 @@ def main():
     info = wwlib.device_info(0)
     peak = wwlib.peak_mul32(0, 0)
     cap = wwlib.list_bound(LO, LO + CHUNK)
+    copy_ms = wwlib.copy_probe(args.copy_probe) if args.copy_probe else None
+    step_checksum = None
     with wwlib.Context(0, list_capacity=cap) as ctx:
         for _ in range(args.warmup):
             ctx.clear_results()
             ctx.search_range(LO, HI, CHUNK)
-        ctx.finish()
+            ctx.finish()
+            step_checksum = ctx.counters().checksum
         ctx.kernel_times()  # discard warmup kernel times
         ctx.clear_results()
         ctx.finish()
@@ -113,11 +122,14 @@ def main():
         "data": "synthetic: the number line [2^63, 2^63+2^33), generated on the device",
         "config": {"workload": "plain form, range [2^63, 2^63+2^33) in 2^32-wide chunks",
                    "range": [LO, HI], "chunk": CHUNK, "prps_per_step": per_step,
-                   "checksum": f"{c.checksum:016X}", "device": info.name.decode(),
+                   "step_checksum": None if step_checksum is None else f"{step_checksum:016X}",
+                   "device": info.name.decode(),
                    "arch": info.arch.decode(), "cus": info.compute_units},
         "roofline": {"bound": "valu", "kernel": "k_ww_tests",
                      "achieved": ach / 1e12, "peak": peak / 1e12, "unit": "TMUL32/s",
-                     "frac": ach / peak, "traffic": None,
+                     "frac": ach / peak,
+                     "traffic": per_launch * TRAFFIC_BYTES_PER_ENTRY,
+                     "traffic_algorithmic": per_launch * 8,
                      "per_unit": f"{mul32_tests(BITS)} MUL32 per tested prime (89b-103, b=64)",
                      "avg_launch_ms": avg_tests_s * 1e3, "launches": launches},
         "roofline_path": {"achieved": path_ach / 1e12, "peak": peak / 1e12, "unit": "TMUL32/s",
@@ -126,6 +138,8 @@ def main():
         "kernel_ms_per_step": {"k_sieve_prp": ms_sieve / args.steps,
                                "k_ww_tests": ms_tests / args.steps},
     }
+    if copy_ms is not None:
+        out["copy_probe"] = {"bytes": args.copy_probe, "ms": copy_ms}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)

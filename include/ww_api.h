@@ -171,6 +171,12 @@ int ww_cpu_range(uint64_t low, uint64_t high, uint64_t out[3]);
    path, which is bound by these multiplies. */
 int ww_peak_mul32(int device, int kind, double *ops_per_s);
 
+/* Measurement support, no counterpart in the reference: one launch of a plain copy
+   kernel moving `bytes` (a multiple of 16) between two fresh device buffers; *ms is
+   its duration. Calibrates the HBM traffic counters (FETCH_SIZE, WRITE_SIZE) against
+   a known byte count. */
+int ww_copy_probe(int device, uint64_t bytes, double *ms);
+
 #ifdef __cplusplus
 }
 #endif

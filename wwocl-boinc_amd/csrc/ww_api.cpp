@@ -115,6 +115,7 @@ int ww_device_info(int device, ww_device_info_t *info)
     HIP_TRY(hipGetDeviceProperties(&p, device));
     memset(info, 0, sizeof *info);
     snprintf(info->name, sizeof info->name, "%s", p.name);
+    if (!info->name[0]) (void)hipDeviceGetName(info->name, (int)sizeof info->name, device);
     snprintf(info->arch, sizeof info->arch, "%s", p.gcnArchName);
     info->compute_units = p.multiProcessorCount;
     info->clock_khz = p.clockRate;

@@ -45,7 +45,44 @@ __global__ __launch_bounds__(256) void k_mul_hi_u32(uint64_t *sink, uint32_t m, 
     if (s == 0x9e3779b9u) sink[0] = s;
 }
 
+__global__ __launch_bounds__(256) void k_copy_probe(const uint4 *src, uint4 *dst, uint64_t n)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256)
+        dst[i] = src[i];
+}
+
 } // namespace
+
+extern "C" int ww_copy_probe(int device, uint64_t bytes, double *ms)
+{
+    if (!ms || bytes < 16 || bytes % 16) return WW_E_ARG;
+    if (hipSetDevice(device) != hipSuccess) return WW_E_HIP;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return WW_E_HIP;
+    uint4 *a = nullptr, *b = nullptr;
+    if (hipMalloc(&a, bytes) != hipSuccess) return WW_E_HIP;
+    if (hipMalloc(&b, bytes) != hipSuccess) {
+        (void)hipFree(a);
+        return WW_E_HIP;
+    }
+    (void)hipMemset(a, 1, bytes);
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    (void)hipEventRecord(e0, 0);
+    hipLaunchKernelGGL(k_copy_probe, dim3(prop.multiProcessorCount * 8), dim3(256), 0, 0, a, b,
+                       bytes / 16);
+    (void)hipEventRecord(e1, 0);
+    hipError_t e = hipEventSynchronize(e1);
+    float t = 0;
+    (void)hipEventElapsedTime(&t, e0, e1);
+    *ms = t;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    return e == hipSuccess ? WW_OK : WW_E_HIP;
+}
 
 extern "C" int ww_peak_mul32(int device, int kind, double *ops_per_s)
 {

@@ -82,6 +82,7 @@ _SIGNATURES = [
                                     ctypes.POINTER(ctypes.c_uint8)]),
     ("ww_peak_mul32", ctypes.c_int, [ctypes.c_int, ctypes.c_int,
                                      ctypes.POINTER(ctypes.c_double)]),
+    ("ww_copy_probe", ctypes.c_int, [ctypes.c_int, U64, ctypes.POINTER(ctypes.c_double)]),
     ("ww_cpu_is_prime", ctypes.c_int, [U64]),
     ("ww_cpu_is_prp", ctypes.c_int, [U64]),
     ("ww_cpu_range", ctypes.c_int, [U64, U64, ctypes.POINTER(U64)]),
@@ -131,6 +132,13 @@ def peak_mul32(device: int = 0, kind: int = 0) -> float:
     """Measured issue rate (ops/s) of v_mad_u64_u32 (kind 0) or v_mul_hi_u32 (kind 1)."""
     v = ctypes.c_double(0)
     _check(load().ww_peak_mul32(device, kind, ctypes.byref(v)))
+    return v.value
+
+
+def copy_probe(bytes_: int, device: int = 0) -> float:
+    """Duration (ms) of one plain device-to-device copy kernel of `bytes_` bytes."""
+    v = ctypes.c_double(0)
+    _check(load().ww_copy_probe(device, bytes_, ctypes.byref(v)))
     return v.value
 
 
