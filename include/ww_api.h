@@ -165,6 +165,50 @@ int ww_cpu_is_prp(uint64_t n);
    {prp count, checksum, cpu result count}. */
 int ww_cpu_range(uint64_t low, uint64_t high, uint64_t out[3]);
 
+/* ---------------------------------------------------------------------------
+ * Persistent form: many ranges in one launch. No counterpart in the reference, which
+ * runs one range per launch sequence (main.cpp:1683-1741); per-range checksums and
+ * counts equal what the plain entry points give for each range alone.
+ * ------------------------------------------------------------------------- */
+
+typedef struct {
+    uint64_t lo, hi; /* half-open, WW_MIN_START <= lo < hi */
+} ww_range;
+
+typedef struct {
+    uint64_t checksum;  /* as gpuChecksum for this range alone */
+    uint64_t prp_count; /* tested primes (list entries) of this range */
+} ww_range_sum;
+
+typedef struct ww_batch ww_batch;
+
+/* Upload a table of n >= 1 ranges and allocate their sums and `result_capacity`
+   records. Allocates device memory. */
+int ww_batch_create(ww_ctx *ctx, const ww_range *ranges, uint32_t n, uint32_t result_capacity,
+                    ww_batch **out);
+
+/* Launch geometry: workgroups (4 waves each) per CU; 0 = as many as fit. */
+int ww_batch_set_grid(ww_batch *b, int blocks_per_cu);
+
+/* Stream-ordered search of every range of the batch: clears the batch's sums, then one
+   persistent kernel whose waves claim 8192-integer segments from a shared cursor.
+   Allocates nothing and records no memset, so it may run under graph capture. */
+int ww_batch_launch(ww_ctx *ctx, ww_batch *b);
+
+/* The same launch captured once into a HIP graph on first use and replayed. */
+int ww_batch_launch_graph(ww_ctx *ctx, ww_batch *b);
+
+/* Blocking read: per-range sums (n entries, in table order) and up to `cap` records
+   whose pad field holds the range's index; *nrec = records copied, *total = records
+   produced (more than the capacity means overflow). Any pointer may be null. */
+int ww_batch_read(ww_ctx *ctx, ww_batch *b, ww_range_sum *sums, ww_result *recs, uint32_t cap,
+                  uint32_t *nrec, uint32_t *total);
+
+/* Kernel milliseconds of the last completed batch launch (HIP events on the stream). */
+int ww_batch_time(ww_ctx *ctx, ww_batch *b, double *ms);
+
+int ww_batch_destroy(ww_batch *b);
+
 /* Measurement support, no counterpart in the reference: the issue rate of
    independent 32x32->64 multiplies on every CU of `device` (kind 0: v_mad_u64_u32,
    kind 1: v_mul_hi_u32), in operations per second. Prices the roofline of the

@@ -1,0 +1,94 @@
+"""The persistent form (ww_batch_*): per-range results equal the plain form's and the
+oracle's; graph replays equal eager launches; results do not depend on the mix."""
+import random
+
+import pytest
+
+import oracle
+import wwlib
+from test_gpu_parity import RANGES
+
+pytestmark = pytest.mark.gpu
+
+
+def test_batch_edge_ranges_vs_oracle(gctx):
+    with wwlib.Batch(gctx, RANGES) as b:
+        b.launch()
+        got = b.read()
+    for (lo, hi), (prp, cks, recs) in zip(RANGES, got):
+        want = oracle.range_results(lo, hi)
+        assert (prp, cks, recs) == want, (lo, hi)
+
+
+def test_batch_ragged_mix_vs_oracle(gctx):
+    rng = random.Random(17)
+    ranges = []
+    for _ in range(300):
+        bits = rng.randint(8, 64)
+        lo = max(127, rng.getrandbits(bits))
+        hi = min(lo + rng.choice([1, 2, 61, 8191, 8192, 8193, 20000, 70000]), (1 << 64) - 1)
+        if hi > lo:
+            ranges.append((lo, hi))
+    with wwlib.Batch(gctx, ranges) as b:
+        b.launch()
+        got = b.read()
+    for (lo, hi), g in zip(ranges, got):
+        assert g == oracle.range_results(lo, hi), (lo, hi)
+
+
+def test_batch_workload_equals_plain_per_range(gctx):
+    """4096 ranges (seed 0). Per-range checksums and counts bit-identical to
+    plain per-range launches; 64 of the narrow ones also against the oracle."""
+    ranges = wwlib.batch_workload()
+    with wwlib.Batch(gctx, ranges) as b:
+        b.launch()
+        got = b.read()
+        ms = b.time_ms()
+    total = sum(g[0] for g in got)
+    print(f"batch: {len(ranges)} ranges, {total} PRPs in {ms:.1f} ms")
+    for (lo, hi), g in zip(ranges, got):
+        plain = gctx.run_range(lo, hi, 1 << 32)
+        assert (g[0], g[1], g[2]) == (plain.prp_count, plain.checksum, plain.results), (lo, hi)
+    narrow = sorted(range(len(ranges)), key=lambda i: ranges[i][1] - ranges[i][0])[:64]
+    for i in narrow:
+        lo, hi = ranges[i]
+        assert got[i] == oracle.range_results(lo, hi), (lo, hi)
+
+
+def test_batch_graph_replays_equal_eager(gctx):
+    ranges = wwlib.batch_workload(n=256, seed=5, wmax=24)
+    with wwlib.Batch(gctx, ranges) as b:
+        b.launch()
+        eager = b.read()
+        for _ in range(3):
+            b.launch(graph=True)
+            assert b.read() == eager
+            gctx.run_range(10**12, 10**12 + 12345)  # eager work between replays
+            b.launch()
+            assert b.read() == eager
+
+
+def test_batch_result_independent_of_order_and_grid(gctx):
+    ranges = wwlib.batch_workload(n=512, seed=9, wmax=22)
+    with wwlib.Batch(gctx, ranges) as b:
+        b.launch()
+        ref = b.read()
+    perm = list(range(len(ranges)))
+    random.Random(1).shuffle(perm)
+    with wwlib.Batch(gctx, [ranges[i] for i in perm]) as b:
+        for bpc in (1, 3, 0):
+            b.set_grid(bpc)
+            b.launch()
+            got = b.read()
+            assert [got[perm.index(i)] for i in range(len(ranges))] == ref
+
+
+def test_batch_single_range_equals_plain_at_full_size(gctx):
+    lo, hi = 1 << 63, (1 << 63) + (1 << 33)
+    with wwlib.Batch(gctx, [(lo, hi)]) as b:
+        b.launch()
+        (prp, cks, recs), = b.read()
+        ms = b.time_ms()
+    plain = gctx.run_range(lo, hi, 1 << 32)
+    print(f"persistent cfg3: {prp} PRPs in {ms:.1f} ms = {prp / ms * 1e3:.3e} /s")
+    assert (prp, cks, recs) == (plain.prp_count, plain.checksum, plain.results)

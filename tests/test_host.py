@@ -1,0 +1,161 @@
+"""The drop-in host program (wwocl-boinc_amd/bin/wwocl): command line, checkpoint files
+and results file, against the reference's behaviour (main.cpp) and known answers
+(tests.txt). CPU tests stop before the device is opened; GPU tests run whole ranges."""
+import os
+import subprocess
+
+import pytest
+
+import oracle
+import wwlib
+
+EXE = wwlib.HOST_EXE
+
+
+def run(tmp_path, *args, timeout=600):
+    r = subprocess.run([EXE, *map(str, args)], cwd=tmp_path, capture_output=True, text=True,
+                       timeout=timeout)
+    return r.returncode, r.stdout, r.stderr
+
+
+def write_state(tmp_path, name, start, stop, current, cksm=0, psum=0.0, pcnt=0):
+    (tmp_path / name).write_text(f"{start} {stop} {current} {cksm} {psum:f} {pcnt}\n")
+
+
+def results(tmp_path):
+    return (tmp_path / "results-WW.txt").read_text()
+
+
+def no_gpu():
+    return wwlib.device_count() == 0
+
+
+# ----------------------------------------------------------------------------- CPU
+
+
+def test_usage_without_arguments(tmp_path):
+    rc, out, _ = run(tmp_path)
+    assert rc == 1 and "Usage:" in out
+
+
+def test_end_must_exceed_start(tmp_path):
+    rc, out, err = run(tmp_path, "1e3", 1000)
+    assert rc == 1 and "END must be > START" in out + err
+
+
+def test_unparsable_exponent_form(tmp_path):
+    rc, out, _ = run(tmp_path, "0e5", 1000)
+    assert rc == 1 and "unable to parse START" in out
+    rc, out, _ = run(tmp_path, "1e30", "1e31")
+    assert rc == 1 and "START is out of range" in out
+
+
+def test_finished_workunit_resumes_complete(tmp_path):
+    """A state file whose current == END: 'Workunit complete.', exit 0 (main.cpp:1544-1551)."""
+    write_state(tmp_path, "stateA-WW.txt", 2276300000000000, 2276360000000000,
+                2276360000000000, 7)
+    rc, out, _ = run(tmp_path, "227630e10", "227636e10")
+    assert rc == 0
+    assert "Resuming search from checkpoint. Current: 2276360000000000" in out
+    assert "Workunit complete." in out
+
+
+def test_garbage_state_a_falls_back_to_b(tmp_path):
+    (tmp_path / "stateA-WW.txt").write_text("garbage\n")
+    write_state(tmp_path, "stateB-WW.txt", 1000, 5000, 5000)
+    rc, out, err = run(tmp_path, 1000, 5000)
+    assert rc == 0 and "Cannot parse stateA-WW.txt" in err and "Workunit complete." in out
+
+
+def test_newest_state_file_wins(tmp_path):
+    write_state(tmp_path, "stateA-WW.txt", 1000, 5000, 5000)
+    write_state(tmp_path, "stateB-WW.txt", 1000, 5000, 3000)
+    rc, out, _ = run(tmp_path, 1000, 5000)
+    assert rc == 0 and "Current: 5000" in out
+
+
+@pytest.mark.skipif(not no_gpu(), reason="would start a search on the visible GPU")
+def test_mismatched_state_starts_fresh_and_truncates_results(tmp_path):
+    write_state(tmp_path, "stateA-WW.txt", 1, 2, 2)
+    (tmp_path / "results-WW.txt").write_text("old line\n")
+    rc, out, _ = run(tmp_path, 1000, 5000)
+    assert "Beginning a new search" in out
+    assert results(tmp_path) == ""
+    assert rc == 1  # no device here
+
+
+@pytest.mark.skipif(not no_gpu(), reason="would start a search on the visible GPU")
+def test_start_clamps_to_127_and_exponent_form(tmp_path):
+    rc, out, _ = run(tmp_path, 1, "6e10")
+    assert "START below minimum. Starting search at 127" in out
+    assert "Starting search at: 127\nStopping search at: 60000000000" in out
+
+
+# ----------------------------------------------------------------------------- GPU
+
+
+def expected_lines(lo, hi):
+    lines = []
+    for p, code, rem, quot in oracle.range_results(lo, hi)[2]:
+        if not oracle.is_prime(p):
+            continue
+        if code == 1093:
+            lines.append(f"{p} is a Wieferich prime" if (rem, quot) == (1, 0) else
+                         f"{p} is a Wieferich special instance ({rem:+d} {quot:+d} p)")
+        elif code == 1:
+            lines.append(f"{p} is a WallSunSun prime")
+        else:
+            lines.append(f"{p} is a WallSunSun special instance (+0 {quot:+d} p)")
+    return lines
+
+
+@pytest.mark.gpu
+def test_host_small_range_matches_oracle(tmp_path):
+    rc, out, err = run(tmp_path, 1, "3e6")
+    assert rc == 0, out + err
+    assert "GPU Wieferich/WallSunSun kernel tests passed" in out
+    text = results(tmp_path).splitlines()
+    prp, cks, _ = oracle.range_results(127, 3_000_000)
+    assert text[:-1] == expected_lines(127, 3_000_000)
+    assert text[-1] == f"{cks:016X}"
+    assert "1093 is a Wieferich prime" in text and "3511 is a Wieferich prime" in text
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("start,end,want", [
+    ("18446744013709551615", "18446744073709551615", "509DE6503B2A6E8E\n"),
+    ("227630e10", "227636e10",
+     "2276306935816523 is a Wieferich special instance (-1 -3 p)\n6522CBC4CA2E4CFF\n"),
+    ("338772621946054253", "338772681946054253",
+     "338772621946054253 is a WallSunSun special instance (+0 +91 p)\n589A69274A6AB5A8\n"),
+], ids=["T1", "T3", "T4"])
+def test_host_known_answers(tmp_path, start, end, want):
+    rc, out, err = run(tmp_path, start, end)
+    assert rc == 0, out + err
+    assert results(tmp_path) == want
+    assert "Workunit complete." in out
+
+
+@pytest.mark.gpu
+def test_host_known_answer_t2_last_line(tmp_path):
+    rc, out, err = run(tmp_path, 1, "6e10")
+    assert rc == 0, out + err
+    assert results(tmp_path).splitlines()[-1] == "8B77D2DA053842DD"
+
+
+@pytest.mark.gpu
+def test_host_resume_gives_the_same_answer(tmp_path):
+    """Kill-and-resume equivalent: a checkpoint in the middle of T3 carries the partial
+    checksum; the resumed run must end with the same results file tail."""
+    lo, hi = 2276300000000000, 2276360000000000
+    mid = lo + 3 * (1 << 32)
+    with wwlib.Context(0, list_capacity=wwlib.list_bound(1 << 32, 1 << 33)) as ctx:
+        part = ctx.run_range(lo, mid, 1 << 32)
+    write_state(tmp_path, "stateA-WW.txt", lo, hi, mid, part.checksum, 1.0, 1)
+    (tmp_path / "results-WW.txt").write_text("")
+    rc, out, err = run(tmp_path, "227630e10", "227636e10")
+    assert rc == 0, out + err
+    assert "Resuming search from checkpoint. Current: %d" % mid in out
+    assert results(tmp_path).splitlines()[-1] == "6522CBC4CA2E4CFF"
+    st = (tmp_path / "stateB-WW.txt").read_text().split()
+    assert st[:3] == [str(lo), str(hi), str(hi)]

@@ -16,9 +16,13 @@
 
 #include "../../include/ww_api.h"
 #include "ww_arith.h"
+#include "ww_batch.h"
 #include "ww_kernels.h"
 
+using ww::BatchSpan;
+using ww::BatchState;
 using ww::DevState;
+using ww::SlotSum;
 
 struct ww_ctx {
     int device = 0;
@@ -29,6 +33,7 @@ struct ww_ctx {
     ww_result *d_results = nullptr;
     uint32_t result_cap = 0;
     int test_grid = 0;
+    int cus = 0;
     uint64_t default_chunk = 0;
     /* event pairs of launches whose durations have not been collected yet */
     std::vector<hipEvent_t> ev_sieve, ev_tests;
@@ -159,7 +164,11 @@ int ww_open(int device, uint64_t list_capacity, uint32_t result_capacity, ww_ctx
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) != hipSuccess)
         return bail(fail(WW_E_HIP, "hipGetDeviceProperties"));
-    c->test_grid = prop.multiProcessorCount * 8;
+    c->cus = prop.multiProcessorCount;
+    {
+        int nb = ww::tests_blocks_per_cu();
+        c->test_grid = prop.multiProcessorCount * (nb > 0 ? nb : 4);
+    }
     c->default_chunk = 1ull << 32;
     hipError_t e;
     if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess)
@@ -386,6 +395,196 @@ int ww_prp_flags(ww_ctx *c, const uint64_t *n, uint64_t count, uint8_t *out)
     (void)hipFree(dn);
     if (dout) (void)hipFree(dout);
     if (e != hipSuccess) return fail(WW_E_HIP, "ww_prp_flags: %s", hipGetErrorString(e));
+    return WW_OK;
+}
+
+} // extern "C"
+
+/* ---------------------------------------------------------------------------
+ * Persistent form
+ * ------------------------------------------------------------------------- */
+
+struct ww_batch {
+    int device = 0;
+    uint32_t n = 0;
+    uint32_t res_cap = 0;
+    int blocks_per_cu = 0;
+    BatchSpan *d_spans = nullptr;
+    uint64_t *d_prefix = nullptr;
+    SlotSum *d_sums = nullptr;
+    BatchState *d_state = nullptr;
+    ww_result *d_res = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int graph_grid = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+namespace {
+
+int batch_grid(ww_ctx *c, ww_batch *b)
+{
+    int nb = b->blocks_per_cu > 0 ? b->blocks_per_cu : ww::batch_blocks_per_cu();
+    if (nb <= 0) nb = 4;
+    return c->cus * nb;
+}
+
+int batch_enqueue(ww_ctx *c, ww_batch *b, int grid)
+{
+    HIP_TRY(ww::launch_batch(b->d_spans, b->d_prefix, b->n, b->d_state, b->d_sums, b->n, b->d_res,
+                             b->res_cap, grid, c->stream));
+    return WW_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int ww_batch_create(ww_ctx *c, const ww_range *ranges, uint32_t n, uint32_t result_capacity,
+                    ww_batch **out)
+{
+    int rc = use(c);
+    if (rc) return rc;
+    if (!out || !ranges || n == 0 || result_capacity == 0) return fail(WW_E_ARG, "bad batch arguments");
+    *out = nullptr;
+    std::vector<BatchSpan> spans(n);
+    std::vector<uint64_t> prefix(n + 1, 0);
+    for (uint32_t i = 0; i < n; ++i) {
+        const ww_range &r = ranges[i];
+        if (r.lo < WW_MIN_START || r.hi <= r.lo)
+            return fail(WW_E_ARG, "range %u [%llu, %llu) is empty or starts below %llu", i,
+                        (unsigned long long)r.lo, (unsigned long long)r.hi, WW_MIN_START);
+        spans[i].lo = r.lo;
+        spans[i].hi = r.hi;
+        spans[i].seg0 = 0;
+        spans[i].slot = i;
+        spans[i].pad = 0;
+        prefix[i + 1] = prefix[i] + ww::segments_of(r.lo, r.hi);
+    }
+    ww_batch *b = new ww_batch;
+    b->device = c->device;
+    b->n = n;
+    b->res_cap = result_capacity;
+    hipError_t e = hipMalloc(&b->d_spans, n * sizeof(BatchSpan));
+    if (e == hipSuccess) e = hipMalloc(&b->d_prefix, (n + 1) * sizeof(uint64_t));
+    if (e == hipSuccess) e = hipMalloc(&b->d_sums, n * sizeof(SlotSum));
+    if (e == hipSuccess) e = hipMalloc(&b->d_state, sizeof(BatchState));
+    if (e == hipSuccess) e = hipMalloc(&b->d_res, (size_t)result_capacity * sizeof(ww_result));
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(b->d_spans, spans.data(), n * sizeof(BatchSpan), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(b->d_prefix, prefix.data(), (n + 1) * sizeof(uint64_t), hipMemcpyHostToDevice,
+                           c->stream);
+    if (e == hipSuccess) e = hipEventCreate(&b->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&b->ev1);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+        ww_batch_destroy(b);
+        return fail(WW_E_HIP, "ww_batch_create: %s", hipGetErrorString(e));
+    }
+    *out = b;
+    return WW_OK;
+}
+
+int ww_batch_set_grid(ww_batch *b, int blocks_per_cu)
+{
+    if (!b || blocks_per_cu < 0) return fail(WW_E_ARG, "bad grid");
+    b->blocks_per_cu = blocks_per_cu;
+    return WW_OK;
+}
+
+int ww_batch_launch(ww_ctx *c, ww_batch *b)
+{
+    int rc = use(c);
+    if (rc) return rc;
+    if (!b) return fail(WW_E_ARG, "null batch");
+    HIP_TRY(hipEventRecord(b->ev0, c->stream));
+    if ((rc = batch_enqueue(c, b, batch_grid(c, b)))) return rc;
+    HIP_TRY(hipEventRecord(b->ev1, c->stream));
+    return WW_OK;
+}
+
+int ww_batch_launch_graph(ww_ctx *c, ww_batch *b)
+{
+    int rc = use(c);
+    if (rc) return rc;
+    if (!b) return fail(WW_E_ARG, "null batch");
+    int grid = batch_grid(c, b);
+    if (b->exec && b->graph_grid != grid) {
+        (void)hipGraphExecDestroy(b->exec);
+        (void)hipGraphDestroy(b->graph);
+        b->exec = nullptr;
+        b->graph = nullptr;
+    }
+    if (!b->exec) {
+        HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        rc = batch_enqueue(c, b, grid);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(c->stream, &g);
+        if (rc) return rc;
+        if (e != hipSuccess) return fail(WW_E_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+        b->graph = g;
+        HIP_TRY(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+        b->graph_grid = grid;
+    }
+    HIP_TRY(hipEventRecord(b->ev0, c->stream));
+    HIP_TRY(hipGraphLaunch(b->exec, c->stream));
+    HIP_TRY(hipEventRecord(b->ev1, c->stream));
+    return WW_OK;
+}
+
+int ww_batch_read(ww_ctx *c, ww_batch *b, ww_range_sum *sums, ww_result *recs, uint32_t cap,
+                  uint32_t *nrec, uint32_t *total)
+{
+    int rc = use(c);
+    if (rc) return rc;
+    if (!b) return fail(WW_E_ARG, "null batch");
+    BatchState st;
+    HIP_TRY(hipMemcpyAsync(&st, b->d_state, sizeof st, hipMemcpyDeviceToHost, c->stream));
+    if (sums) {
+        static_assert(sizeof(ww_range_sum) == sizeof(SlotSum), "layout");
+        HIP_TRY(hipMemcpyAsync(sums, b->d_sums, b->n * sizeof(SlotSum), hipMemcpyDeviceToHost, c->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    uint32_t avail = st.resultcount < b->res_cap ? st.resultcount : b->res_cap;
+    uint32_t m = recs ? (avail < cap ? avail : cap) : 0;
+    if (m) {
+        HIP_TRY(hipMemcpyAsync(recs, b->d_res, (size_t)m * sizeof(ww_result), hipMemcpyDeviceToHost,
+                               c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+    }
+    if (nrec) *nrec = m;
+    if (total) *total = st.resultcount;
+    return WW_OK;
+}
+
+int ww_batch_time(ww_ctx *c, ww_batch *b, double *ms)
+{
+    int rc = use(c);
+    if (rc) return rc;
+    if (!b || !ms) return fail(WW_E_ARG, "null argument");
+    float t = 0;
+    HIP_TRY(hipEventSynchronize(b->ev1));
+    HIP_TRY(hipEventElapsedTime(&t, b->ev0, b->ev1));
+    *ms = t;
+    return WW_OK;
+}
+
+int ww_batch_destroy(ww_batch *b)
+{
+    if (!b) return WW_OK;
+    (void)hipSetDevice(b->device);
+    (void)hipDeviceSynchronize();
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->ev1) (void)hipEventDestroy(b->ev1);
+    if (b->d_res) (void)hipFree(b->d_res);
+    if (b->d_state) (void)hipFree(b->d_state);
+    if (b->d_sums) (void)hipFree(b->d_sums);
+    if (b->d_prefix) (void)hipFree(b->d_prefix);
+    if (b->d_spans) (void)hipFree(b->d_spans);
+    delete b;
     return WW_OK;
 }
 

@@ -1,0 +1,231 @@
+/*
+ * ww_batch.hip -- the persistent form: one launch searches many ranges.
+ *
+ * Work unit: a segment of 8192 consecutive integers (64 lanes x one 64-bit word of odd
+ * numbers each) of one range. The ranges' segments are laid end to end ("spans", with a
+ * prefix sum of their segment counts) and every wave claims the next segment from one
+ * global cursor (work stealing), so uneven ranges balance by construction. Per segment a
+ * wave sieves by 3..113 in registers, compacts candidates into LDS, runs the base-2
+ * strong PRP 64 candidates at a time and queues the survivors (p, range slot) in LDS;
+ * whenever 64 are queued it runs the Wieferich + Wall-Sun-Sun tests on them, one per
+ * lane, so the test lanes stay full across segment and range boundaries. Checksums and
+ * tested counts accumulate per range slot (a wave keeps a running sum while its slot
+ * does not change); result records carry the slot in their pad field.
+ *
+ * Replaces the reference's per-chunk launch sequence (main.cpp:1683-1741) for many
+ * ranges at once; the arithmetic and classification are those of the plain kernels.
+ */
+#include "ww_batch.h"
+#include "ww_device.h"
+
+namespace ww {
+
+namespace {
+
+constexpr int kBatchWaves = 4;
+constexpr int kQueueCap = 128; /* < 64 left over + <= 64 appended per PRP round */
+
+/* Wave-uniform values as scalars (the compiler cannot see that a lane-0 broadcast is
+   uniform, and would keep them in vector registers). */
+__device__ __forceinline__ u32 uniform32(u32 v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ u64 uniform64(u64 v)
+{
+    return ((u64)uniform32((u32)(v >> 32)) << 32) | uniform32((u32)v);
+}
+
+__device__ __forceinline__ void wave_sync_lds()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+struct WaveAcc {
+    u32 slot;
+    u64 cks, cnt;
+};
+
+__device__ __forceinline__ void acc_flush(WaveAcc &a, SlotSum *sums, u32 lane)
+{
+    if (a.slot != 0xffffffffu && lane == 0 && a.cnt) {
+        atomicAdd((unsigned long long *)&sums[a.slot].checksum, (unsigned long long)a.cks);
+        atomicAdd((unsigned long long *)&sums[a.slot].count, (unsigned long long)a.cnt);
+    }
+    a.cks = 0;
+    a.cnt = 0;
+}
+
+/* Tests the first n (1..64) queued entries, one per lane, and drops them from the queue. */
+__device__ __forceinline__ void tests_round(u32 n, u32 &qn, u64 *wq, u32 *ws, u32 lane,
+                                            WaveAcc &acc, SlotSum *sums, ww_result *res,
+                                            u32 res_cap, u32 *rcount)
+{
+    const bool act = lane < n;
+    const u64 p = act ? wq[lane] : 0;
+    const u32 slot = act ? ws[lane] : 0;
+    const u32 rest = qn - n;
+    u64 pr = 0;
+    u32 sr = 0;
+    if (lane < rest) {
+        pr = wq[n + lane];
+        sr = ws[n + lane];
+    }
+    wave_sync_lds();
+    if (lane < rest) {
+        wq[lane] = pr;
+        ws[lane] = sr;
+    }
+    wave_sync_lds();
+    qn = rest;
+
+    u64 v = 0;
+    if (act) v = test_prime(p, res, res_cap, rcount, (int32_t)slot);
+
+    const u32 slot0 = uniform32(slot);
+    if (__ballot(act && slot != slot0) == 0) {
+        u64 sum = v;
+        for (int d = 32; d > 0; d >>= 1) sum += __shfl_xor(sum, d, 64);
+        sum = uniform64(sum);
+        if (slot0 != acc.slot) {
+            acc_flush(acc, sums, lane);
+            acc.slot = slot0;
+        }
+        acc.cks += sum;
+        acc.cnt += n;
+    } else if (act) {
+        atomicAdd((unsigned long long *)&sums[slot].checksum, (unsigned long long)v);
+        atomicAdd((unsigned long long *)&sums[slot].count, 1ull);
+    }
+}
+
+__global__ __launch_bounds__(64 * kBatchWaves) __attribute__((amdgpu_waves_per_eu(6))) void k_batch(const BatchSpan *spans,
+                                                             const u64 *prefix, u32 nspans,
+                                                             BatchState *st, SlotSum *sums,
+                                                             ww_result *res, u32 res_cap)
+{
+    __shared__ uint16_t cand[kBatchWaves][kWaveCandCap];
+    __shared__ u64 qp[kBatchWaves][kQueueCap];
+    __shared__ u32 qs[kBatchWaves][kQueueCap];
+
+    const u32 lane = threadIdx.x & 63;
+    const u32 wave = threadIdx.x >> 6;
+    uint16_t *wc = cand[wave];
+    u64 *wq = qp[wave];
+    u32 *ws = qs[wave];
+    const u64 total = prefix[nspans];
+
+    u32 qn = 0;
+    WaveAcc acc;
+    acc.slot = 0xffffffffu;
+    acc.cks = acc.cnt = 0;
+
+    /* One wave-uniform state machine with a single site per phase (keeps each phase's
+       registers to itself): drain a full test round, else run the next PRP round of the
+       current segment, else claim the next segment. */
+    bool done = false;
+    u32 j0 = 0, ncand = 0, slot = 0;
+    u64 group = 0;
+    for (;;) {
+        if (qn >= 64 || (done && qn > 0)) {
+            tests_round(qn >= 64 ? 64 : qn, qn, wq, ws, lane, acc, sums, res, res_cap,
+                        &st->resultcount);
+            continue;
+        }
+        if (done) break;
+        if (j0 < ncand) {
+            /* strong PRP, 64 candidates per round; survivors join the test queue */
+            const u32 j = j0 + lane;
+            const bool in = j < ncand;
+            u64 n = 0;
+            bool prp = false;
+            if (in) {
+                n = group + 2ull * wc[j];
+                prp = spsp2(n, inv64(n), mont_one(n));
+            }
+            const u64 bal = __ballot(prp);
+            if (prp) {
+                const u32 at = qn + (u32)__popcll(bal & ((1ull << lane) - 1));
+                wq[at] = n;
+                ws[at] = slot;
+            }
+            qn += (u32)__popcll(bal);
+            j0 += 64;
+            wave_sync_lds();
+            continue;
+        }
+        u64 s = 0;
+        if (lane == 0) s = atomicAdd((unsigned long long *)&st->cursor, 1ull);
+        s = uniform64(__shfl(s, 0, 64));
+        if (s >= total) {
+            done = true;
+            continue;
+        }
+        /* span holding segment s: largest a with prefix[a] <= s */
+        u32 a = 0, b = nspans;
+        while (b - a > 1) {
+            u32 m = (a + b) >> 1;
+            if (prefix[m] <= s) a = m;
+            else b = m;
+        }
+        const BatchSpan sp = spans[a];
+        slot = sp.slot;
+        group = (sp.lo | 1) + (sp.seg0 + (s - prefix[a])) * kIntsPerWave;
+
+        /* sieve, then compact candidate positions into LDS */
+        u64 bits = sieve_word(group, lane * (u32)kIntsPerWord, sp.hi);
+        const u32 cnt = __popcll(bits);
+        u32 incl = cnt;
+        for (int d = 1; d < 64; d <<= 1) {
+            u32 v = __shfl_up(incl, d, 64);
+            if (lane >= (u32)d) incl += v;
+        }
+        ncand = uniform32(__shfl(incl, 63, 64));
+        j0 = 0;
+        u32 pos = incl - cnt;
+        while (bits) {
+            u32 bit = __builtin_ctzll(bits);
+            wc[pos++] = (uint16_t)(lane * 64 + bit);
+            bits &= bits - 1;
+        }
+        wave_sync_lds();
+    }
+    acc_flush(acc, sums, lane);
+}
+
+__global__ void k_batch_clear(BatchState *st, SlotSum *sums, u32 nslots)
+{
+    u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        st->cursor = 0;
+        st->resultcount = 0;
+        st->reserved = 0;
+    }
+    if (i < nslots) {
+        sums[i].checksum = 0;
+        sums[i].count = 0;
+    }
+}
+
+} // namespace
+
+int batch_blocks_per_cu()
+{
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_batch, 64 * kBatchWaves, 0) != hipSuccess)
+        return 0;
+    return nb;
+}
+
+hipError_t launch_batch(const BatchSpan *spans, const u64 *prefix, u32 nspans, BatchState *st,
+                        SlotSum *sums, u32 nslots, ww_result *res, u32 res_cap, int grid,
+                        hipStream_t s)
+{
+    hipLaunchKernelGGL(k_batch_clear, dim3((nslots + 255) / 256), dim3(256), 0, s, st, sums, nslots);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_batch, dim3(grid), dim3(64 * kBatchWaves), 0, s, spans, prefix, nspans, st,
+                       sums, res, res_cap);
+    return hipGetLastError();
+}
+
+} // namespace ww

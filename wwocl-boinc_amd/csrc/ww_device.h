@@ -1,0 +1,127 @@
+/*
+ * ww_device.h -- device pieces shared by the plain and the persistent kernels: the
+ * small-prime word sieve, result classification and record append.
+ */
+#ifndef WW_DEVICE_H
+#define WW_DEVICE_H
+
+#include "ww_arith.h"
+#include "ww_kernels.h"
+
+namespace ww {
+
+/* Bits 0, L, 2L, ... of a 64-bit word: one lane's word holds odd numbers
+   w0, w0+2, ..., w0+126, so multiples of L recur every L bits. */
+__device__ constexpr u64 stride_pattern(u32 L)
+{
+    u64 m = 0;
+    for (u32 i = 0; i < 64; i += L) m |= 1ull << i;
+    return m;
+}
+
+/* Marks the bits of `comp` whose odd number w0 + 2i is divisible by L, given
+   base_mod = (word group base) mod L and lane_off = w0 - (word group base) < 2^32. */
+template <u32 L>
+__device__ __forceinline__ void sieve_prime(u64 &comp, u32 base_mod, u32 lane_off)
+{
+    constexpr u32 inv2 = (L + 1) / 2; /* 2^-1 mod L */
+    constexpr u64 pat = stride_pattern(L);
+    u32 r = (base_mod + lane_off % L) % L;  /* w0 mod L */
+    u32 i0 = ((L - r) * inv2) % L;          /* first i with L | w0 + 2i */
+    if (L < 64) {
+        comp |= pat << i0;
+    } else {
+        comp |= (i0 < 64) ? (1ull << i0) : 0ull;
+    }
+}
+
+/* The odd primes 3..113: together with oddness, the candidate filter of the reference's
+   wheel + bit tables (getsegprimes.cl:167-278) and of its CPU loop (main.cpp:653-682). */
+#define WW_SIEVE_PRIMES(X)                                                            \
+    X(3) X(5) X(7) X(11) X(13) X(17) X(19) X(23) X(29) X(31) X(37) X(41) X(43) X(47) \
+    X(53) X(59) X(61) X(67) X(71) X(73) X(79) X(83) X(89) X(97) X(101) X(103) X(107) \
+    X(109) X(113)
+
+/* Candidate bits of the word of odd numbers w0 + 2i (i < 64), w0 = group + lane_off:
+   no factor in 3..113, and w0 + 2i < hi. Words that wrapped past 2^64 (w0 < group)
+   yield nothing. `group` is uniform across the wave, so its residues are scalar work. */
+__device__ __forceinline__ u64 sieve_word(u64 group, u32 lane_off, u64 hi)
+{
+    /* An empty asm makes lane_off opaque, so a persistent caller recomputes the per-lane
+       residues lane_off % L (a few instructions each) instead of hoisting all 29 of them
+       out of its loop into live registers. */
+    asm volatile("" : "+v"(lane_off));
+    const u64 w0 = group + lane_off;
+    u64 comp = 0;
+#define WW_SIEVE_ONE(L) sieve_prime<L>(comp, (u32)(group % L), lane_off);
+    WW_SIEVE_PRIMES(WW_SIEVE_ONE)
+#undef WW_SIEVE_ONE
+    u64 valid = 0;
+    if (w0 >= group && w0 < hi) {
+        u64 left = hi - w0;       /* >= 1 */
+        u64 nv = (left + 1) >> 1; /* odd numbers in [w0, hi) */
+        valid = nv >= 64 ? ~0ull : ((1ull << nv) - 1);
+    }
+    return ~comp & valid;
+}
+
+/* Append one record; `tag` goes to the record's pad field (the range slot in the
+   persistent form, 0 in the plain form). */
+__device__ __forceinline__ void put_result(ww_result *res, u32 res_cap, u32 *count, u64 p,
+                                           int32_t code, int32_t rem, int32_t quot, int32_t tag)
+{
+    u32 ix = atomicAdd(count, 1u);
+    if (ix < res_cap) {
+        ww_result r;
+        r.p = p;
+        r.result = code;
+        r.rem = rem;
+        r.quot = quot;
+        r.pad = tag;
+        res[ix] = r;
+    }
+}
+
+/* Wieferich classification of wieferich.cl:247-293 (unsigned 64-bit comparisons). */
+__device__ __forceinline__ void classify_w(u64 p, u64 s0, u64 s1, ww_result *res, u32 cap,
+                                           u32 *count, int32_t tag)
+{
+    const u64 T = 1000; /* SPECIAL_THRESHOLD, WW.h:9 */
+    if ((s1 == 0 && s0 == 1) || (s1 == p - 1 && s0 == p - 1)) {
+        put_result(res, cap, count, p, 1093, 1, 0, tag);
+    } else if (s1 <= T) {
+        if (s0 == 1) put_result(res, cap, count, p, 1093, 1, (int32_t)s1, tag);
+        else if (s0 == p - 1) put_result(res, cap, count, p, 1093, -1, (int32_t)(s1 + 1), tag);
+    } else if (s1 >= p - T) {
+        if (s0 == 1) put_result(res, cap, count, p, 1093, 1, (int32_t)(u32)(s1 - p), tag);
+        else if (s0 == p - 1)
+            put_result(res, cap, count, p, 1093, -1, (int32_t)(u32)(s1 + 1 - p), tag);
+    }
+}
+
+/* Wall-Sun-Sun classification of wallsunsun.cl:325-349. */
+__device__ __forceinline__ void classify_wss(u64 p, u64 c21, ww_result *res, u32 cap,
+                                             u32 *count, int32_t tag)
+{
+    const u64 T = 1000;
+    if (c21 == 0) put_result(res, cap, count, p, 1, 0, 0, tag);
+    else if (c21 <= T) put_result(res, cap, count, p, 2, 0, (int32_t)c21, tag);
+    else if (c21 >= p - T) put_result(res, cap, count, p, 2, 0, (int32_t)(u32)(c21 - p), tag);
+}
+
+/* Both tests on one list entry; returns the entry's checksum contribution
+   low32(s1) + low32(c21) (wieferich.cl:315, wallsunsun.cl:352). */
+__device__ __forceinline__ u64 test_prime(u64 p, ww_result *res, u32 cap, u32 *count, int32_t tag)
+{
+    const u64 q = inv64(p);
+    const m2 one = m2_one(p);
+    const m2 s = wieferich_residue(p, q, one);
+    const m2 f = wallsunsun_residue(p, q, one);
+    classify_w(p, s.d0, s.d1, res, cap, count, tag);
+    classify_wss(p, f.d1, res, cap, count, tag);
+    return (s.d1 & 0xffffffffu) + (f.d1 & 0xffffffffu);
+}
+
+} // namespace ww
+
+#endif

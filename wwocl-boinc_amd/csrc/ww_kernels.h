@@ -40,6 +40,7 @@ hipError_t launch_sieve_prp(uint64_t lo, uint64_t hi, uint64_t *list, uint64_t c
                             DevState *st, hipStream_t s);
 hipError_t launch_ww_tests(const uint64_t *list, uint64_t cap, DevState *st,
                            ww_result *res, uint32_t res_cap, int grid, hipStream_t s);
+int tests_blocks_per_cu();
 hipError_t launch_residues(const uint64_t *p, uint64_t n, uint64_t *out, hipStream_t s);
 hipError_t launch_prp_flags(const uint64_t *n, uint64_t count, uint8_t *out, hipStream_t s);
 

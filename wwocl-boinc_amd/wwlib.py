@@ -43,6 +43,14 @@ class Counters(ctypes.Structure):
                 ("primecount_max", U32), ("resultcount", U32), ("reserved", U32)]
 
 
+class Range(ctypes.Structure):
+    _fields_ = [("lo", U64), ("hi", U64)]
+
+
+class RangeSum(ctypes.Structure):
+    _fields_ = [("checksum", U64), ("prp_count", U64)]
+
+
 class DeviceInfo(ctypes.Structure):
     _fields_ = [("name", ctypes.c_char * 256), ("arch", ctypes.c_char * 64),
                 ("compute_units", ctypes.c_int), ("clock_khz", ctypes.c_int),
@@ -80,6 +88,17 @@ _SIGNATURES = [
                                    ctypes.POINTER(U64)]),
     ("ww_prp_flags", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(U64), U64,
                                     ctypes.POINTER(ctypes.c_uint8)]),
+    ("ww_batch_create", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Range), U32, U32,
+                                       ctypes.POINTER(ctypes.c_void_p)]),
+    ("ww_batch_set_grid", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    ("ww_batch_launch", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    ("ww_batch_launch_graph", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    ("ww_batch_read", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(RangeSum),
+                                     ctypes.POINTER(Result), U32, ctypes.POINTER(U32),
+                                     ctypes.POINTER(U32)]),
+    ("ww_batch_time", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p,
+                                     ctypes.POINTER(ctypes.c_double)]),
+    ("ww_batch_destroy", ctypes.c_int, [ctypes.c_void_p]),
     ("ww_peak_mul32", ctypes.c_int, [ctypes.c_int, ctypes.c_int,
                                      ctypes.POINTER(ctypes.c_double)]),
     ("ww_copy_probe", ctypes.c_int, [ctypes.c_int, U64, ctypes.POINTER(ctypes.c_double)]),
@@ -274,3 +293,113 @@ class Context:
         if c.resultcount > self.result_capacity:
             raise WWError(WW_E_CAPACITY, f"{c.resultcount} results overflowed the buffer")
         return RangeResult(c.totalprimecount, c.checksum, sort_results(self.results()))
+
+
+class Batch:
+    """Many ranges searched by one persistent launch (ww_batch_*)."""
+
+    def __init__(self, ctx: Context, ranges, result_capacity: int = 1 << 16):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        self.ranges = [(int(lo), int(hi)) for lo, hi in ranges]
+        self.n = len(self.ranges)
+        arr = (Range * self.n)(*[Range(lo, hi) for lo, hi in self.ranges])
+        self.h = ctypes.c_void_p()
+        self.result_capacity = result_capacity
+        _check(self.lib.ww_batch_create(ctx.h, arr, self.n, result_capacity, ctypes.byref(self.h)))
+
+    def set_grid(self, blocks_per_cu: int):
+        _check(self.lib.ww_batch_set_grid(self.h, blocks_per_cu))
+
+    def launch(self, graph: bool = False):
+        fn = self.lib.ww_batch_launch_graph if graph else self.lib.ww_batch_launch
+        _check(fn(self.ctx.h, self.h))
+
+    def time_ms(self) -> float:
+        v = ctypes.c_double()
+        _check(self.lib.ww_batch_time(self.ctx.h, self.h, ctypes.byref(v)))
+        return v.value
+
+    def read(self):
+        """[(prp_count, checksum, sorted records)] per range, in table order."""
+        sums = (RangeSum * self.n)()
+        recs = (Result * self.result_capacity)()
+        nrec, total = U32(), U32()
+        _check(self.lib.ww_batch_read(self.ctx.h, self.h, sums, recs, self.result_capacity,
+                                      ctypes.byref(nrec), ctypes.byref(total)))
+        if total.value > self.result_capacity:
+            raise WWError(WW_E_CAPACITY, f"{total.value} records overflowed the buffer")
+        per = [[] for _ in range(self.n)]
+        for i in range(nrec.value):
+            r = recs[i]
+            per[r.pad].append(r.key())
+        return [(sums[i].prp_count, sums[i].checksum, sort_results(per[i]))
+                for i in range(self.n)]
+
+    def close(self):
+        if self.h:
+            self.lib.ww_batch_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------------------------
+# Synthetic workloads. Neither the reference nor the configuration fixes a random
+# generator, so these use splitmix64 (Steele, Lea & Flood 2014) with the stated seeds.
+# ---------------------------------------------------------------------------------
+
+class SplitMix64:
+    def __init__(self, seed: int):
+        self.x = seed & (2**64 - 1)
+
+    def next(self) -> int:
+        self.x = (self.x + 0x9E3779B97F4A7C15) & (2**64 - 1)
+        z = self.x
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & (2**64 - 1)
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & (2**64 - 1)
+        return z ^ (z >> 31)
+
+    def unit(self) -> float:
+        return (self.next() >> 11) / float(1 << 53)
+
+
+def batch_workload(n: int = 4096, seed: int = 0, wmin: int = 16, wmax: int = 28,
+                   smin: int = 1 << 40):
+    """n ranges: widths floor(2^(wmin + (wmax-wmin) u)) (log-uniform), starts uniform in
+    [smin, 2^64 - 2^wmax] and forced odd."""
+    rng = SplitMix64(seed)
+    smax = (1 << 64) - (1 << wmax)
+    out = []
+    for _ in range(n):
+        w = int(2.0 ** (wmin + (wmax - wmin) * rng.unit()))
+        s = (smin + rng.next() % (smax - smin + 1)) | 1
+        out.append((s, s + w))
+    return out
+
+
+def serving_workload(rate: float, duration_s: float, seed: int = 1, wmin: int = 20,
+                     wmax: int = 30, smin: int = 1 << 32):
+    """Poisson arrivals at `rate` per second for `duration_s`: [(t_arrival, lo, hi)],
+    widths log-uniform in [2^wmin, 2^wmax], starts uniform in [smin, 2^64 - 2^wmax], odd."""
+    import math
+    rng = SplitMix64(seed)
+    smax = (1 << 64) - (1 << wmax)
+    t, out = 0.0, []
+    while True:
+        t += -math.log(1.0 - rng.unit()) / rate
+        if t >= duration_s:
+            return out
+        w = int(2.0 ** (wmin + (wmax - wmin) * rng.unit()))
+        s = (smin + rng.next() % (smax - smin + 1)) | 1
+        out.append((t, s, s + w))
