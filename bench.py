@@ -27,6 +27,10 @@ BITS = 64  # every p in the range has bit length 64
 # launch (factors from a 1 GiB copy kernel) over the entries of the launch, measured with
 # rocprofv3 --pmc on MI355X for this workload. Algorithmic: the 8-byte list read.
 TRAFFIC_BYTES_PER_ENTRY = 8.00
+# k_batch keeps candidates and PRPs on chip (LDS queues); its HBM traffic per tested
+# prime is the span table lookups and the cursor/slot atomics only (each atomic reads as a
+# 32-byte write), measured the same way: 0.3 MB fetched + 34 MB written per step.
+BATCH_TRAFFIC_BYTES_PER_PRP = 0.175
 
 
 def mul32_path(b: int) -> int:
@@ -62,12 +66,100 @@ def cpu_baseline():
             "sample": f"16 slices [2^63 + k*2^29, +2^22), k=0..15: {total} PRPs in {dt:.2f} s"}
 
 
+def run_plain(ctx, steps, warmup):
+    """Plain form: per 2^32 chunk, clear the list, sieve + PRP, then the test kernel."""
+    step_checksum = None
+    for _ in range(warmup):
+        ctx.clear_results()
+        ctx.search_range(LO, HI, CHUNK)
+        ctx.finish()
+        step_checksum = ctx.counters().checksum
+    ctx.kernel_times()  # discard warmup kernel times
+    ctx.clear_results()
+    ctx.finish()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.search_range(LO, HI, CHUNK)
+    ctx.finish()
+    elapsed = time.perf_counter() - t0
+    ms_sieve, ms_tests, launches = ctx.kernel_times()
+    c = ctx.counters()
+    tested = c.totalprimecount
+    per_launch = tested / launches
+    avg_tests_s = ms_tests / launches * 1e-3
+    return {"value": tested / elapsed, "ms_per_step": elapsed / steps * 1e3,
+            "prps_per_step": tested // steps,
+            "step_checksum": None if step_checksum is None else f"{step_checksum:016X}",
+            "kernel_ms_per_step": {"k_sieve_prp": ms_sieve / steps, "k_ww_tests": ms_tests / steps},
+            "tests_kernel": {"avg_launch_ms": avg_tests_s * 1e3, "launches": launches,
+                             "mul32_per_s": per_launch * mul32_tests(BITS) / avg_tests_s,
+                             "entries_per_launch": per_launch}}
+
+
+def run_persistent(ctx, steps, warmup):
+    """Persistent form: the whole range in one graph-launched k_batch."""
+    with wwlib.Batch(ctx, [(LO, HI)]) as b:
+        res = None
+        for _ in range(max(warmup, 1)):
+            b.launch(graph=True)
+            res = b.read()[0]
+        ctx.finish()
+        t0 = time.perf_counter()
+        kms = 0.0
+        for _ in range(steps):
+            b.launch(graph=True)
+            kms += b.time_ms()
+        ctx.finish()
+        elapsed = time.perf_counter() - t0
+        last = b.read()[0]
+    assert last[:2] == res[:2], "persistent form is not deterministic"
+    prps, cks = res[0], res[1]
+    return {"value": prps * steps / elapsed, "ms_per_step": elapsed / steps * 1e3,
+            "prps_per_step": prps, "step_checksum": f"{cks:016X}",
+            "kernel_ms_per_step": kms / steps}
+
+
+def run_batched_workload(ctx):
+    """The 4096-range workload (seed 0): one persistent launch vs plain per-range launches
+    (clear, chunk loop, read back), both on the same context."""
+    ranges = wwlib.batch_workload()
+    with wwlib.Batch(ctx, ranges) as b:
+        b.launch(graph=True)
+        ref = b.read()
+        ctx.finish()
+        t0 = time.perf_counter()
+        b.launch(graph=True)
+        kms = b.time_ms()
+        got = b.read()
+        t_batch = time.perf_counter() - t0
+    assert got == ref
+    prps = sum(g[0] for g in got)
+    t0 = time.perf_counter()
+    plain_prps = 0
+    for (lo, hi), g in zip(ranges, got):
+        ctx.clear_results()
+        ctx.clear_prp_count()
+        ctx.search_range(lo, hi, CHUNK)
+        c = ctx.counters()
+        plain_prps += c.totalprimecount
+        assert (c.totalprimecount, c.checksum) == (g[0], g[1]), (lo, hi)
+    t_plain = time.perf_counter() - t0
+    return {"ranges": len(ranges), "prps": prps,
+            "persistent": {"value": prps / t_batch, "seconds": t_batch, "kernel_ms": kms},
+            "plain_per_range": {"value": plain_prps / t_plain, "seconds": t_plain},
+            "speedup": t_plain / t_batch, "per_range_checksums_equal": True}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 4096-range workload")
+    ap.add_argument("--form", choices=["both", "persistent", "plain"], default="both")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the autotuner (defaults: deep bound 4096, full occupancy)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="also launch one plain copy kernel of BYTES (counter calibration)")
     args = ap.parse_args()
@@ -80,64 +172,74 @@ def main():
     peak = wwlib.peak_mul32(0, 0)
     cap = wwlib.list_bound(LO, LO + CHUNK)
     copy_ms = wwlib.copy_probe(args.copy_probe) if args.copy_probe else None
-    step_checksum = None
+    plain = pers = batched = tuning = None
     with wwlib.Context(0, list_capacity=cap) as ctx:
-        for _ in range(args.warmup):
-            ctx.clear_results()
-            ctx.search_range(LO, HI, CHUNK)
-            ctx.finish()
-            step_checksum = ctx.counters().checksum
-        ctx.kernel_times()  # discard warmup kernel times
-        ctx.clear_results()
-        ctx.finish()
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            ctx.search_range(LO, HI, CHUNK)
-        ctx.finish()
-        elapsed = time.perf_counter() - t0
-        ms_sieve, ms_tests, launches = ctx.kernel_times()
-        c = ctx.counters()
+        if not args.no_tune:
+            t0 = time.perf_counter()
+            best, trials = ctx.autotune(LO, LO + (1 << 31))
+            tuning = {"sample": "[2^63, 2^63+2^31)", "best": best, "trials": len(trials),
+                      "seconds": time.perf_counter() - t0,
+                      "by_deep_bound": {str(b): max(t["prps_per_s"] for t in trials
+                                                    if t["deep_bound"] == b)
+                                        for b in sorted({t["deep_bound"] for t in trials})}}
+        if args.form in ("both", "persistent"):
+            pers = run_persistent(ctx, args.steps, args.warmup)
+        if args.form in ("both", "plain"):
+            plain = run_plain(ctx, args.steps, args.warmup)
+        if not args.no_batch:
+            batched = run_batched_workload(ctx)
 
-    tested = c.totalprimecount
-    per_step = tested // args.steps
-    value = tested / elapsed
-    # dominant kernel: Wieferich + Wall-Sun-Sun; algorithmic MUL32 per launch over the
-    # average launch duration (events on the context's stream)
-    avg_tests_s = ms_tests / launches * 1e-3
-    per_launch = tested / launches
-    ach = per_launch * mul32_tests(BITS) / avg_tests_s
-    path_ach = tested * mul32_path(BITS) / elapsed
+    head = pers if pers is not None else plain
     out = {
-        "metric": "tested primes/s at p~2^63 (plain form, 1 GPU)",
-        "value": value,
+        "metric": "tested primes/s at p~2^63 (batched vs plain, 1 GPU)",
+        "value": head["value"],
         "unit": "tested primes/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": elapsed / args.steps * 1e3,
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "u64",
         "data": "synthetic: the number line [2^63, 2^63+2^33), generated on the device",
-        "config": {"workload": "plain form, range [2^63, 2^63+2^33) in 2^32-wide chunks",
-                   "range": [LO, HI], "chunk": CHUNK, "prps_per_step": per_step,
-                   "step_checksum": None if step_checksum is None else f"{step_checksum:016X}",
-                   "device": info.name.decode(),
+        "config": {"workload": ("persistent form (one graph-launched k_batch)" if pers else
+                                "plain form (2^32-wide chunks)") +
+                               ", range [2^63, 2^63+2^33)",
+                   "range": [LO, HI], "prps_per_step": head["prps_per_step"],
+                   "step_checksum": head["step_checksum"], "device": info.name.decode(),
                    "arch": info.arch.decode(), "cus": info.compute_units},
-        "roofline": {"bound": "valu", "kernel": "k_ww_tests",
-                     "achieved": ach / 1e12, "peak": peak / 1e12, "unit": "TMUL32/s",
-                     "frac": ach / peak,
-                     "traffic": per_launch * TRAFFIC_BYTES_PER_ENTRY,
-                     "traffic_algorithmic": per_launch * 8,
-                     "per_unit": f"{mul32_tests(BITS)} MUL32 per tested prime (89b-103, b=64)",
-                     "avg_launch_ms": avg_tests_s * 1e3, "launches": launches},
-        "roofline_path": {"achieved": path_ach / 1e12, "peak": peak / 1e12, "unit": "TMUL32/s",
-                          "frac": path_ach / peak,
-                          "per_unit": f"{mul32_path(BITS)} MUL32 per tested prime (99b-123)"},
-        "kernel_ms_per_step": {"k_sieve_prp": ms_sieve / args.steps,
-                               "k_ww_tests": ms_tests / args.steps},
     }
+    if pers is not None:
+        kern_s = pers["kernel_ms_per_step"] * 1e-3
+        ach = pers["prps_per_step"] * mul32_path(BITS) / kern_s
+        out["roofline"] = {
+            "bound": "valu", "kernel": "k_batch", "achieved": ach / 1e12, "peak": peak / 1e12,
+            "unit": "TMUL32/s", "frac": ach / peak,
+            "traffic": pers["prps_per_step"] * BATCH_TRAFFIC_BYTES_PER_PRP,
+            "traffic_algorithmic": 0,
+            "per_unit": f"{mul32_path(BITS)} MUL32 per tested prime (99b-123, b=64: sieve PRP "
+                        "+ Wieferich + Wall-Sun-Sun in one kernel)",
+            "avg_launch_ms": pers["kernel_ms_per_step"], "launches": args.steps}
+    if plain is not None:
+        tk = plain["tests_kernel"]
+        key = "roofline" if pers is None else "roofline_plain_tests_kernel"
+        out[key] = {
+            "bound": "valu", "kernel": "k_ww_tests", "achieved": tk["mul32_per_s"] / 1e12,
+            "peak": peak / 1e12, "unit": "TMUL32/s", "frac": tk["mul32_per_s"] / peak,
+            "traffic": tk["entries_per_launch"] * TRAFFIC_BYTES_PER_ENTRY,
+            "traffic_algorithmic": tk["entries_per_launch"] * 8,
+            "per_unit": f"{mul32_tests(BITS)} MUL32 per tested prime (89b-103, b=64)",
+            "avg_launch_ms": tk["avg_launch_ms"], "launches": tk["launches"]}
+        out["plain"] = {k: plain[k] for k in ("value", "ms_per_step", "prps_per_step",
+                                              "step_checksum", "kernel_ms_per_step")}
+    if pers is not None and plain is not None:
+        out["persistent_vs_plain"] = pers["value"] / plain["value"]
+        assert pers["step_checksum"] == plain["step_checksum"], "forms disagree"
+    if batched is not None:
+        out["batched_workload"] = batched
+    if tuning is not None:
+        out["tuning"] = tuning
     if copy_ms is not None:
         out["copy_probe"] = {"bytes": args.copy_probe, "ms": copy_ms}
     if not args.no_cpu:

@@ -48,6 +48,8 @@ extern "C" {
 #define WW_MIN_START 127ull
 /* Result-buffer size of the reference (numresults, main.cpp:1119). */
 #define WW_DEFAULT_RESULTS 15000u
+/* Default bound of the deep sieve (see ww_set_deep_bound). */
+#define WW_DEFAULT_DEEP_BOUND 4096u
 
 /* One result record (the four arrays gpuSpecialPrime/gpuResult/gpuRem/gpuQuot of
    main.cpp:1458-1493, as one struct). */
@@ -103,6 +105,15 @@ int ww_close(ww_ctx *ctx);
    strong pseudoprimes it could hold. Replaces the primesieve-based sizing
    (main.cpp:1087-1091). */
 uint64_t ww_list_bound(uint64_t low, uint64_t high);
+
+/* Bound B (0 or 127..8192) of the deep sieve applied by the candidate generators after
+   the reference's own filter by 3..113. No counterpart in the reference: it removes
+   composites with a factor in 127..B *except* the single residue class per factor in
+   which base-2 pseudoprimes divisible by it must lie, so the PRP list is unchanged and
+   only fewer composites reach the strong PRP test. B = 0 turns it off. Takes effect for
+   launches issued afterwards. Default WW_DEFAULT_DEEP_BOUND. */
+int ww_set_deep_bound(ww_ctx *ctx, uint32_t bound);
+int ww_get_deep_bound(ww_ctx *ctx, uint32_t *bound);
 
 /* clearresult (clearresult.cl:36-50): zero checksum, resultcount, totalprimecount
    and primecount_max. Stream-ordered. */
@@ -160,6 +171,18 @@ int ww_prp_flags(ww_ctx *ctx, const uint64_t *n, uint64_t count, uint8_t *out);
 int ww_cpu_is_prime(uint64_t n);
 int ww_cpu_is_prp(uint64_t n);
 
+/* One deep-sieve prime: ell, its base-2 pseudoprime class rho mod m with
+   m = ell * lcm(2, ord_ell(2)) (rho = 0 mod ell, rho = 1 mod lcm(2, ord_ell(2))), and
+   reduction constants c32_* = 2^32 mod *, inv_* = 1.0 / *. */
+typedef struct {
+    uint32_t ell, m, rho, c32_ell, c32_m, pad;
+    double inv_ell, inv_m;
+} ww_deep_prime;
+
+/* Host-side (CPU) table of the deep-sieve primes 127 <= ell <= bound, ascending; writes
+   up to cap entries and returns the number written. No counterpart in the reference. */
+uint32_t ww_deep_table(uint32_t bound, ww_deep_prime *out, uint32_t cap);
+
 /* The reference CPU path over [low, high) (main.cpp:648-697), on the host with the
    same arithmetic: the startup self-test's comparison target (testKernels). out =
    {prp count, checksum, cpu result count}. */
@@ -187,8 +210,9 @@ typedef struct ww_batch ww_batch;
 int ww_batch_create(ww_ctx *ctx, const ww_range *ranges, uint32_t n, uint32_t result_capacity,
                     ww_batch **out);
 
-/* Launch geometry: workgroups (4 waves each) per CU; 0 = as many as fit. */
-int ww_batch_set_grid(ww_batch *b, int blocks_per_cu);
+/* Launch geometry: workgroups (4 waves each) per CU (0 = as many as fit) and the
+   number of consecutive 8192-integer segments a wave claims at once (0 = 8). */
+int ww_batch_set_grid(ww_batch *b, int blocks_per_cu, uint32_t claim);
 
 /* Stream-ordered search of every range of the batch: clears the batch's sums, then one
    persistent kernel whose waves claim 8192-integer segments from a shared cursor.
@@ -208,6 +232,26 @@ int ww_batch_read(ww_ctx *ctx, ww_batch *b, ww_range_sum *sums, ww_result *recs,
 int ww_batch_time(ww_ctx *ctx, ww_batch *b, double *ms);
 
 int ww_batch_destroy(ww_batch *b);
+
+/* One configuration of the persistent form and its measured speed. */
+typedef struct {
+    uint32_t deep_bound; /* ww_set_deep_bound */
+    int32_t blocks_per_cu; /* ww_batch_set_grid; 0 = as many as fit */
+    uint32_t claim;        /* segments per claim */
+    uint32_t pad;
+    double ms;             /* kernel time of the sample */
+    double prps_per_s;     /* tested primes per second on the sample */
+} ww_tuning;
+
+/* Autotuner replacing the reference's work-size profiling (profileGPU,
+   main.cpp:959-1107): times the persistent form on the sample range [lo, hi) for every
+   combination of the given deep-sieve bounds, workgroups per CU and claim sizes (null
+   lists = built-in defaults), keeps the fastest as this context's deep bound and as the
+   default geometry of batches created afterwards, and returns it in *best. `trials`
+   (may be null) receives up to cap measured configurations; *ntrials their count. */
+int ww_autotune(ww_ctx *ctx, uint64_t lo, uint64_t hi, const uint32_t *bounds, uint32_t nb,
+                const int32_t *blocks, uint32_t nbl, const uint32_t *claims, uint32_t nc,
+                ww_tuning *best, ww_tuning *trials, uint32_t cap, uint32_t *ntrials);
 
 /* Measurement support, no counterpart in the reference: the issue rate of
    independent 32x32->64 multiplies on every CU of `device` (kind 0: v_mad_u64_u32,

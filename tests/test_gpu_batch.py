@@ -76,8 +76,8 @@ def test_batch_result_independent_of_order_and_grid(gctx):
     perm = list(range(len(ranges)))
     random.Random(1).shuffle(perm)
     with wwlib.Batch(gctx, [ranges[i] for i in perm]) as b:
-        for bpc in (1, 3, 0):
-            b.set_grid(bpc)
+        for bpc, claim in ((1, 1), (3, 5), (0, 64), (0, 0)):
+            b.set_grid(bpc, claim)
             b.launch()
             got = b.read()
             assert [got[perm.index(i)] for i in range(len(ranges))] == ref
@@ -92,3 +92,19 @@ def test_batch_single_range_equals_plain_at_full_size(gctx):
     plain = gctx.run_range(lo, hi, 1 << 32)
     print(f"persistent cfg3: {prp} PRPs in {ms:.1f} ms = {prp / ms * 1e3:.3e} /s")
     assert (prp, cks, recs) == (plain.prp_count, plain.checksum, plain.results)
+
+
+def test_autotune_keeps_results(gctx):
+    lo, hi = (1 << 62) + 12345, (1 << 62) + 12345 + (1 << 24)
+    plain = gctx.run_range(lo, hi)
+    try:
+        best, trials = gctx.autotune(lo, hi, bounds=[0, 4096], blocks=[0, 4], claims=[1, 16])
+        assert len(trials) == 8
+        assert best["ms"] == min(t["ms"] for t in trials)
+        assert gctx.deep_bound() == best["deep_bound"]
+        with wwlib.Batch(gctx, [(lo, hi)]) as b:
+            b.launch()
+            (prp, cks, recs), = b.read()
+        assert (prp, cks, recs) == (plain.prp_count, plain.checksum, plain.results)
+    finally:
+        gctx.set_deep_bound(wwlib.WW_DEFAULT_DEEP_BOUND)

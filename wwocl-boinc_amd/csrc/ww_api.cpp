@@ -17,6 +17,7 @@
 #include "../../include/ww_api.h"
 #include "ww_arith.h"
 #include "ww_batch.h"
+static_assert(sizeof(ww_deep_prime) == sizeof(ww::DeepPrime), "deep table layout");
 #include "ww_kernels.h"
 
 using ww::BatchSpan;
@@ -34,6 +35,13 @@ struct ww_ctx {
     uint32_t result_cap = 0;
     int test_grid = 0;
     int cus = 0;
+    ww::DeepPrime *d_deep = nullptr; /* primes 127..kMaxDeepBound, ascending */
+    uint32_t ndeep_max = 0;
+    uint32_t ndeep = 0; /* entries in use: primes <= the deep-sieve bound */
+    uint32_t deep_bound = 0;
+    std::vector<uint32_t> deep_ells;
+    int tuned_blocks = 0;      /* defaults for batches created from now on */
+    uint32_t tuned_claim = 8;
     uint64_t default_chunk = 0;
     /* event pairs of launches whose durations have not been collected yet */
     std::vector<hipEvent_t> ev_sieve, ev_tests;
@@ -180,6 +188,19 @@ int ww_open(int device, uint64_t list_capacity, uint32_t result_capacity, ww_ctx
                          (unsigned long long)list_capacity, hipGetErrorString(e)));
     if ((e = hipMalloc(&c->d_results, (size_t)result_capacity * sizeof(ww_result))) != hipSuccess)
         return bail(fail(WW_E_HIP, "hipMalloc(results): %s", hipGetErrorString(e)));
+    {
+        std::vector<ww::DeepPrime> t(ww::kMaxDeepBound / 8);
+        t.resize(ww_deep_table(ww::kMaxDeepBound, reinterpret_cast<ww_deep_prime *>(t.data()),
+                               (uint32_t)t.size()));
+        c->ndeep_max = (uint32_t)t.size();
+        for (const ww::DeepPrime &d : t) c->deep_ells.push_back(d.ell);
+        if ((e = hipMalloc(&c->d_deep, t.size() * sizeof(ww::DeepPrime))) != hipSuccess)
+            return bail(fail(WW_E_HIP, "hipMalloc(deep): %s", hipGetErrorString(e)));
+        if ((e = hipMemcpyAsync(c->d_deep, t.data(), t.size() * sizeof(ww::DeepPrime),
+                                hipMemcpyHostToDevice, c->stream)) != hipSuccess)
+            return bail(fail(WW_E_HIP, "hipMemcpy(deep): %s", hipGetErrorString(e)));
+        ww_set_deep_bound(c, WW_DEFAULT_DEEP_BOUND);
+    }
     if ((e = hipMemsetAsync(c->d_state, 0, sizeof(DevState), c->stream)) != hipSuccess)
         return bail(fail(WW_E_HIP, "hipMemsetAsync: %s", hipGetErrorString(e)));
     if ((e = hipStreamSynchronize(c->stream)) != hipSuccess)
@@ -195,11 +216,30 @@ int ww_close(ww_ctx *c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (hipEvent_t e : c->ev_sieve) (void)hipEventDestroy(e);
     for (hipEvent_t e : c->ev_tests) (void)hipEventDestroy(e);
+    if (c->d_deep) (void)hipFree(c->d_deep);
     if (c->d_results) (void)hipFree(c->d_results);
     if (c->d_list) (void)hipFree(c->d_list);
     if (c->d_state) (void)hipFree(c->d_state);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
+    return WW_OK;
+}
+
+int ww_set_deep_bound(ww_ctx *c, uint32_t bound)
+{
+    if (!c) return fail(WW_E_ARG, "null context");
+    if (bound > ww::kMaxDeepBound) return fail(WW_E_ARG, "deep bound above %u", ww::kMaxDeepBound);
+    uint32_t n = 0;
+    while (n < c->deep_ells.size() && c->deep_ells[n] <= bound) ++n;
+    c->ndeep = n;
+    c->deep_bound = bound;
+    return WW_OK;
+}
+
+int ww_get_deep_bound(ww_ctx *c, uint32_t *bound)
+{
+    if (!c || !bound) return fail(WW_E_ARG, "null argument");
+    *bound = c->deep_bound;
     return WW_OK;
 }
 
@@ -232,7 +272,8 @@ int ww_getsegprimes(ww_ctx *c, uint64_t low, uint64_t high)
     hipEvent_t a, b;
     if ((rc = event_pair(c, c->ev_sieve, &a, &b))) return rc;
     HIP_TRY(hipEventRecord(a, c->stream));
-    HIP_TRY(ww::launch_sieve_prp(low, high, c->d_list, c->list_cap, c->d_state, c->stream));
+    HIP_TRY(ww::launch_sieve_prp(low, high, c->d_list, c->list_cap, c->d_state, c->d_deep,
+                                 c->ndeep, c->stream));
     HIP_TRY(hipEventRecord(b, c->stream));
     c->launches++;
     if (c->ev_sieve.size() >= kMaxPendingEvents) return collect(c->ev_sieve, &c->ms_sieve);
@@ -409,6 +450,7 @@ struct ww_batch {
     uint32_t n = 0;
     uint32_t res_cap = 0;
     int blocks_per_cu = 0;
+    uint32_t claim = 8;
     BatchSpan *d_spans = nullptr;
     uint64_t *d_prefix = nullptr;
     SlotSum *d_sums = nullptr;
@@ -417,6 +459,8 @@ struct ww_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     int graph_grid = 0;
+    uint32_t graph_claim = 0;
+    uint32_t graph_ndeep = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
 
@@ -432,7 +476,7 @@ int batch_grid(ww_ctx *c, ww_batch *b)
 int batch_enqueue(ww_ctx *c, ww_batch *b, int grid)
 {
     HIP_TRY(ww::launch_batch(b->d_spans, b->d_prefix, b->n, b->d_state, b->d_sums, b->n, b->d_res,
-                             b->res_cap, grid, c->stream));
+                             b->res_cap, grid, b->claim, c->d_deep, c->ndeep, c->stream));
     return WW_OK;
 }
 
@@ -463,6 +507,8 @@ int ww_batch_create(ww_ctx *c, const ww_range *ranges, uint32_t n, uint32_t resu
     }
     ww_batch *b = new ww_batch;
     b->device = c->device;
+    b->blocks_per_cu = c->tuned_blocks;
+    b->claim = c->tuned_claim;
     b->n = n;
     b->res_cap = result_capacity;
     hipError_t e = hipMalloc(&b->d_spans, n * sizeof(BatchSpan));
@@ -486,10 +532,11 @@ int ww_batch_create(ww_ctx *c, const ww_range *ranges, uint32_t n, uint32_t resu
     return WW_OK;
 }
 
-int ww_batch_set_grid(ww_batch *b, int blocks_per_cu)
+int ww_batch_set_grid(ww_batch *b, int blocks_per_cu, uint32_t claim)
 {
-    if (!b || blocks_per_cu < 0) return fail(WW_E_ARG, "bad grid");
+    if (!b || blocks_per_cu < 0 || claim > 4096) return fail(WW_E_ARG, "bad grid");
     b->blocks_per_cu = blocks_per_cu;
+    b->claim = claim ? claim : 8;
     return WW_OK;
 }
 
@@ -510,7 +557,8 @@ int ww_batch_launch_graph(ww_ctx *c, ww_batch *b)
     if (rc) return rc;
     if (!b) return fail(WW_E_ARG, "null batch");
     int grid = batch_grid(c, b);
-    if (b->exec && b->graph_grid != grid) {
+    if (b->exec && (b->graph_grid != grid || b->graph_claim != b->claim ||
+                    b->graph_ndeep != c->ndeep)) {
         (void)hipGraphExecDestroy(b->exec);
         (void)hipGraphDestroy(b->graph);
         b->exec = nullptr;
@@ -526,6 +574,8 @@ int ww_batch_launch_graph(ww_ctx *c, ww_batch *b)
         b->graph = g;
         HIP_TRY(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
         b->graph_grid = grid;
+        b->graph_claim = b->claim;
+        b->graph_ndeep = c->ndeep;
     }
     HIP_TRY(hipEventRecord(b->ev0, c->stream));
     HIP_TRY(hipGraphLaunch(b->exec, c->stream));
@@ -585,6 +635,75 @@ int ww_batch_destroy(ww_batch *b)
     if (b->d_prefix) (void)hipFree(b->d_prefix);
     if (b->d_spans) (void)hipFree(b->d_spans);
     delete b;
+    return WW_OK;
+}
+
+int ww_autotune(ww_ctx *c, uint64_t lo, uint64_t hi, const uint32_t *bounds, uint32_t nb,
+                const int32_t *blocks, uint32_t nbl, const uint32_t *claims, uint32_t nc,
+                ww_tuning *best, ww_tuning *trials, uint32_t cap, uint32_t *ntrials)
+{
+    int rc = use(c);
+    if (rc) return rc;
+    static const uint32_t kBounds[] = {0, 1024, 2048, 4096, 8192};
+    static const int32_t kBlocks[] = {0, 4, 5};
+    static const uint32_t kClaims[] = {1, 4, 16, 64};
+    if (!bounds || !nb) {
+        bounds = kBounds;
+        nb = sizeof kBounds / sizeof *kBounds;
+    }
+    if (!blocks || !nbl) {
+        blocks = kBlocks;
+        nbl = sizeof kBlocks / sizeof *kBlocks;
+    }
+    if (!claims || !nc) {
+        claims = kClaims;
+        nc = sizeof kClaims / sizeof *kClaims;
+    }
+    ww_range r = {lo, hi};
+    ww_batch *b = nullptr;
+    if ((rc = ww_batch_create(c, &r, 1, 1u << 16, &b))) return rc;
+    const uint32_t saved_bound = c->deep_bound;
+    ww_tuning top;
+    memset(&top, 0, sizeof top);
+    top.ms = 1e300;
+    uint32_t n = 0;
+    for (uint32_t i = 0; i < nb && rc == WW_OK; ++i) {
+        if ((rc = ww_set_deep_bound(c, bounds[i]))) break;
+        for (uint32_t j = 0; j < nbl && rc == WW_OK; ++j) {
+            for (uint32_t k = 0; k < nc && rc == WW_OK; ++k) {
+                if ((rc = ww_batch_set_grid(b, blocks[j], claims[k]))) break;
+                double ms = 0, t = 0;
+                ww_range_sum sum;
+                for (int rep = 0; rep < 2 && rc == WW_OK; ++rep) { /* first run warms */
+                    if ((rc = ww_batch_launch(c, b))) break;
+                    if ((rc = ww_batch_time(c, b, &t))) break;
+                    ms = t;
+                }
+                if (rc) break;
+                if ((rc = ww_batch_read(c, b, &sum, nullptr, 0, nullptr, nullptr))) break;
+                ww_tuning tr;
+                tr.deep_bound = bounds[i];
+                tr.blocks_per_cu = blocks[j];
+                tr.claim = claims[k];
+                tr.pad = 0;
+                tr.ms = ms;
+                tr.prps_per_s = ms > 0 ? sum.prp_count / (ms * 1e-3) : 0;
+                if (trials && n < cap) trials[n] = tr;
+                ++n;
+                if (ms < top.ms) top = tr;
+            }
+        }
+    }
+    ww_batch_destroy(b);
+    if (rc) {
+        ww_set_deep_bound(c, saved_bound);
+        return rc;
+    }
+    ww_set_deep_bound(c, top.deep_bound);
+    c->tuned_blocks = top.blocks_per_cu;
+    c->tuned_claim = top.claim;
+    if (best) *best = top;
+    if (ntrials) *ntrials = n;
     return WW_OK;
 }
 

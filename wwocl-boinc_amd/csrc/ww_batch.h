@@ -37,11 +37,13 @@ inline uint64_t segments_of(uint64_t lo, uint64_t hi)
 
 int batch_blocks_per_cu();
 
-/* Clears the state and the slot sums, then runs the persistent kernel over all spans.
+/* Clears the state and the slot sums, then runs the persistent kernel over all spans;
+   each wave claims `claim` consecutive segments at a time.
    No allocation, no memset: safe to capture into a graph. */
 hipError_t launch_batch(const BatchSpan *spans, const uint64_t *prefix, uint32_t nspans,
                         BatchState *st, SlotSum *sums, uint32_t nslots, ww_result *res,
-                        uint32_t res_cap, int grid, hipStream_t s);
+                        uint32_t res_cap, int grid, uint32_t claim, const DeepPrime *deep,
+                        uint32_t ndeep, hipStream_t s);
 
 } // namespace ww
 

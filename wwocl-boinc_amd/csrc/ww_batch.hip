@@ -5,7 +5,8 @@
  * numbers each) of one range. The ranges' segments are laid end to end ("spans", with a
  * prefix sum of their segment counts) and every wave claims the next segment from one
  * global cursor (work stealing), so uneven ranges balance by construction. Per segment a
- * wave sieves by 3..113 in registers, compacts candidates into LDS, runs the base-2
+ * wave sieves by 3..113 in registers (and optionally by 127..B in LDS, see deep_sieve),
+ * compacts candidates into LDS, runs the base-2
  * strong PRP 64 candidates at a time and queues the survivors (p, range slot) in LDS;
  * whenever 64 are queued it runs the Wieferich + Wall-Sun-Sun tests on them, one per
  * lane, so the test lanes stay full across segment and range boundaries. Checksums and
@@ -81,29 +82,35 @@ __device__ __forceinline__ void tests_round(u32 n, u32 &qn, u64 *wq, u32 *ws, u3
     u64 v = 0;
     if (act) v = test_prime(p, res, res_cap, rcount, (int32_t)slot);
 
-    const u32 slot0 = uniform32(slot);
-    if (__ballot(act && slot != slot0) == 0) {
-        u64 sum = v;
+    /* per-slot sums: one wave reduction per distinct slot in the round (entries of one
+       segment are contiguous, so a round holds one slot or a few) */
+    u64 todo = __ballot(act);
+    while (todo) {
+        const u32 si = uniform32(__shfl(slot, (int)__builtin_ctzll(todo), 64));
+        const bool mine = act && slot == si;
+        const u64 m = __ballot(mine);
+        u64 sum = mine ? v : 0;
         for (int d = 32; d > 0; d >>= 1) sum += __shfl_xor(sum, d, 64);
         sum = uniform64(sum);
-        if (slot0 != acc.slot) {
+        if (si != acc.slot) {
             acc_flush(acc, sums, lane);
-            acc.slot = slot0;
+            acc.slot = si;
         }
         acc.cks += sum;
-        acc.cnt += n;
-    } else if (act) {
-        atomicAdd((unsigned long long *)&sums[slot].checksum, (unsigned long long)v);
-        atomicAdd((unsigned long long *)&sums[slot].count, 1ull);
+        acc.cnt += (u64)__popcll(m);
+        todo &= ~m;
     }
 }
 
 __global__ __launch_bounds__(64 * kBatchWaves) __attribute__((amdgpu_waves_per_eu(6))) void k_batch(const BatchSpan *spans,
                                                              const u64 *prefix, u32 nspans,
                                                              BatchState *st, SlotSum *sums,
-                                                             ww_result *res, u32 res_cap)
+                                                             ww_result *res, u32 res_cap,
+                                                             u32 claim, const DeepPrime *deep,
+                                                             u32 ndeep)
 {
     __shared__ uint16_t cand[kBatchWaves][kWaveCandCap];
+    __shared__ u64 deep_bm[kBatchWaves][64];
     __shared__ u64 qp[kBatchWaves][kQueueCap];
     __shared__ u32 qs[kBatchWaves][kQueueCap];
 
@@ -125,6 +132,7 @@ __global__ __launch_bounds__(64 * kBatchWaves) __attribute__((amdgpu_waves_per_e
     bool done = false;
     u32 j0 = 0, ncand = 0, slot = 0;
     u64 group = 0;
+    u64 seg_next = 0, seg_end = 0; /* segments claimed but not yet sieved */
     for (;;) {
         if (qn >= 64 || (done && qn > 0)) {
             tests_round(qn >= 64 ? 64 : qn, qn, wq, ws, lane, acc, sums, res, res_cap,
@@ -153,13 +161,18 @@ __global__ __launch_bounds__(64 * kBatchWaves) __attribute__((amdgpu_waves_per_e
             wave_sync_lds();
             continue;
         }
-        u64 s = 0;
-        if (lane == 0) s = atomicAdd((unsigned long long *)&st->cursor, 1ull);
-        s = uniform64(__shfl(s, 0, 64));
-        if (s >= total) {
-            done = true;
-            continue;
+        if (seg_next >= seg_end) {
+            /* claim `claim` consecutive segments: one atomic, and runs of one range */
+            u64 c = 0;
+            if (lane == 0) c = atomicAdd((unsigned long long *)&st->cursor, (unsigned long long)claim);
+            seg_next = uniform64(__shfl(c, 0, 64));
+            seg_end = seg_next + claim < total ? seg_next + claim : total;
+            if (seg_next >= total) {
+                done = true;
+                continue;
+            }
         }
+        const u64 s = seg_next++;
         /* span holding segment s: largest a with prefix[a] <= s */
         u32 a = 0, b = nspans;
         while (b - a > 1) {
@@ -173,6 +186,13 @@ __global__ __launch_bounds__(64 * kBatchWaves) __attribute__((amdgpu_waves_per_e
 
         /* sieve, then compact candidate positions into LDS */
         u64 bits = sieve_word(group, lane * (u32)kIntsPerWord, sp.hi);
+        if (ndeep) {
+            u64 *bm = deep_bm[wave];
+            bm[lane] = 0;
+            wave_sync_lds();
+            deep_sieve(bm, group, deep, ndeep, lane);
+            bits &= ~bm[lane];
+        }
         const u32 cnt = __popcll(bits);
         u32 incl = cnt;
         for (int d = 1; d < 64; d <<= 1) {
@@ -218,13 +238,13 @@ int batch_blocks_per_cu()
 
 hipError_t launch_batch(const BatchSpan *spans, const u64 *prefix, u32 nspans, BatchState *st,
                         SlotSum *sums, u32 nslots, ww_result *res, u32 res_cap, int grid,
-                        hipStream_t s)
+                        u32 claim, const DeepPrime *deep, u32 ndeep, hipStream_t s)
 {
     hipLaunchKernelGGL(k_batch_clear, dim3((nslots + 255) / 256), dim3(256), 0, s, st, sums, nslots);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_batch, dim3(grid), dim3(64 * kBatchWaves), 0, s, spans, prefix, nspans, st,
-                       sums, res, res_cap);
+                       sums, res, res_cap, claim < 1 ? 1u : claim, deep, ndeep);
     return hipGetLastError();
 }
 

@@ -6,6 +6,8 @@
  *   ww_cpu_is_prp     isPRP, main.cpp:245-291
  *   ww_cpu_range      testKernels' CPU loop, main.cpp:648-697
  */
+#include <vector>
+
 #include "../../include/ww_api.h"
 #include "ww_arith.h"
 
@@ -97,6 +99,44 @@ int ww_cpu_range(uint64_t low, uint64_t high, uint64_t out[3])
     out[1] = sum;
     out[2] = flags;
     return WW_OK;
+}
+
+uint32_t ww_deep_table(uint32_t bound, ww_deep_prime *out, uint32_t cap)
+{
+    std::vector<uint8_t> comp(bound + 1, 0);
+    uint32_t n = 0;
+    for (uint32_t i = 2; i <= bound; ++i) {
+        if (comp[i]) continue;
+        for (uint64_t j = (uint64_t)i * i; j <= bound; j += i) comp[j] = 1;
+        if (i <= 113) continue;
+        if (n < cap && out) {
+            uint32_t ord = 1;
+            for (uint64_t x = 2 % i; x != 1; x = x * 2 % i) ++ord;
+            const int64_t L = (ord % 2 == 0) ? ord : 2 * (int64_t)ord;
+            /* t = l^-1 mod L (extended Euclid); rho = l t is 0 mod l and 1 mod L */
+            int64_t r0 = L, r1 = i % L, t0 = 0, t1 = 1;
+            while (r1) {
+                int64_t q = r0 / r1, tmp = r0 - q * r1;
+                r0 = r1;
+                r1 = tmp;
+                tmp = t0 - q * t1;
+                t0 = t1;
+                t1 = tmp;
+            }
+            const uint64_t t = (uint64_t)(((t0 % L) + L) % L);
+            ww_deep_prime &d = out[n];
+            d.ell = i;
+            d.m = (uint32_t)(i * (uint64_t)L);
+            d.rho = (uint32_t)(i * t);
+            d.c32_ell = (uint32_t)((1ull << 32) % i);
+            d.c32_m = (uint32_t)((1ull << 32) % d.m);
+            d.pad = 0;
+            d.inv_ell = 1.0 / (double)i;
+            d.inv_m = 1.0 / (double)d.m;
+        }
+        ++n;
+    }
+    return n < cap ? n : cap;
 }
 
 } // extern "C"

@@ -65,6 +65,66 @@ __device__ __forceinline__ u64 sieve_word(u64 group, u32 lane_off, u64 hi)
     return ~comp & valid;
 }
 
+/* ---------------------------------------------------------------------------
+ * Deep sieve by the primes 127 <= l <= B over one wave's 4096 odd numbers, with the
+ * exact rescue of base-2 pseudoprimes.
+ *
+ * A composite n = l*k that is a base-2 Fermat pseudoprime (every strong one is) has
+ * 2^(n-1) = 1 (mod l), so ord_l(2) | n - 1: n lies in the single class n = 0 (mod l),
+ * n = 1 (mod lcm(2, ord_l(2))), i.e. n = rho (mod M) with M = l * lcm(2, ord_l(2)).
+ * Marking every multiple of l and then clearing that class again therefore removes only
+ * numbers that cannot be base-2 strong probable primes; l itself (a prime candidate) is
+ * never marked. The strong PRP test still decides membership, so the list is unchanged.
+ * ------------------------------------------------------------------------- */
+
+/* x mod m for x < 2^53 with a double reciprocal: the quotient estimate is off by at
+   most one and is corrected exactly. */
+__device__ __forceinline__ u32 mod_small(u64 x, u32 m, double inv)
+{
+    const u64 q = (u64)((double)x * inv);
+    int64_t r = (int64_t)x - (int64_t)(q * m);
+    if (r < 0) r += m;
+    else if (r >= (int64_t)m) r -= m;
+    return (u32)r;
+}
+
+/* x mod m for any 64-bit x and m < 2^26, with c32 = 2^32 mod m. */
+__device__ __forceinline__ u32 mod_u64(u64 x, u32 m, double inv, u32 c32)
+{
+    const u32 rh = mod_small(x >> 32, m, inv);
+    const u32 rl = mod_small((u32)x, m, inv);
+    return mod_small((u64)rh * c32 + rl, m, inv); /* < 2^52 */
+}
+
+/* Marks (bits set) the multiples of the deep primes among group + 2i, i < 4096, in the
+   wave's bitmap bm[64] and then clears each prime's pseudoprime class. Lanes share the
+   primes round-robin. Called by all 64 lanes of a wave; bm must be zeroed before. */
+__device__ __forceinline__ void deep_sieve(u64 *bm, u64 group, const DeepPrime *deep, u32 ndeep,
+                                           u32 lane)
+{
+    for (u32 j = lane; j < ndeep; j += 64) {
+        const DeepPrime d = deep[j];
+        const u32 r = mod_u64(group, d.ell, d.inv_ell, d.c32_ell);
+        u32 i = mod_small((u64)(d.ell - r) * ((d.ell + 1) >> 1), d.ell, d.inv_ell); /* ell | group+2i */
+        if (group <= d.ell && group + 2ull * i == d.ell) i += d.ell;  /* never ell itself */
+        for (; i < 4096; i += d.ell) atomicOr((unsigned long long *)&bm[i >> 6], 1ull << (i & 63));
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (u32 j = lane; j < ndeep; j += 64) {
+        const DeepPrime d = deep[j];
+        const u32 rm = mod_u64(group, d.m, d.inv_m, d.c32_m);
+        /* group + 2i = rho (mod m), both odd, m even */
+        const u32 diff = d.rho >= rm ? d.rho - rm : d.rho + d.m - rm;
+        for (u32 i = diff >> 1; i < 4096; i += d.m >> 1)
+            atomicAnd((unsigned long long *)&bm[i >> 6], ~(1ull << (i & 63)));
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 /* Append one record; `tag` goes to the record's pad field (the range slot in the
    persistent form, 0 in the plain form). */
 __device__ __forceinline__ void put_result(ww_result *res, u32 res_cap, u32 *count, u64 p,

@@ -36,8 +36,17 @@ constexpr int kWaveCandCap = 64 * kMaxCandPerWord;                 /* 2304 */
 
 constexpr int kTestThreads = 256;
 
+/* One prime of the deep sieve (127 <= ell <= 8192) with its base-2 pseudoprime class
+   rho mod m, m = ell * lcm(2, ord_ell(2)) < 2^26; c32_* = 2^32 mod *, inv_* = 1 / *.
+   Same layout as ww_deep_prime of the C ABI, which builds the table on the host. */
+struct DeepPrime {
+    uint32_t ell, m, rho, c32_ell, c32_m, pad;
+    double inv_ell, inv_m;
+};
+constexpr uint32_t kMaxDeepBound = 8192;
+
 hipError_t launch_sieve_prp(uint64_t lo, uint64_t hi, uint64_t *list, uint64_t cap,
-                            DevState *st, hipStream_t s);
+                            DevState *st, const DeepPrime *deep, uint32_t ndeep, hipStream_t s);
 hipError_t launch_ww_tests(const uint64_t *list, uint64_t cap, DevState *st,
                            ww_result *res, uint32_t res_cap, int grid, hipStream_t s);
 int tests_blocks_per_cu();

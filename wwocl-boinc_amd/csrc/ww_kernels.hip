@@ -2,7 +2,8 @@
  * ww_kernels.hip -- gfx950 kernels of the search path.
  *
  *   k_sieve_prp   segmented candidate generation (odd-only bitsieve by 3..113 in
- *                 registers), wave-local compaction into LDS, base-2 strong PRP on the
+ *                 registers, optional deep sieve by 127..B in LDS with exact rescue of
+ *                 base-2 pseudoprimes), wave-local compaction into LDS, base-2 strong PRP on the
  *                 compacted candidates, in-place compaction of the survivors and one
  *                 global append per workgroup. Replaces getsegprimes.cl:228-301.
  *   k_ww_tests    Wieferich + Wall-Sun-Sun on every list entry, one entry per lane,
@@ -16,9 +17,11 @@
 namespace ww {
 
 __global__ __launch_bounds__(kSieveThreads) void k_sieve_prp(u64 base, u64 hi, u64 *list,
-                                                              u64 cap, DevState *st)
+                                                              u64 cap, DevState *st,
+                                                              const DeepPrime *deep, u32 ndeep)
 {
     __shared__ uint16_t cand[kSieveWaves * kWaveCandCap];
+    __shared__ u64 deep_bm[kSieveWaves][64];
     __shared__ u32 s_cnt[kSieveWaves];
     __shared__ u32 s_off;
 
@@ -31,6 +34,15 @@ __global__ __launch_bounds__(kSieveThreads) void k_sieve_prp(u64 base, u64 hi, u
 
     /* ---- sieve this lane's 64 odd numbers ---- */
     u64 bits = sieve_word(block_base, lane_pos, hi);
+    if (ndeep) {
+        u64 *bm = deep_bm[wave];
+        bm[lane] = 0;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        deep_sieve(bm, wave_base, deep, ndeep, lane);
+        bits &= ~bm[lane];
+    }
 
     /* ---- wave-local compaction of candidate positions into LDS ---- */
     u32 cnt = __popcll(bits);
@@ -144,14 +156,15 @@ int tests_blocks_per_cu()
     return nb;
 }
 
-hipError_t launch_sieve_prp(u64 lo, u64 hi, u64 *list, u64 cap, DevState *st, hipStream_t s)
+hipError_t launch_sieve_prp(u64 lo, u64 hi, u64 *list, u64 cap, DevState *st,
+                            const DeepPrime *deep, u32 ndeep, hipStream_t s)
 {
     u64 base = lo | 1;
     if (base >= hi) return hipSuccess;
     u64 span = hi - base;
     u64 blocks = (span + kIntsPerBlock - 1) / kIntsPerBlock;
     hipLaunchKernelGGL(k_sieve_prp, dim3((unsigned)blocks), dim3(kSieveThreads), 0, s, base, hi,
-                       list, cap, st);
+                       list, cap, st, deep, ndeep);
     return hipGetLastError();
 }
 

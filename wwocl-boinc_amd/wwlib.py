@@ -21,6 +21,7 @@ WW_E_NODEV = -3
 WW_E_CAPACITY = -4
 WW_MIN_START = 127
 WW_DEFAULT_RESULTS = 15000
+WW_DEFAULT_DEEP_BOUND = 4096
 
 U64 = ctypes.c_uint64
 U32 = ctypes.c_uint32
@@ -51,6 +52,20 @@ class RangeSum(ctypes.Structure):
     _fields_ = [("checksum", U64), ("prp_count", U64)]
 
 
+class DeepPrime(ctypes.Structure):
+    _fields_ = [("ell", U32), ("m", U32), ("rho", U32), ("c32_ell", U32), ("c32_m", U32),
+                ("pad", U32), ("inv_ell", ctypes.c_double), ("inv_m", ctypes.c_double)]
+
+
+class Tuning(ctypes.Structure):
+    _fields_ = [("deep_bound", U32), ("blocks_per_cu", ctypes.c_int32), ("claim", U32),
+                ("pad", U32), ("ms", ctypes.c_double), ("prps_per_s", ctypes.c_double)]
+
+    def as_dict(self):
+        return {"deep_bound": self.deep_bound, "blocks_per_cu": self.blocks_per_cu,
+                "claim": self.claim, "ms": self.ms, "prps_per_s": self.prps_per_s}
+
+
 class DeviceInfo(ctypes.Structure):
     _fields_ = [("name", ctypes.c_char * 256), ("arch", ctypes.c_char * 64),
                 ("compute_units", ctypes.c_int), ("clock_khz", ctypes.c_int),
@@ -71,6 +86,8 @@ _SIGNATURES = [
     ("ww_open", ctypes.c_int, [ctypes.c_int, U64, U32, ctypes.POINTER(ctypes.c_void_p)]),
     ("ww_close", ctypes.c_int, [ctypes.c_void_p]),
     ("ww_list_bound", U64, [U64, U64]),
+    ("ww_set_deep_bound", ctypes.c_int, [ctypes.c_void_p, U32]),
+    ("ww_get_deep_bound", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(U32)]),
     ("ww_clear_results", ctypes.c_int, [ctypes.c_void_p]),
     ("ww_clear_prp_count", ctypes.c_int, [ctypes.c_void_p]),
     ("ww_getsegprimes", ctypes.c_int, [ctypes.c_void_p, U64, U64]),
@@ -90,7 +107,7 @@ _SIGNATURES = [
                                     ctypes.POINTER(ctypes.c_uint8)]),
     ("ww_batch_create", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Range), U32, U32,
                                        ctypes.POINTER(ctypes.c_void_p)]),
-    ("ww_batch_set_grid", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    ("ww_batch_set_grid", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, U32]),
     ("ww_batch_launch", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     ("ww_batch_launch_graph", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     ("ww_batch_read", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(RangeSum),
@@ -99,12 +116,17 @@ _SIGNATURES = [
     ("ww_batch_time", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p,
                                      ctypes.POINTER(ctypes.c_double)]),
     ("ww_batch_destroy", ctypes.c_int, [ctypes.c_void_p]),
+    ("ww_autotune", ctypes.c_int, [ctypes.c_void_p, U64, U64, ctypes.POINTER(U32), U32,
+                                   ctypes.POINTER(ctypes.c_int32), U32, ctypes.POINTER(U32), U32,
+                                   ctypes.POINTER(Tuning), ctypes.POINTER(Tuning), U32,
+                                   ctypes.POINTER(U32)]),
     ("ww_peak_mul32", ctypes.c_int, [ctypes.c_int, ctypes.c_int,
                                      ctypes.POINTER(ctypes.c_double)]),
     ("ww_copy_probe", ctypes.c_int, [ctypes.c_int, U64, ctypes.POINTER(ctypes.c_double)]),
     ("ww_cpu_is_prime", ctypes.c_int, [U64]),
     ("ww_cpu_is_prp", ctypes.c_int, [U64]),
     ("ww_cpu_range", ctypes.c_int, [U64, U64, ctypes.POINTER(U64)]),
+    ("ww_deep_table", U32, [U32, ctypes.POINTER(DeepPrime), U32]),
 ]
 
 EXPORTED = [s[0] for s in _SIGNATURES]
@@ -173,6 +195,14 @@ def cpu_is_prp(n: int) -> bool:
     return bool(load().ww_cpu_is_prp(n))
 
 
+def deep_table(bound: int):
+    """[(ell, m, rho)] of the deep sieve up to `bound` (host-side, no device)."""
+    cap = 2048
+    buf = (DeepPrime * cap)()
+    n = load().ww_deep_table(bound, buf, cap)
+    return [(buf[i].ell, buf[i].m, buf[i].rho, buf[i].c32_ell, buf[i].c32_m) for i in range(n)]
+
+
 def cpu_range(low: int, high: int):
     out = (U64 * 3)()
     _check(load().ww_cpu_range(low, high, out))
@@ -219,6 +249,29 @@ class Context:
             self.close()
         except Exception:
             pass
+
+    def set_deep_bound(self, bound: int):
+        _check(self.lib.ww_set_deep_bound(self.h, bound))
+
+    def deep_bound(self) -> int:
+        v = U32()
+        _check(self.lib.ww_get_deep_bound(self.h, ctypes.byref(v)))
+        return v.value
+
+    def autotune(self, lo: int, hi: int, bounds=None, blocks=None, claims=None):
+        """Tune the persistent form on [lo, hi); returns (best, [trials]) as dicts."""
+        def arr(t, v):
+            return ((t * len(v))(*v), len(v)) if v else (None, 0)
+        b, nb = arr(U32, bounds)
+        bl, nbl = arr(ctypes.c_int32, blocks)
+        c, nc = arr(U32, claims)
+        best = Tuning()
+        cap = 256
+        trials = (Tuning * cap)()
+        n = U32()
+        _check(self.lib.ww_autotune(self.h, lo, hi, b, nb, bl, nbl, c, nc, ctypes.byref(best),
+                                    trials, cap, ctypes.byref(n)))
+        return best.as_dict(), [trials[i].as_dict() for i in range(min(n.value, cap))]
 
     # reference kernel-level entry points
     def clear_results(self):
@@ -308,8 +361,8 @@ class Batch:
         self.result_capacity = result_capacity
         _check(self.lib.ww_batch_create(ctx.h, arr, self.n, result_capacity, ctypes.byref(self.h)))
 
-    def set_grid(self, blocks_per_cu: int):
-        _check(self.lib.ww_batch_set_grid(self.h, blocks_per_cu))
+    def set_grid(self, blocks_per_cu: int, claim: int = 0):
+        _check(self.lib.ww_batch_set_grid(self.h, blocks_per_cu, claim))
 
     def launch(self, graph: bool = False):
         fn = self.lib.ww_batch_launch_graph if graph else self.lib.ww_batch_launch
