@@ -4,7 +4,8 @@ One step = the whole range [2^63, 2^63 + 2^33) through the HIP path (per 2^32-wi
 chunk: clear the list, sieve + PRP, Wieferich + Wall-Sun-Sun), inputs generated on the
 device, the checksum accumulated on the device. Prints one JSON line.
 
-    python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--no-cpu]
+    python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--no-cpu] [--no-batch]
+                    [--serve-seconds 10] [--serve-rates 20,50,100] [--no-tune]
 """
 from __future__ import annotations
 
@@ -150,6 +151,34 @@ def run_batched_workload(ctx):
             "speedup": t_plain / t_batch, "per_range_checksums_equal": True}
 
 
+def run_serving(ctx, rates, seconds):
+    """Serving form: Poisson request streams (seed 1, widths 2^20..2^30, starts in
+    [2^32, 2^64 - 2^30]) submitted in real time; p50/p99 latency (submit to result) and
+    tested primes/s; 16 requests per rate re-run in the plain form must agree."""
+    out = {}
+    for rate in rates:
+        reqs = wwlib.serving_workload(rate=rate, duration_s=seconds, seed=1)
+        with wwlib.Server(ctx) as srv:
+            done, ids, wall = wwlib.serve(srv, reqs, realtime=True)
+            st = srv.stats()
+        tested = sum(c["prp_count"] for c in done.values())
+        span = max(c["t_done"] for c in done.values()) - min(c["t_submit"] for c in done.values())
+        rec = {"requests": len(done), "tested": tested, "tested_per_s": tested / span,
+               "offered_req_per_s": rate, "seconds": seconds, "ticks": st.ticks}
+        rec.update(wwlib.latency_summary(done))
+        for rid in sorted(ids)[:: max(1, len(ids) // 16)][:16]:
+            lo, hi = ids[rid]
+            ctx.clear_results()
+            ctx.clear_prp_count()
+            ctx.search_range(lo, hi, CHUNK)
+            c = ctx.counters()
+            assert (c.totalprimecount, c.checksum) == (done[rid]["prp_count"],
+                                                       done[rid]["checksum"]), (lo, hi)
+        rec["checked_vs_plain"] = 16
+        out[str(rate)] = rec
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -158,6 +187,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-batch", action="store_true", help="skip the 4096-range workload")
     ap.add_argument("--form", choices=["both", "persistent", "plain"], default="both")
+    ap.add_argument("--serve-seconds", type=float, default=10.0,
+                    help="length of each serving run (0 = skip serving)")
+    ap.add_argument("--serve-rates", default="20,50,100", help="request rates (per second)")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the autotuner (defaults: deep bound 4096, full occupancy)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
@@ -188,6 +220,10 @@ def main():
             plain = run_plain(ctx, args.steps, args.warmup)
         if not args.no_batch:
             batched = run_batched_workload(ctx)
+        serving = None
+        if args.serve_seconds > 0:
+            serving = run_serving(ctx, [float(r) for r in args.serve_rates.split(",")],
+                                  args.serve_seconds)
 
     head = pers if pers is not None else plain
     out = {
@@ -240,6 +276,8 @@ def main():
         out["batched_workload"] = batched
     if tuning is not None:
         out["tuning"] = tuning
+    if serving is not None:
+        out["serving"] = serving
     if copy_ms is not None:
         out["copy_probe"] = {"bytes": args.copy_probe, "ms": copy_ms}
     if not args.no_cpu:

@@ -253,6 +253,56 @@ int ww_autotune(ww_ctx *ctx, uint64_t lo, uint64_t hi, const uint32_t *bounds, u
                 const int32_t *blocks, uint32_t nbl, const uint32_t *claims, uint32_t nc,
                 ww_tuning *best, ww_tuning *trials, uint32_t cap, uint32_t *ntrials);
 
+/* ---------------------------------------------------------------------------
+ * Serving form: continuous batching of range requests. No counterpart in the
+ * reference (one range per process). An engine thread admits requests into device
+ * slots as slots and HBM budget free up and runs one persistent launch per tick over
+ * the next segments of every active request; per request the checksum, count and
+ * records equal the plain form's for that range alone.
+ * ------------------------------------------------------------------------- */
+
+typedef struct ww_server ww_server;
+
+typedef struct {
+    uint32_t slots;             /* requests resident at once (0 = 4096) */
+    uint32_t tick_segments;     /* 8192-integer segments per launch (0 = 2^17) */
+    uint64_t hbm_budget;        /* device bytes admissions may reserve (0 = 64 GiB; capped at 90% of free HBM) */
+    uint64_t bytes_per_request; /* reservation of one admitted request (0 = 1 MiB) */
+    uint32_t claim;             /* segments per claim (0 = context default) */
+    int32_t blocks_per_cu;      /* workgroups per CU (-1 = context default, 0 = as many as fit) */
+} ww_server_config;
+
+typedef struct {
+    uint64_t id, lo, hi;
+    uint64_t checksum, prp_count; /* as the plain form gives for [lo, hi) alone */
+    double t_submit, t_admit, t_done; /* seconds since the server started */
+    uint32_t nrecords, pad;
+} ww_request_info;
+
+typedef struct {
+    double t_start;
+    uint64_t submitted, completed, tested, ticks, segments, pending, active;
+} ww_server_stats;
+
+/* Start a server on the context's device (uses its deep-sieve table and tuned geometry;
+   cfg may be null). Allocates device and pinned memory. */
+int ww_server_create(ww_ctx *ctx, const ww_server_config *cfg, ww_server **out);
+
+/* Queue a request for [lo, hi) (WW_MIN_START <= lo < hi); non-blocking. */
+int ww_server_submit(ww_server *s, uint64_t lo, uint64_t hi, uint64_t *id);
+
+/* Completed requests, oldest first: up to cap into out, count in *n; waits up to
+   wait_ms for the first one. Their records stay available to ww_server_records. */
+int ww_server_poll(ww_server *s, ww_request_info *out, uint32_t cap, uint32_t *n, int wait_ms);
+
+/* The result records of a polled request (pad = slot index); releases them. */
+int ww_server_records(ww_server *s, uint64_t id, ww_result *out, uint32_t cap, uint32_t *n);
+
+int ww_server_get_stats(ww_server *s, ww_server_stats *out);
+
+/* Stop after the work already submitted, then free everything. */
+int ww_server_destroy(ww_server *s);
+
 /* Measurement support, no counterpart in the reference: the issue rate of
    independent 32x32->64 multiplies on every CU of `device` (kind 0: v_mad_u64_u32,
    kind 1: v_mul_hi_u32), in operations per second. Prices the roofline of the

@@ -107,6 +107,29 @@ constexpr size_t kMaxPendingEvents = 512;
 
 } // namespace
 
+int ww_set_error(int code, const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+/* Context internals the serving engine needs (same library). */
+int ww_ctx_internals(ww_ctx *c, int *device, const ww::DeepPrime **deep, uint32_t *ndeep,
+                     int *cus, int *tuned_blocks, uint32_t *tuned_claim)
+{
+    if (!c) return fail(WW_E_ARG, "null context");
+    *device = c->device;
+    *deep = c->d_deep;
+    *ndeep = c->ndeep;
+    *cus = c->cus;
+    *tuned_blocks = c->tuned_blocks;
+    *tuned_claim = c->tuned_claim;
+    return WW_OK;
+}
+
 extern "C" {
 
 const char *ww_last_error(void) { return g_err; }

@@ -37,8 +37,9 @@ inline uint64_t segments_of(uint64_t lo, uint64_t hi)
 
 int batch_blocks_per_cu();
 
-/* Clears the state and the slot sums, then runs the persistent kernel over all spans;
-   each wave claims `claim` consecutive segments at a time.
+/* Clears the state and the first `nslots` slot sums (0: none, so sums accumulate over
+   launches), then runs the persistent kernel over all spans; each wave claims `claim`
+   consecutive segments at a time.
    No allocation, no memset: safe to capture into a graph. */
 hipError_t launch_batch(const BatchSpan *spans, const uint64_t *prefix, uint32_t nspans,
                         BatchState *st, SlotSum *sums, uint32_t nslots, ww_result *res,

@@ -240,7 +240,8 @@ hipError_t launch_batch(const BatchSpan *spans, const u64 *prefix, u32 nspans, B
                         SlotSum *sums, u32 nslots, ww_result *res, u32 res_cap, int grid,
                         u32 claim, const DeepPrime *deep, u32 ndeep, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_batch_clear, dim3((nslots + 255) / 256), dim3(256), 0, s, st, sums, nslots);
+    hipLaunchKernelGGL(k_batch_clear, dim3(nslots > 256 ? (nslots + 255) / 256 : 1), dim3(256), 0, s,
+                       st, sums, nslots);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_batch, dim3(grid), dim3(64 * kBatchWaves), 0, s, spans, prefix, nspans, st,

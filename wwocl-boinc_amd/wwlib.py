@@ -66,6 +66,23 @@ class Tuning(ctypes.Structure):
                 "claim": self.claim, "ms": self.ms, "prps_per_s": self.prps_per_s}
 
 
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("slots", U32), ("tick_segments", U32), ("hbm_budget", U64),
+                ("bytes_per_request", U64), ("claim", U32), ("blocks_per_cu", ctypes.c_int32)]
+
+
+class RequestInfo(ctypes.Structure):
+    _fields_ = [("id", U64), ("lo", U64), ("hi", U64), ("checksum", U64), ("prp_count", U64),
+                ("t_submit", ctypes.c_double), ("t_admit", ctypes.c_double),
+                ("t_done", ctypes.c_double), ("nrecords", U32), ("pad", U32)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("t_start", ctypes.c_double), ("submitted", U64), ("completed", U64),
+                ("tested", U64), ("ticks", U64), ("segments", U64), ("pending", U64),
+                ("active", U64)]
+
+
 class DeviceInfo(ctypes.Structure):
     _fields_ = [("name", ctypes.c_char * 256), ("arch", ctypes.c_char * 64),
                 ("compute_units", ctypes.c_int), ("clock_khz", ctypes.c_int),
@@ -120,6 +137,15 @@ _SIGNATURES = [
                                    ctypes.POINTER(ctypes.c_int32), U32, ctypes.POINTER(U32), U32,
                                    ctypes.POINTER(Tuning), ctypes.POINTER(Tuning), U32,
                                    ctypes.POINTER(U32)]),
+    ("ww_server_create", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ServerConfig),
+                                        ctypes.POINTER(ctypes.c_void_p)]),
+    ("ww_server_submit", ctypes.c_int, [ctypes.c_void_p, U64, U64, ctypes.POINTER(U64)]),
+    ("ww_server_poll", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(RequestInfo), U32,
+                                      ctypes.POINTER(U32), ctypes.c_int]),
+    ("ww_server_records", ctypes.c_int, [ctypes.c_void_p, U64, ctypes.POINTER(Result), U32,
+                                         ctypes.POINTER(U32)]),
+    ("ww_server_get_stats", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ServerStats)]),
+    ("ww_server_destroy", ctypes.c_int, [ctypes.c_void_p]),
     ("ww_peak_mul32", ctypes.c_int, [ctypes.c_int, ctypes.c_int,
                                      ctypes.POINTER(ctypes.c_double)]),
     ("ww_copy_probe", ctypes.c_int, [ctypes.c_int, U64, ctypes.POINTER(ctypes.c_double)]),
@@ -456,3 +482,95 @@ def serving_workload(rate: float, duration_s: float, seed: int = 1, wmin: int = 
         w = int(2.0 ** (wmin + (wmax - wmin) * rng.unit()))
         s = (smin + rng.next() % (smax - smin + 1)) | 1
         out.append((t, s, s + w))
+
+
+class Server:
+    """Continuous-batching server (ww_server_*): submit ranges, poll completions."""
+
+    def __init__(self, ctx: Context, slots: int = 0, tick_segments: int = 0,
+                 hbm_budget: int = 0, bytes_per_request: int = 0, claim: int = 0,
+                 blocks_per_cu: int = -1):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        cfg = ServerConfig(slots, tick_segments, hbm_budget, bytes_per_request, claim,
+                           blocks_per_cu)
+        self.h = ctypes.c_void_p()
+        _check(self.lib.ww_server_create(ctx.h, ctypes.byref(cfg), ctypes.byref(self.h)))
+
+    def submit(self, lo: int, hi: int) -> int:
+        rid = U64()
+        _check(self.lib.ww_server_submit(self.h, lo, hi, ctypes.byref(rid)))
+        return rid.value
+
+    def poll(self, wait_ms: int = 0, cap: int = 1024):
+        """Completed requests: list of dicts (id, lo, hi, checksum, prp_count, times,
+        sorted records)."""
+        buf = (RequestInfo * cap)()
+        n = U32()
+        _check(self.lib.ww_server_poll(self.h, buf, cap, ctypes.byref(n), wait_ms))
+        out = []
+        for i in range(n.value):
+            r = buf[i]
+            recs = []
+            if r.nrecords:
+                rb = (Result * r.nrecords)()
+                m = U32()
+                _check(self.lib.ww_server_records(self.h, r.id, rb, r.nrecords, ctypes.byref(m)))
+                recs = sort_results([rb[k].key() for k in range(m.value)])
+            out.append({"id": r.id, "lo": r.lo, "hi": r.hi, "checksum": r.checksum,
+                        "prp_count": r.prp_count, "t_submit": r.t_submit,
+                        "t_admit": r.t_admit, "t_done": r.t_done, "records": recs})
+        return out
+
+    def stats(self) -> ServerStats:
+        st = ServerStats()
+        _check(self.lib.ww_server_get_stats(self.h, ctypes.byref(st)))
+        return st
+
+    def close(self):
+        if self.h:
+            self.lib.ww_server_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def serve(server: Server, requests, realtime: bool = True):
+    """Submit (t_arrival, lo, hi) requests at their arrival times (seconds from now) and
+    collect every completion. Returns (completions by id, {id: (lo, hi)}, wall seconds)."""
+    import time
+    t0 = time.perf_counter()
+    ids, done = {}, {}
+    i = 0
+    while i < len(requests) or len(done) < len(ids):
+        now = time.perf_counter() - t0
+        while i < len(requests) and (not realtime or requests[i][0] <= now):
+            _, lo, hi = requests[i]
+            ids[server.submit(lo, hi)] = (lo, hi)
+            i += 1
+        wait = 0
+        if i < len(requests) and realtime:
+            wait = max(0, int((requests[i][0] - (time.perf_counter() - t0)) * 1000))
+        for c in server.poll(wait_ms=min(wait, 20) if wait else 2):
+            done[c["id"]] = c
+    return done, ids, time.perf_counter() - t0
+
+
+def latency_summary(done):
+    lat = sorted(c["t_done"] - c["t_submit"] for c in done.values())
+    if not lat:
+        return {}
+    def pct(q):
+        return lat[min(len(lat) - 1, int(q * len(lat)))]
+    return {"p50_ms": pct(0.50) * 1e3, "p99_ms": pct(0.99) * 1e3, "max_ms": lat[-1] * 1e3,
+            "mean_ms": sum(lat) / len(lat) * 1e3}
