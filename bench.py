@@ -36,13 +36,19 @@ BATCH_TRAFFIC_BYTES_PER_PRP = 0.175
 
 def mul32_path(b: int) -> int:
     """Algorithmic 32x32->64 multiplies per tested prime of bit length b over the whole
-    path: Wieferich 21(b-3)+14, Wall-Sun-Sun 68(b-1)+14, PRP of the prime 10(b-2)."""
-    return 99 * b - 123
+    path. Counting a 64x64->128 product as 4, its low or high word alone as 3 or 4, and a
+    64-bit square as 3: the mod-p^2 Montgomery square costs 21 and the product 26.
+    Wieferich: b-2 squares + the final reduction (14) = 21(b-2)+14.
+    Wall-Sun-Sun: b-2 ladder steps of one square + one product, V_2 (21), the division by
+    5 (10) and the final reduction (14) = 47(b-2)+45.
+    The prime's own base-2 PRP test: at least b-4 Montgomery squarings of 10 each.
+    Composite PRP candidates and the sieve are not counted."""
+    return 78 * b - 117
 
 
 def mul32_tests(b: int) -> int:
-    """The same for the Wieferich + Wall-Sun-Sun kernel alone."""
-    return 89 * b - 103
+    """The same for the Wieferich + Wall-Sun-Sun kernel alone: 68(b-2)+59."""
+    return 68 * b - 77
 
 
 def cpu_baseline():
@@ -254,7 +260,7 @@ def main():
             "unit": "TMUL32/s", "frac": ach / peak,
             "traffic": pers["prps_per_step"] * BATCH_TRAFFIC_BYTES_PER_PRP,
             "traffic_algorithmic": 0,
-            "per_unit": f"{mul32_path(BITS)} MUL32 per tested prime (99b-123, b=64: sieve PRP "
+            "per_unit": f"{mul32_path(BITS)} MUL32 per tested prime (78b-117, b=64: PRP "
                         "+ Wieferich + Wall-Sun-Sun in one kernel)",
             "avg_launch_ms": pers["kernel_ms_per_step"], "launches": args.steps}
     if plain is not None:
@@ -265,7 +271,7 @@ def main():
             "peak": peak / 1e12, "unit": "TMUL32/s", "frac": tk["mul32_per_s"] / peak,
             "traffic": tk["entries_per_launch"] * TRAFFIC_BYTES_PER_ENTRY,
             "traffic_algorithmic": tk["entries_per_launch"] * 8,
-            "per_unit": f"{mul32_tests(BITS)} MUL32 per tested prime (89b-103, b=64)",
+            "per_unit": f"{mul32_tests(BITS)} MUL32 per tested prime (68b-77, b=64)",
             "avg_launch_ms": tk["avg_launch_ms"], "launches": tk["launches"]}
         out["plain"] = {k: plain[k] for k in ("value", "ms_per_step", "prps_per_step",
                                               "step_checksum", "kernel_ms_per_step")}

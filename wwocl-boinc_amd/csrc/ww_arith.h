@@ -13,8 +13,8 @@
  * with factor 2^64 -- the representation of the reference (wwcpu.cpp:104-250;
  * Dorais & Klyve, J. Integer Seq. 14 (2011)). All outputs are exact, so any exact
  * evaluation order yields the reference's values; the Wall-Sun-Sun ladder here uses
- * Lucas V_n (one square + one product per exponent bit) instead of the reference's
- * Fibonacci pair (two squares + one product, wallsunsun.cl:274-294).
+ * the Lucas sequence V_n(3, 1) (one square + one product per exponent bit) instead of the
+ * reference's Fibonacci pair (two squares + one product, wallsunsun.cl:274-294).
  */
 #ifndef WW_ARITH_H
 #define WW_ARITH_H
@@ -88,8 +88,17 @@ WW_HD u128 sqr_wide(u64 a)
 }
 
 /* ---------------------------------------------------------------------------
- * mod n (64-bit) helpers
+ * mod n (64-bit) helpers. Borrows and carries come from the overflow builtins so the
+ * backend reuses the carry-out of v_sub_co/v_add_co instead of comparing again.
  * ------------------------------------------------------------------------- */
+
+/* x - y, and whether it borrowed. */
+WW_HD u64 sub_b(u64 x, u64 y, bool *borrow)
+{
+    u64 r;
+    *borrow = __builtin_sub_overflow(x, y, &r);
+    return r;
+}
 
 /* x + y mod p for x, y < p (prime.h:34-43). */
 WW_HD u64 add_mod(u64 x, u64 y, u64 p)
@@ -101,8 +110,9 @@ WW_HD u64 add_mod(u64 x, u64 y, u64 p)
 /* x - y mod p for x < p, y <= p. */
 WW_HD u64 sub_mod(u64 x, u64 y, u64 p)
 {
-    u64 r = x - y;
-    return x < y ? r + p : r;
+    bool b;
+    u64 r = sub_b(x, y, &b);
+    return b ? r + p : r;
 }
 
 /* p^-1 mod 2^64 for odd p: Newton from the 5-bit seed (3p) xor 2, three 32-bit steps
@@ -269,14 +279,27 @@ WW_HD m2 m2_redc(u128 t, u64 c_lo, u64 c_hi, u64 c_top, u64 p, u64 q)
     return r;
 }
 
-/* x^2 / 2^64 mod p^2 (wwcpu.cpp:140-167). */
+/* x^2 / 2^64 mod p^2 (wwcpu.cpp:140-167). The cross term 2 x0 x1 enters only through
+   its REDC into the high digit, so only its value mod p matters: it is formed as
+   x0 * (2 x1 mod p) + u0 <= (p-1)^2 + 2^64 - 1 < 2^128, whose high word is already below
+   p. That drops the 129-bit sum and its conditional subtraction of the general product. */
 WW_HD m2 m2_sqr(m2 x, u64 p, u64 q)
 {
     u128 t = sqr_wide(x.d0);
-    u128 c = mul_wide(x.d0, x.d1);
-    u64 top = c.hi >> 63;
-    u64 hi2 = (c.hi << 1) | (c.lo >> 63);
-    return m2_redc(t, c.lo << 1, hi2, top, p, q);
+    u64 u0 = t.lo * q;
+    u64 v1 = mul_hi(p, u0);
+    u128 c = mul_wide(x.d0, add_mod(x.d1, x.d1, p));
+    u64 tlo;
+    bool cy = __builtin_add_overflow(c.lo, u0, &tlo);
+    u64 t1p = c.hi + (u64)cy; /* < p */
+    u64 up0 = tlo * q;
+    u64 v1p = mul_hi(p, up0);
+    bool b;
+    u64 z0 = sub_b(t.hi, v1, &b);
+    m2 r;
+    r.d0 = b ? z0 + p : z0;
+    r.d1 = sub_mod(t1p, v1p + (u64)b, p);
+    return r;
 }
 
 /* x*y / 2^64 mod p^2 (wwcpu.cpp:210-237). */
@@ -358,36 +381,36 @@ WW_HD m2 wieferich_residue(u64 p, u64 q, m2 one)
     return m2_get(a, p, q);
 }
 
-/* F_{p-(p/5)} mod p^2 as digits (c20, c21) (wallsunsun.cl:242-300, wwcpu.cpp:311-372),
-   by the Lucas ladder on (V_k, V_{k+1}) with P = 1, Q = -1:
-     V_2j = V_j^2 - 2(-1)^j,   V_2k+1 = V_k V_k+1 - (-1)^k,
-   and 5 F_m = 2 V_{m+1} - V_m at the end. */
+/* F_{p-(p/5)} mod p^2 as digits (c20, c21) (wallsunsun.cl:242-300, wwcpu.cpp:311-372).
+   m = p - (p/5) is even, and F_{2n} = U_n(3, 1), the Lucas sequence with P = 3, Q = 1
+   (its roots are the squares of the golden ratio and its conjugate). Its companion V
+   has constant-coefficient doubling formulas,
+     V_2k = V_k^2 - 2,   V_2k+1 = V_k V_k+1 - 3,
+   so the ladder on (V_k, V_k+1) over the bits of n = m/2 costs one square and one product
+   per bit with no sign-dependent constants, and 5 U_n = 2 V_{n+1} - 3 V_n at the end. */
 WW_HD m2 wallsunsun_residue(u64 p, u64 q, m2 one)
 {
     u32 pm5 = mod5(p);
-    u64 m = (pm5 == 2 || pm5 == 3) ? p + 1 : p - 1;
+    u64 n = ((pm5 == 2 || pm5 == 3) ? p + 1 : p - 1) >> 1;
     m2 two = m2_add(one, one, p);
-    m2 mone = m2_neg(one, p), mtwo = m2_neg(two, p);
-    m2 vk = one;                   /* V_1 = 1 */
-    m2 vk1 = m2_add(two, one, p);  /* V_2 = 3 */
-    bool kodd = true;
-    for (int i = 62 - clz64(m); i >= 0; --i) {
-        bool bit = (m >> i) & 1;
-        m2 pr = m2_mul(vk, vk1, p, q);
-        pr = m2_add(pr, kodd ? one : mone, p);            /* V_{2k+1} */
+    m2 three = m2_add(two, one, p);
+    m2 vk = three;                       /* V_1 = 3 */
+    m2 vk1 = m2_sub(m2_sqr(three, p, q), two, p); /* V_2 = 7 */
+    for (int i = 62 - clz64(n); i >= 0; --i) {
+        bool bit = (n >> i) & 1;
+        m2 pr = m2_sub(m2_mul(vk, vk1, p, q), three, p); /* V_{2k+1} */
         m2 sin;
         sin.d0 = bit ? vk1.d0 : vk.d0;
         sin.d1 = bit ? vk1.d1 : vk.d1;
-        bool jodd = bit ? !kodd : kodd;
-        m2 sq = m2_sqr(sin, p, q);
-        sq = m2_add(sq, jodd ? two : mtwo, p);             /* V_{2j} */
+        m2 sq = m2_sub(m2_sqr(sin, p, q), two, p);       /* V_{2k} or V_{2k+2} */
         vk.d0 = bit ? pr.d0 : sq.d0;
         vk.d1 = bit ? pr.d1 : sq.d1;
         vk1.d0 = bit ? sq.d0 : pr.d0;
         vk1.d1 = bit ? sq.d1 : pr.d1;
-        kodd = bit;
     }
-    m2 f5 = m2_sub(m2_add(vk1, vk1, p), vk, p);
+    m2 two_v1 = m2_add(vk1, vk1, p);
+    m2 three_v0 = m2_add(m2_add(vk, vk, p), vk, p);
+    m2 f5 = m2_sub(two_v1, three_v0, p);
     return m2_get(m2_div5(f5, p), p, q);
 }
 
