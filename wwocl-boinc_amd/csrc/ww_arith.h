@@ -28,6 +28,16 @@
 #define WW_HD static inline
 #endif
 
+/* WW_LOOP_LOCAL(x): an empty asm that makes x opaque at the top of a loop body. Loop-invariant
+   64-bit operands otherwise have their 32-bit halves hoisted as zero-extended 64-bit values,
+   and instruction selection, which sees one block at a time, then multiplies by the known-
+   zero high halves too (8 v_mad_u64_u32 for a 4-MUL32 high product). */
+#if defined(__HIP_DEVICE_COMPILE__)
+#define WW_LOOP_LOCAL(x) asm volatile("" : "+v"(x))
+#else
+#define WW_LOOP_LOCAL(x) ((void)0)
+#endif
+
 namespace ww {
 
 typedef uint64_t u64;
@@ -80,6 +90,50 @@ WW_HD u128 sqr_wide(u64 a)
     u64 t0 = (u64)a0 * a0;
     u64 t1 = mad32(a0, a1, t0 >> 32);     /* a0a1 + hi(a0^2)        */
     u64 t2 = mad32(a0, a1, (u32)t1);      /* a0a1 + lo(t1)          */
+    u64 t3 = mad32(a1, a1, (t1 >> 32) + (t2 >> 32));
+    u128 r;
+    r.lo = (u64)(u32)t0 | (t2 << 32);
+    r.hi = t3;
+    return r;
+}
+
+/* a*b + c + d, exact when the result is below 2^128. The addends enter the schoolbook
+   columns as 32-bit halves, two per MAD at most, so no partial sum exceeds
+   (2^32-1)^2 + 2(2^32-1) = 2^64-1 and no separate carry chain is needed. */
+WW_HD u128 mul_wide_add2(u64 a, u64 b, u64 c, u64 d)
+{
+    u32 a0 = (u32)a, a1 = (u32)(a >> 32), b0 = (u32)b, b1 = (u32)(b >> 32);
+    u64 t0 = mad32(a0, b0, (u64)(u32)c + (u32)d);
+    u64 t1 = mad32(a0, b1, (t0 >> 32) + (c >> 32));
+    u64 t2 = mad32(a1, b0, (u64)(u32)t1 + (d >> 32));
+    u64 t3 = mad32(a1, b1, (t1 >> 32) + (t2 >> 32));
+    u128 r;
+    r.lo = (u64)(u32)t0 | (t2 << 32);
+    r.hi = t3;
+    return r;
+}
+
+/* a*b + c (always below 2^128). */
+WW_HD u128 mul_wide_add(u64 a, u64 b, u64 c)
+{
+    u32 a0 = (u32)a, a1 = (u32)(a >> 32), b0 = (u32)b, b1 = (u32)(b >> 32);
+    u64 t0 = mad32(a0, b0, (u32)c);
+    u64 t1 = mad32(a0, b1, (t0 >> 32) + (c >> 32));
+    u64 t2 = mad32(a1, b0, (u32)t1);
+    u64 t3 = mad32(a1, b1, (t1 >> 32) + (t2 >> 32));
+    u128 r;
+    r.lo = (u64)(u32)t0 | (t2 << 32);
+    r.hi = t3;
+    return r;
+}
+
+/* a^2 + c (always below 2^128). */
+WW_HD u128 sqr_wide_add(u64 a, u64 c)
+{
+    u32 a0 = (u32)a, a1 = (u32)(a >> 32);
+    u64 t0 = mad32(a0, a0, (u32)c);
+    u64 t1 = mad32(a0, a1, (t0 >> 32) + (c >> 32));
+    u64 t2 = mad32(a0, a1, (u32)t1);
     u64 t3 = mad32(a1, a1, (t1 >> 32) + (t2 >> 32));
     u128 r;
     r.lo = (u64)(u32)t0 | (t2 << 32);
@@ -317,6 +371,52 @@ WW_HD m2 m2_mul(m2 x, m2 y, u64 p, u64 q)
     return m2_redc(t, lo, hi, top, p, q);
 }
 
+/* REDC(x^2 + k) for a constant residue k = k0 + p*k1 (digits in [0, p)): with
+   k = -c*R^2 mod p^2 this is the Montgomery square minus c, so the ladder's "- 2" costs
+   two 32-bit additions instead of a mod p^2 subtraction. Bounds: x0^2 + k0 < p^2 and
+   x0*(2x1 mod p) + u0 + k1 < p*2^64, so both high words are below p as in m2_sqr. */
+WW_HD m2 m2_sqr_add(m2 x, m2 k, u64 p, u64 q)
+{
+    u128 t = sqr_wide_add(x.d0, k.d0);
+    u64 u0 = t.lo * q;
+    u64 v1 = mul_hi(p, u0);
+    u128 c = mul_wide_add2(x.d0, add_mod(x.d1, x.d1, p), u0, k.d1);
+    u64 up0 = c.lo * q;
+    u64 v1p = mul_hi(p, up0);
+    bool b;
+    u64 z0 = sub_b(t.hi, v1, &b);
+    m2 r;
+    r.d0 = b ? z0 + p : z0;
+    r.d1 = sub_mod(c.hi, v1p + (u64)b, p);
+    return r;
+}
+
+/* REDC(x*y + k), the Montgomery product plus a constant as in m2_sqr_add. The cross sum
+   x0*y1 + u0 + x1*y0 + k1 is below 2p*2^64 (at most 2p^2 - 3p + 2^64), a 129-bit value
+   whose high part one conditional subtraction of p brings below p. */
+WW_HD m2 m2_mul_add(m2 x, m2 y, m2 k, u64 p, u64 q)
+{
+    u128 t = mul_wide_add(x.d0, y.d0, k.d0);
+    u64 u0 = t.lo * q;
+    u64 v1 = mul_hi(p, u0);
+    u128 a = mul_wide_add(x.d0, y.d1, u0);
+    u128 e = mul_wide_add(x.d1, y.d0, k.d1);
+    u64 lo, hi, hi2;
+    bool c1 = __builtin_add_overflow(a.lo, e.lo, &lo);
+    bool c2 = __builtin_add_overflow(a.hi, e.hi, &hi);
+    bool c3 = __builtin_add_overflow(hi, (u64)c1, &hi2);
+    bool big = c2 | c3 | (hi2 >= p);
+    u64 t1p = big ? hi2 - p : hi2;
+    u64 up0 = lo * q;
+    u64 v1p = mul_hi(p, up0);
+    bool b;
+    u64 z0 = sub_b(t.hi, v1, &b);
+    m2 r;
+    r.d0 = b ? z0 + p : z0;
+    r.d1 = sub_mod(t1p, v1p + (u64)b, p);
+    return r;
+}
+
 /* Montgomery form back to ordinary digits (REDC, wwcpu.cpp:189-206). */
 WW_HD m2 m2_get(m2 x, u64 p, u64 q)
 {
@@ -387,27 +487,42 @@ WW_HD m2 wieferich_residue(u64 p, u64 q, m2 one)
    has constant-coefficient doubling formulas,
      V_2k = V_k^2 - 2,   V_2k+1 = V_k V_k+1 - 3,
    so the ladder on (V_k, V_k+1) over the bits of n = m/2 costs one square and one product
-   per bit with no sign-dependent constants, and 5 U_n = 2 V_{n+1} - 3 V_n at the end. */
+   per bit with no sign-dependent constants, and 5 U_n = 2 V_{n+1} - 3 V_n at the end.
+   The pair lives in two fixed slots (x, y): the product is symmetric and goes to x, the
+   square of the operand the bit picks goes to y, so after bit i slot x holds V_k exactly
+   when bit i is 1. The square's operand is therefore y iff bits i and i+1 of n are equal,
+   and only that one operand is selected per bit. */
 WW_HD m2 wallsunsun_residue(u64 p, u64 q, m2 one)
 {
     u32 pm5 = mod5(p);
     u64 n = ((pm5 == 2 || pm5 == 3) ? p + 1 : p - 1) >> 1;
+    u64 same = ~(n ^ (n >> 1)); /* bit i: bits i and i+1 of n are equal */
     m2 two = m2_add(one, one, p);
     m2 three = m2_add(two, one, p);
-    m2 vk = three;                       /* V_1 = 3 */
-    m2 vk1 = m2_sub(m2_sqr(three, p, q), two, p); /* V_2 = 7 */
+    /* R^2 mod p^2 is the Montgomery form of 2^64: six squarings of 2 */
+    m2 r2 = two;
+    for (int j = 0; j < 6; ++j) r2 = m2_sqr(r2, p, q);
+    m2 r2x2 = m2_add(r2, r2, p);
+    m2 minus2 = m2_neg(r2x2, p);                    /* -2 R^2 */
+    m2 minus3 = m2_neg(m2_add(r2x2, r2, p), p);     /* -3 R^2 */
+    m2 x = three;                                   /* V_1 = 3, slot x holds V_k */
+    m2 y = m2_sqr_add(three, minus2, p, q);         /* V_2 = 7 */
     for (int i = 62 - clz64(n); i >= 0; --i) {
-        bool bit = (n >> i) & 1;
-        m2 pr = m2_sub(m2_mul(vk, vk1, p, q), three, p); /* V_{2k+1} */
+        WW_LOOP_LOCAL(p);
+        bool use_y = (same >> i) & 1;
         m2 sin;
-        sin.d0 = bit ? vk1.d0 : vk.d0;
-        sin.d1 = bit ? vk1.d1 : vk.d1;
-        m2 sq = m2_sub(m2_sqr(sin, p, q), two, p);       /* V_{2k} or V_{2k+2} */
-        vk.d0 = bit ? pr.d0 : sq.d0;
-        vk.d1 = bit ? pr.d1 : sq.d1;
-        vk1.d0 = bit ? sq.d0 : pr.d0;
-        vk1.d1 = bit ? sq.d1 : pr.d1;
+        sin.d0 = use_y ? y.d0 : x.d0;
+        sin.d1 = use_y ? y.d1 : x.d1;
+        x = m2_mul_add(x, y, minus3, p, q); /* V_{2k+1} = V_k V_{k+1} - 3 */
+        y = m2_sqr_add(sin, minus2, p, q);  /* V_{2k} or V_{2k+2} = V^2 - 2 */
     }
+    /* slot x holds V_n iff bit 0 of n is 1 (or the loop did not run, n = 1) */
+    bool odd = n & 1;
+    m2 vk, vk1;
+    vk.d0 = odd ? x.d0 : y.d0;
+    vk.d1 = odd ? x.d1 : y.d1;
+    vk1.d0 = odd ? y.d0 : x.d0;
+    vk1.d1 = odd ? y.d1 : x.d1;
     m2 two_v1 = m2_add(vk1, vk1, p);
     m2 three_v0 = m2_add(m2_add(vk, vk, p), vk, p);
     m2 f5 = m2_sub(two_v1, three_v0, p);
