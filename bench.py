@@ -1,11 +1,16 @@
-"""Throughput of the MI355X search path: tested primes/s at p ~ 2^63, plain form.
+"""Throughput of the MI355X search path: tested primes/s at p ~ 2^63.
 
-One step = the whole range [2^63, 2^63 + 2^33) through the HIP path (per 2^32-wide
-chunk: clear the list, sieve + PRP, Wieferich + Wall-Sun-Sun), inputs generated on the
-device, the checksum accumulated on the device. Prints one JSON line.
+One step = the whole range [2^63, 2^63 + 2^33) (196,705,536 tested primes) through the HIP
+path, inputs generated on the device, the checksum accumulated on the device. The headline
+`value` is the persistent form (one graph-launched k_batch: sieve, PRP, Wieferich and
+Wall-Sun-Sun in one kernel); the plain form (per 2^32-wide chunk: clear the list, sieve +
+PRP, then the test kernel) runs beside it and must give the same step checksum. Also: the
+4096-range batched workload, the autotuner, the serving form's latency, and the reference's
+own CPU path on a bounded sample. Prints one JSON line.
 
     python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--no-cpu] [--no-batch]
-                    [--serve-seconds 10] [--serve-rates 20,50,100] [--no-tune]
+                    [--form both|persistent|plain] [--serve-seconds 10]
+                    [--serve-rates 20,50,100] [--no-tune] [--copy-probe BYTES]
 """
 from __future__ import annotations
 
@@ -30,8 +35,8 @@ BITS = 64  # every p in the range has bit length 64
 TRAFFIC_BYTES_PER_ENTRY = 8.00
 # k_batch keeps candidates and PRPs on chip (LDS queues); its HBM traffic per tested
 # prime is the span table lookups and the cursor/slot atomics only (each atomic reads as a
-# 32-byte write), measured the same way: 0.3 MB fetched + 34 MB written per step.
-BATCH_TRAFFIC_BYTES_PER_PRP = 0.175
+# 32-byte write), measured the same way: 3.0 MB fetched + 7.9 MB written per step.
+BATCH_TRAFFIC_BYTES_PER_PRP = 0.0555
 
 
 def mul32_path(b: int) -> int:
@@ -76,7 +81,7 @@ def cpu_baseline():
 def run_plain(ctx, steps, warmup):
     """Plain form: per 2^32 chunk, clear the list, sieve + PRP, then the test kernel."""
     step_checksum = None
-    for _ in range(warmup):
+    for _ in range(max(warmup, 1)):  # at least one untimed step gives the step checksum
         ctx.clear_results()
         ctx.search_range(LO, HI, CHUNK)
         ctx.finish()
