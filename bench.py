@@ -40,25 +40,33 @@ BATCH_TRAFFIC_BYTES_PER_PRP = 0.0555
 
 
 def mul32_path(b: int) -> int:
-    """Algorithmic 32x32->64 multiplies per tested prime of bit length b over the whole
-    path. Counting a 64x64->128 product as 4, its low or high word alone as 3 or 4, and a
-    64-bit square as 3: the mod-p^2 Montgomery square costs 21 and the product 26.
-    Wieferich: b-2 squares + the final reduction (14) = 21(b-2)+14.
-    Wall-Sun-Sun: b-2 ladder steps of one square + one product, V_2 (21), the division by
-    5 (10) and the final reduction (14) = 47(b-2)+45.
-    The prime's own base-2 PRP test: at least b-4 Montgomery squarings of 10 each.
-    Composite PRP candidates and the sieve are not counted."""
-    return 78 * b - 117
+    """Algorithmic work per tested prime of bit length b, in 32x32->64 multiplies (MUL32),
+    as the reference's algorithm spends it (the roofline's per-unit figure): Wieferich
+    21(b-3)+14, Wall-Sun-Sun 68(b-1)+14 (two squares and one product per bit), the prime's
+    own PRP 10(b-2); A(b) = 99b - 123. Composite candidates and the sieve are not counted."""
+    return 99 * b - 123
 
 
 def mul32_tests(b: int) -> int:
-    """The same for the Wieferich + Wall-Sun-Sun kernel alone: 68(b-2)+59."""
-    return 68 * b - 77
+    """The same for the Wieferich + Wall-Sun-Sun kernel alone: 89b - 103."""
+    return 89 * b - 103
+
+
+def mul32_path_executed(b: int) -> int:
+    """MUL32 this implementation's algorithm needs per tested prime, counted the same way
+    (64x64->128 = 4, a low or high half = 3 or 4, a 64-bit square = 3; mod-p^2 Montgomery
+    square 21, product 26): Wieferich 21(b-2)+14; Wall-Sun-Sun on V(3,1), one square and one
+    product per bit, 47(b-2)+45; the prime's own PRP at least 10(b-4). = 78b - 117."""
+    return 78 * b - 117
+
+
+CPU_SLICE = 1 << 27  # 16 slices of 2^27: about 12 s of reference CPU work on 16 threads
 
 
 def cpu_baseline():
-    """The reference's own CPU path (compiled into oracle/_ref) on 16 sampled 2^22-wide
-    slices of the workload, one slice range per thread."""
+    """The reference's own CPU path (compiled into oracle/_ref) on 16 sampled 2^27-wide
+    slices of the workload, [2^63 + k*2^29, +2^27) for k = 0..15, each slice split over
+    the threads."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
 
@@ -69,13 +77,13 @@ def cpu_baseline():
     for k in range(16):
         lo = LO + k * (1 << 29)
         if kind == "reference":
-            prp = oracle.ref_range(lo, lo + (1 << 22), threads)[0]
+            prp = oracle.ref_range(lo, lo + CPU_SLICE, threads)[0]
         else:
-            prp = oracle.range_summary(lo, lo + (1 << 22), threads)[0]
+            prp = oracle.range_summary(lo, lo + CPU_SLICE, threads)[0]
         total += prp
     dt = time.perf_counter() - t0
     return {"value": total / dt, "unit": "tested primes/s", "cores": threads, "kind": kind,
-            "sample": f"16 slices [2^63 + k*2^29, +2^22), k=0..15: {total} PRPs in {dt:.2f} s"}
+            "sample": f"16 slices [2^63 + k*2^29, +2^27), k=0..15: {total} PRPs in {dt:.2f} s"}
 
 
 def run_plain(ctx, steps, warmup):
@@ -265,8 +273,11 @@ def main():
             "unit": "TMUL32/s", "frac": ach / peak,
             "traffic": pers["prps_per_step"] * BATCH_TRAFFIC_BYTES_PER_PRP,
             "traffic_algorithmic": 0,
-            "per_unit": f"{mul32_path(BITS)} MUL32 per tested prime (78b-117, b=64: PRP "
-                        "+ Wieferich + Wall-Sun-Sun in one kernel)",
+            "per_unit": f"{mul32_path(BITS)} MUL32 per tested prime (A(b) = 99b-123 at b=64: "
+                        "PRP + Wieferich + Wall-Sun-Sun in one kernel, counted as the "
+                        "reference's algorithm spends them)",
+            "executed_per_unit": f"{mul32_path_executed(BITS)} MUL32 per tested prime "
+                                 "(78b-117: this build's ladders)",
             "avg_launch_ms": pers["kernel_ms_per_step"], "launches": args.steps}
     if plain is not None:
         tk = plain["tests_kernel"]
@@ -276,7 +287,7 @@ def main():
             "peak": peak / 1e12, "unit": "TMUL32/s", "frac": tk["mul32_per_s"] / peak,
             "traffic": tk["entries_per_launch"] * TRAFFIC_BYTES_PER_ENTRY,
             "traffic_algorithmic": tk["entries_per_launch"] * 8,
-            "per_unit": f"{mul32_tests(BITS)} MUL32 per tested prime (68b-77, b=64)",
+            "per_unit": f"{mul32_tests(BITS)} MUL32 per tested prime (89b-103, b=64)",
             "avg_launch_ms": tk["avg_launch_ms"], "launches": tk["launches"]}
         out["plain"] = {k: plain[k] for k in ("value", "ms_per_step", "prps_per_step",
                                               "step_checksum", "kernel_ms_per_step")}
