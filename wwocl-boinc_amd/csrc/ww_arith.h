@@ -19,6 +19,7 @@
 #ifndef WW_ARITH_H
 #define WW_ARITH_H
 
+#include <math.h>
 #include <stdint.h>
 
 #if defined(__HIPCC__)
@@ -227,10 +228,10 @@ WW_HD u64 mont_sqr(u64 a, u64 n, u64 q)
     u128 t = sqr_wide(a);
     u64 m = t.lo * q;
     u64 mp = mul_hi(m, n);
-    u64 r = t.hi - mp;
-    return t.hi < mp ? r + n : r;
+    bool b;
+    u64 r = sub_b(t.hi, mp, &b);
+    return b ? r + n : r;
 }
-
 /* Montgomery product a*b/2^64 mod n (prime.h:15-31). */
 WW_HD u64 mont_mul(u64 a, u64 b, u64 n, u64 q)
 {
@@ -241,26 +242,66 @@ WW_HD u64 mont_mul(u64 a, u64 b, u64 n, u64 q)
     return t.hi < mp ? r + n : r;
 }
 
-/* Base-2 strong probable-prime test of odd n > 1 (getsegprimes.cl:118-164,
-   prime.h:66-112). one = 2^64 mod n. */
-WW_HD bool spsp2(u64 n, u64 q, u64 one)
+/* y * 2^v mod n for y < n and v <= 31. The quotient floor(y 2^v / n) < 2^31 is estimated
+   in double precision from ninv ~ 1/n: the estimate's relative error is below 2^-51, so
+   its absolute error is below 2^-20 and the estimate is the quotient or one off; the
+   remainder x - qe*n then lies in [-n, 2n) and one signed correction makes it exact. */
+WW_HD u64 mulpow2_mod(u64 y, u32 v, u64 n, double ninv)
 {
-    u64 nm1 = n - 1;
-    int t = ctz64(nm1);
-    u64 d = nm1 >> t;
+    u64 lo = y << v;
+    u32 hi = (u32)((y >> 1) >> (63 - v)); /* y 2^v = hi:lo, hi < 2^v */
+    u32 qe = (u32)ldexp((double)y * ninv, (int)v);
+    u64 m0 = (u64)qe * (u32)n;
+    u64 m1 = mad32(qe, (u32)(n >> 32), m0 >> 32);
+    u64 qn_lo = (m1 << 32) | (u32)m0;
+    u32 qn_hi = (u32)(m1 >> 32);
+    bool b;
+    u64 r = sub_b(lo, qn_lo, &b);
+    int32_t rh = (int32_t)(hi - qn_hi - (u32)b); /* -1, 0 or 1 */
+    bool big = (rh > 0) | (r >= n);
+    u64 adj = rh < 0 ? n : (big ? 0 - n : 0);
+    return r + adj;
+}
+
+/* Base-2 strong probable-prime test of odd n > 1 (getsegprimes.cl:118-164,
+   prime.h:66-112), q = n^-1 mod 2^64, one = 2^64 mod n.
+   With n - 1 = d 2^t the test asks for x_0 = 2^d = 1 or x_s = x_0^(2^s) = -1 for some
+   s < t. The bits of n - 1 from `top` down to `tmax` are taken in windows of 5 (five
+   squarings, then one multiplication by 2^v), which yields 2^((n-1) >> tmax); the bits
+   below tmax follow one by one, and x_0 .. x_{t-1} appear there (or x_0 at the end of
+   the windows when t = tmax). Any top >= the leading bit of n - 1 and any tmax in
+   [t, top] give the same answer, so a wave can pass common (uniform) values and run one
+   loop for all lanes. */
+WW_HD bool spsp2_window(u64 n, u64 q, u64 one, int top, int tmax)
+{
+    const int W = 5;
+    u64 e = n - 1;
+    int t = ctz64(e);
+    double ninv = 1.0 / (double)n;
     u64 mone = n - one;
-    u64 a = add_mod(one, one, n); /* Montgomery 2: the leading bit of d */
-    for (int i = 62 - clz64(d); i >= 0; --i) {
-        a = mont_sqr(a, n, q);
-        u64 dbl = add_mod(a, a, n);
-        a = ((d >> i) & 1) ? dbl : a;
+    int len = top - tmax + 1;
+    int first = len - ((len - 1) / W) * W; /* 1..W bits */
+    int pos = top - first + 1;
+    u64 a = mulpow2_mod(one, (u32)(e >> pos) & ((1u << first) - 1u), n, ninv);
+    while (pos > tmax) {
+        pos -= W;
+        for (int k = 0; k < W; ++k) a = mont_sqr(a, n, q);
+        a = mulpow2_mod(a, (u32)(e >> pos) & 31u, n, ninv);
     }
-    bool ok = (a == one) | (a == mone);
-    for (int s = 1; s < t && !ok; ++s) {
+    bool ok = (t == tmax) & ((a == one) | (a == mone));
+    for (int i = tmax - 1; i >= 1; --i) {
         a = mont_sqr(a, n, q);
-        ok = (a == mone);
+        if ((e >> i) & 1) a = add_mod(a, a, n);
+        ok |= ((i == t) & ((a == one) | (a == mone))) | ((i < t) & (a == mone));
     }
     return ok;
+}
+
+/* The same with the exact bounds of this n. */
+WW_HD bool spsp2(u64 n, u64 q, u64 one)
+{
+    u64 e = n - 1;
+    return spsp2_window(n, q, one, 63 - clz64(e), ctz64(e));
 }
 
 /* ---------------------------------------------------------------------------

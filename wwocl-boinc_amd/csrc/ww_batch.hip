@@ -144,12 +144,8 @@ __global__ __launch_bounds__(64 * kBatchWaves) __attribute__((amdgpu_waves_per_e
             /* strong PRP, 64 candidates per round; survivors join the test queue */
             const u32 j = j0 + lane;
             const bool in = j < ncand;
-            u64 n = 0;
-            bool prp = false;
-            if (in) {
-                n = group + 2ull * wc[j];
-                prp = spsp2(n, inv64(n), mont_one(n));
-            }
+            const u64 n = group + 2ull * wc[in ? j : j0];
+            const bool prp = spsp2_wave(n, in);
             const u64 bal = __ballot(prp);
             if (prp) {
                 const u32 at = qn + (u32)__popcll(bal & ((1ull << lane) - 1));

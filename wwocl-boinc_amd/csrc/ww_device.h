@@ -77,6 +77,26 @@ __device__ __forceinline__ u64 sieve_word(u64 group, u32 lane_off, u64 hi)
  * never marked. The strong PRP test still decides membership, so the list is unchanged.
  * ------------------------------------------------------------------------- */
 
+/* Largest x over the 64 lanes of the wave, wave-uniform. Every lane must take part. */
+__device__ __forceinline__ int wave_max(int x)
+{
+    for (int off = 32; off >= 1; off >>= 1) x = max(x, __shfl_xor(x, off, 64));
+    return __builtin_amdgcn_readfirstlane(x);
+}
+
+/* Base-2 strong PRP of one candidate per lane (act false: the lane only takes part in
+   the wave's common loop bounds, the largest leading bit and trailing-zero count of
+   n - 1, so that every lane runs the same loops). Called by all 64 lanes. */
+__device__ __forceinline__ bool spsp2_wave(u64 n, bool act)
+{
+    const u64 e = act ? n - 1 : 2;
+    const int top = wave_max(63 - clz64(e));
+    const int tmax = wave_max(ctz64(e));
+    bool prp = false;
+    if (act) prp = spsp2_window(n, inv64(n), mont_one(n), top, tmax);
+    return prp;
+}
+
 /* x mod m for x < 2^53 with a double reciprocal: the quotient estimate is off by at
    most one and is corrected exactly. */
 __device__ __forceinline__ u32 mod_small(u64 x, u32 m, double inv)

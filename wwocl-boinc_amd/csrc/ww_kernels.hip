@@ -67,12 +67,7 @@ __global__ __launch_bounds__(kSieveThreads) void k_sieve_prp(u64 base, u64 hi, u
         u32 j = j0 + lane;
         bool act = j < ncand;
         u32 idx = act ? wc[j] : 0;
-        bool prp = false;
-        if (act) {
-            u64 n = wave_base + 2ull * idx;
-            u64 q = inv64(n);
-            prp = spsp2(n, q, mont_one(n));
-        }
+        const bool prp = spsp2_wave(wave_base + 2ull * idx, act);
         u64 bal = __ballot(prp);
         if (prp) {
             u32 below = __popcll(bal & ((1ull << lane) - 1));
