@@ -263,6 +263,28 @@ WW_HD u64 mulpow2_mod(u64 y, u32 v, u64 n, double ninv)
     return r + adj;
 }
 
+/* y * 2^v + c = quot * n + r for y < n, v <= 31 and c < 2^v; returns r and the quotient
+   (below 2^v). The estimate is that of mulpow2_mod with c folded into one rounding. */
+WW_HD u64 mulpow2_add_qr(u64 y, u32 v, u32 c, u64 n, double ninv, u32 *quot)
+{
+    u64 lo;
+    bool cy = __builtin_add_overflow(y << v, (u64)c, &lo);
+    u32 hi = (u32)((y >> 1) >> (63 - v)) + (u32)cy;
+    u32 qe = (u32)((ldexp((double)y, (int)v) + (double)c) * ninv);
+    u64 m0 = (u64)qe * (u32)n;
+    u64 m1 = mad32(qe, (u32)(n >> 32), m0 >> 32);
+    u64 qn_lo = (m1 << 32) | (u32)m0;
+    u32 qn_hi = (u32)(m1 >> 32);
+    bool b;
+    u64 r = sub_b(lo, qn_lo, &b);
+    int32_t rh = (int32_t)(hi - qn_hi - (u32)b); /* -1, 0 or 1 */
+    bool neg = rh < 0;
+    bool big = !neg & ((rh > 0) | (r >= n)); /* a negative r also reads as >= n */
+    *quot = qe + (u32)big - (u32)neg;
+    u64 adj = neg ? n : (big ? 0 - n : 0);
+    return r + adj;
+}
+
 /* Base-2 strong probable-prime test of odd n > 1 (getsegprimes.cl:118-164,
    prime.h:66-112), q = n^-1 mod 2^64, one = 2^64 mod n.
    With n - 1 = d 2^t the test asks for x_0 = 2^d = 1 or x_s = x_0^(2^s) = -1 for some
@@ -458,6 +480,17 @@ WW_HD m2 m2_mul_add(m2 x, m2 y, m2 k, u64 p, u64 q)
     return r;
 }
 
+/* x * 2^v mod p^2 for v <= 31 (Montgomery form times an integer stays in Montgomery
+   form): d0 2^v = q0 p + r0, then d1 2^v + q0 = q1 p + r1, digits (r0, r1). */
+WW_HD m2 m2_mulpow2(m2 x, u32 v, u64 p, double pinv)
+{
+    u32 q0, q1;
+    m2 r;
+    r.d0 = mulpow2_add_qr(x.d0, v, 0, p, pinv, &q0);
+    r.d1 = mulpow2_add_qr(x.d1, v, q0, p, pinv, &q1);
+    return r;
+}
+
 /* Montgomery form back to ordinary digits (REDC, wwcpu.cpp:189-206). */
 WW_HD m2 m2_get(m2 x, u64 p, u64 q)
 {
@@ -507,19 +540,29 @@ WW_HD m2 m2_div5(m2 x, u64 p)
     return r;
 }
 
-/* 2^((p-1)/2) mod p^2 as digits (s0, s1) (wieferich.cl:219-245, wwcpu.cpp:253-284). */
-WW_HD m2 wieferich_residue(u64 p, u64 q, m2 one)
+/* 2^((p-1)/2) mod p^2 as digits (s0, s1) (wieferich.cl:219-245, wwcpu.cpp:253-284).
+   The exponent's bits from `top` (any bound >= its leading bit, e.g. common to a wave)
+   down to 0 are taken in windows of 5: five Montgomery squares, then one exact
+   multiplication by 2^v, instead of a square and a conditional doubling per bit. */
+WW_HD m2 wieferich_residue_w(u64 p, u64 q, m2 one, double pinv, int top)
 {
+    const int W = 5;
     u64 e = p >> 1;
-    m2 a = m2_add(one, one, p); /* 2: the leading bit of e */
-    for (int i = 62 - clz64(e); i >= 0; --i) {
-        a = m2_sqr(a, p, q);
-        m2 d = m2_add(a, a, p);
-        bool bit = (e >> i) & 1;
-        a.d0 = bit ? d.d0 : a.d0;
-        a.d1 = bit ? d.d1 : a.d1;
+    int len = top + 1;
+    int first = len - ((len - 1) / W) * W; /* 1..W bits */
+    int pos = top - first + 1;
+    m2 a = m2_mulpow2(one, (u32)(e >> pos) & ((1u << first) - 1u), p, pinv);
+    while (pos > 0) {
+        pos -= W;
+        for (int k = 0; k < W; ++k) a = m2_sqr(a, p, q);
+        a = m2_mulpow2(a, (u32)(e >> pos) & 31u, p, pinv);
     }
     return m2_get(a, p, q);
+}
+
+WW_HD m2 wieferich_residue(u64 p, u64 q, m2 one)
+{
+    return wieferich_residue_w(p, q, one, 1.0 / (double)p, 63 - clz64(p >> 1));
 }
 
 /* F_{p-(p/5)} mod p^2 as digits (c20, c21) (wallsunsun.cl:242-300, wwcpu.cpp:311-372).
@@ -527,13 +570,14 @@ WW_HD m2 wieferich_residue(u64 p, u64 q, m2 one)
    (its roots are the squares of the golden ratio and its conjugate). Its companion V
    has constant-coefficient doubling formulas,
      V_2k = V_k^2 - 2,   V_2k+1 = V_k V_k+1 - 3,
-   so the ladder on (V_k, V_k+1) over the bits of n = m/2 costs one square and one product
+   so a ladder on (V_k, V_k+1) over the bits of n = m/2 costs one square and one product
    per bit with no sign-dependent constants, and 5 U_n = 2 V_{n+1} - 3 V_n at the end.
    The pair lives in two fixed slots (x, y): the product is symmetric and goes to x, the
    square of the operand the bit picks goes to y, so after bit i slot x holds V_k exactly
-   when bit i is 1. The square's operand is therefore y iff bits i and i+1 of n are equal,
-   and only that one operand is selected per bit. */
-WW_HD m2 wallsunsun_residue(u64 p, u64 q, m2 one)
+   when bit i is 1, and the square's operand is y iff bits i and i+1 of n are equal. The
+   ladder starts at k = 0, (V_0, V_1) = (2, 3), which leading zero bits leave unchanged,
+   so any top >= the leading bit of n (e.g. common to a wave) gives the same result. */
+WW_HD m2 wallsunsun_residue_w(u64 p, u64 q, m2 one, int top)
 {
     u32 pm5 = mod5(p);
     u64 n = ((pm5 == 2 || pm5 == 3) ? p + 1 : p - 1) >> 1;
@@ -544,11 +588,11 @@ WW_HD m2 wallsunsun_residue(u64 p, u64 q, m2 one)
     m2 r2 = two;
     for (int j = 0; j < 6; ++j) r2 = m2_sqr(r2, p, q);
     m2 r2x2 = m2_add(r2, r2, p);
-    m2 minus2 = m2_neg(r2x2, p);                    /* -2 R^2 */
-    m2 minus3 = m2_neg(m2_add(r2x2, r2, p), p);     /* -3 R^2 */
-    m2 x = three;                                   /* V_1 = 3, slot x holds V_k */
-    m2 y = m2_sqr_add(three, minus2, p, q);         /* V_2 = 7 */
-    for (int i = 62 - clz64(n); i >= 0; --i) {
+    m2 minus2 = m2_neg(r2x2, p);                /* -2 R^2 */
+    m2 minus3 = m2_neg(m2_add(r2x2, r2, p), p); /* -3 R^2 */
+    m2 x = three;                               /* k = 0: bit top+1 is 0, x holds V_1 */
+    m2 y = two;                                 /* and y holds V_0 */
+    for (int i = top; i >= 0; --i) {
         WW_LOOP_LOCAL(p);
         bool use_y = (same >> i) & 1;
         m2 sin;
@@ -557,7 +601,7 @@ WW_HD m2 wallsunsun_residue(u64 p, u64 q, m2 one)
         x = m2_mul_add(x, y, minus3, p, q); /* V_{2k+1} = V_k V_{k+1} - 3 */
         y = m2_sqr_add(sin, minus2, p, q);  /* V_{2k} or V_{2k+2} = V^2 - 2 */
     }
-    /* slot x holds V_n iff bit 0 of n is 1 (or the loop did not run, n = 1) */
+    /* slot x holds V_n iff bit 0 of n is 1 */
     bool odd = n & 1;
     m2 vk, vk1;
     vk.d0 = odd ? x.d0 : y.d0;
@@ -568,6 +612,19 @@ WW_HD m2 wallsunsun_residue(u64 p, u64 q, m2 one)
     m2 three_v0 = m2_add(m2_add(vk, vk, p), vk, p);
     m2 f5 = m2_sub(two_v1, three_v0, p);
     return m2_get(m2_div5(f5, p), p, q);
+}
+
+/* Bit index of the leading bit of the Wall-Sun-Sun ladder's n for this p. */
+WW_HD int wallsunsun_top(u64 p)
+{
+    u32 pm5 = mod5(p);
+    u64 n = ((pm5 == 2 || pm5 == 3) ? p + 1 : p - 1) >> 1;
+    return 63 - clz64(n);
+}
+
+WW_HD m2 wallsunsun_residue(u64 p, u64 q, m2 one)
+{
+    return wallsunsun_residue_w(p, q, one, wallsunsun_top(p));
 }
 
 } // namespace ww

@@ -79,8 +79,7 @@ __device__ __forceinline__ void tests_round(u32 n, u32 &qn, u64 *wq, u32 *ws, u3
     wave_sync_lds();
     qn = rest;
 
-    u64 v = 0;
-    if (act) v = test_prime(p, res, res_cap, rcount, (int32_t)slot);
+    const u64 v = test_prime_wave(p, act, res, res_cap, rcount, (int32_t)slot);
 
     /* per-slot sums: one wave reduction per distinct slot in the round (entries of one
        segment are contiguous, so a round holds one slot or a few) */

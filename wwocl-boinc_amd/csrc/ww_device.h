@@ -189,17 +189,28 @@ __device__ __forceinline__ void classify_wss(u64 p, u64 c21, ww_result *res, u32
     else if (c21 >= p - T) put_result(res, cap, count, p, 2, 0, (int32_t)(u32)(c21 - p), tag);
 }
 
-/* Both tests on one list entry; returns the entry's checksum contribution
-   low32(s1) + low32(c21) (wieferich.cl:315, wallsunsun.cl:352). */
-__device__ __forceinline__ u64 test_prime(u64 p, ww_result *res, u32 cap, u32 *count, int32_t tag)
+/* Both tests on one list entry per lane; returns the entry's checksum contribution
+   low32(s1) + low32(c21) (wieferich.cl:315, wallsunsun.cl:352), 0 on lanes with act
+   false. The ladders run from bounds common to the wave (the largest leading bits of
+   their exponents), so every lane runs the same loops; inactive lanes take part in the
+   bounds only. Called by all 64 lanes. */
+__device__ __forceinline__ u64 test_prime_wave(u64 p, bool act, ww_result *res, u32 cap,
+                                               u32 *count, int32_t tag)
 {
-    const u64 q = inv64(p);
-    const m2 one = m2_one(p);
-    const m2 s = wieferich_residue(p, q, one);
-    const m2 f = wallsunsun_residue(p, q, one);
-    classify_w(p, s.d0, s.d1, res, cap, count, tag);
-    classify_wss(p, f.d1, res, cap, count, tag);
-    return (s.d1 & 0xffffffffu) + (f.d1 & 0xffffffffu);
+    const u64 pb = act ? p : 131; /* any prime > 5 for the bounds */
+    const int top_w = wave_max(63 - clz64(pb >> 1));
+    const int top_f = wave_max(wallsunsun_top(pb));
+    u64 v = 0;
+    if (act) {
+        const u64 q = inv64(p);
+        const m2 one = m2_one(p);
+        const m2 s = wieferich_residue_w(p, q, one, 1.0 / (double)p, top_w);
+        const m2 f = wallsunsun_residue_w(p, q, one, top_f);
+        classify_w(p, s.d0, s.d1, res, cap, count, tag);
+        classify_wss(p, f.d1, res, cap, count, tag);
+        v = (s.d1 & 0xffffffffu) + (f.d1 & 0xffffffffu);
+    }
+    return v;
 }
 
 } // namespace ww

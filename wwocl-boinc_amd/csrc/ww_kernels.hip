@@ -103,9 +103,13 @@ __global__ __launch_bounds__(kTestThreads) void k_ww_tests(const u64 *list, u64 
     const u32 count = st->primecount;
     const u64 n = count < cap ? count : cap;
     u64 sum = 0;
-    for (u64 i = (u64)blockIdx.x * kTestThreads + threadIdx.x; i < n;
-         i += (u64)gridDim.x * kTestThreads) {
-        sum += test_prime(list[i], res, res_cap, &st->resultcount, 0);
+    const u32 lane = threadIdx.x & 63;
+    /* grid-stride over wave-sized pieces, so that a wave's lanes run the loop together */
+    for (u64 base = (u64)blockIdx.x * kTestThreads + (threadIdx.x - lane); base < n;
+         base += (u64)gridDim.x * kTestThreads) {
+        const u64 i = base + lane;
+        const bool act = i < n;
+        sum += test_prime_wave(act ? list[i] : 0, act, res, res_cap, &st->resultcount, 0);
     }
     /* wave reduction, one 64-bit atomic per wave */
     for (int d = 32; d > 0; d >>= 1) sum += __shfl_xor(sum, d, 64);
