@@ -210,7 +210,7 @@ def main():
                     help="length of each serving run (0 = skip serving)")
     ap.add_argument("--serve-rates", default="20,50,100", help="request rates (per second)")
     ap.add_argument("--no-tune", action="store_true",
-                    help="skip the autotuner (defaults: deep bound 4096, full occupancy)")
+                    help="skip the autotuner (defaults: deep bound 16384, full occupancy)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="also launch one plain copy kernel of BYTES (counter calibration)")
     args = ap.parse_args()

@@ -49,7 +49,7 @@ extern "C" {
 /* Result-buffer size of the reference (numresults, main.cpp:1119). */
 #define WW_DEFAULT_RESULTS 15000u
 /* Default bound of the deep sieve (see ww_set_deep_bound). */
-#define WW_DEFAULT_DEEP_BOUND 4096u
+#define WW_DEFAULT_DEEP_BOUND 16384u
 
 /* One result record (the four arrays gpuSpecialPrime/gpuResult/gpuRem/gpuQuot of
    main.cpp:1458-1493, as one struct). */
@@ -106,7 +106,7 @@ int ww_close(ww_ctx *ctx);
    (main.cpp:1087-1091). */
 uint64_t ww_list_bound(uint64_t low, uint64_t high);
 
-/* Bound B (0 or 127..8192) of the deep sieve applied by the candidate generators after
+/* Bound B (0 or 127..65535) of the deep sieve applied by the candidate generators after
    the reference's own filter by 3..113. No counterpart in the reference: it removes
    composites with a factor in 127..B *except* the single residue class per factor in
    which base-2 pseudoprimes divisible by it must lie, so the PRP list is unchanged and
@@ -171,16 +171,18 @@ int ww_prp_flags(ww_ctx *ctx, const uint64_t *n, uint64_t count, uint8_t *out);
 int ww_cpu_is_prime(uint64_t n);
 int ww_cpu_is_prp(uint64_t n);
 
-/* One deep-sieve prime: ell, its base-2 pseudoprime class rho mod m with
-   m = ell * lcm(2, ord_ell(2)) (rho = 0 mod ell, rho = 1 mod lcm(2, ord_ell(2))), and
-   reduction constants c32_* = 2^32 mod *, inv_* = 1.0 / *. */
+/* One deep-sieve prime: ell < 2^16, its base-2 pseudoprime class rho mod m with
+   m = ell * lcm(2, ord_ell(2)) < 2^32 (rho = 0 mod ell, rho = 1 mod lcm(2, ord_ell(2)),
+   which makes rho = ell), and reduction constants c32_m = 2^32 mod m,
+   bar_ell = floor(2^32 / ell), bar_m = floor(2^64 / m). */
 typedef struct {
-    uint32_t ell, m, rho, c32_ell, c32_m, pad;
-    double inv_ell, inv_m;
+    uint32_t ell, m, rho, c32_m, bar_ell, pad;
+    uint64_t bar_m;
 } ww_deep_prime;
 
-/* Host-side (CPU) table of the deep-sieve primes 127 <= ell <= bound, ascending; writes
-   up to cap entries and returns the number written. No counterpart in the reference. */
+/* Host-side (CPU) table of the deep-sieve primes 127 <= ell <= bound (bound < 2^16),
+   ascending; writes up to cap entries and returns the number written. No counterpart in
+   the reference. */
 uint32_t ww_deep_table(uint32_t bound, ww_deep_prime *out, uint32_t cap);
 
 /* The reference CPU path over [low, high) (main.cpp:648-697), on the host with the

@@ -22,14 +22,28 @@ def order2(ell):
 def test_table_entries():
     t = wwlib.deep_table(8192)
     assert len(t) == 1028 - 30  # pi(8192) - pi(113)
-    for ell, m, rho, c32l, c32m in t:
+    for ell, m, rho, c32m, bar_ell, bar_m in t:
         o = order2(ell)
         lcm = o if o % 2 == 0 else 2 * o
         assert m == ell * lcm and 0 <= rho < m
         assert rho % ell == 0 and rho % lcm == 1          # the class of the pseudoprimes
+        assert rho == ell
         assert pow(2, rho - 1, ell) == 1
-        assert c32l == (1 << 32) % ell and c32m == (1 << 32) % m
+        assert c32m == (1 << 32) % m
+        assert bar_ell == (1 << 32) // ell and bar_m == (1 << 64) // m
     assert [e for e, *_ in wwlib.deep_table(4096)] == [e for e, *_ in t if e <= 4096]
+
+
+def test_table_to_the_largest_bound():
+    t = wwlib.deep_table(65535)
+    assert len(t) == 6542 - 30  # pi(2^16) - pi(113)
+    assert t[-1][0] == 65521
+    rng = random.Random(5)
+    for ell, m, rho, c32m, bar_ell, bar_m in rng.sample(t, 200) + t[-5:]:
+        o = order2(ell)
+        assert m == ell * (o if o % 2 == 0 else 2 * o) and m < (1 << 32)
+        assert rho == ell and c32m == (1 << 32) % m
+        assert bar_ell == (1 << 32) // ell and bar_m == (1 << 64) // m
 
 
 def survivors(lo, hi, bound):

@@ -10,7 +10,7 @@ from test_deep_sieve import KNOWN_SPSP2
 
 pytestmark = pytest.mark.gpu
 
-BOUNDS = [0, 127, 1024, 4096, 8192]
+BOUNDS = [0, 127, 1024, 4096, 8192, 20011, 65535]
 
 
 @pytest.fixture
@@ -40,7 +40,7 @@ def test_list_and_ranges_at_bound(deep_ctx, bound):
         assert (g.prp_count, g.checksum, g.results) == oracle.range_results(lo, hi), (lo, hi)
 
 
-@pytest.mark.parametrize("bound", [0, 8192])
+@pytest.mark.parametrize("bound", [0, 8192, 65535])
 def test_batch_at_bound(deep_ctx, bound):
     ctx = deep_ctx
     ctx.set_deep_bound(bound)

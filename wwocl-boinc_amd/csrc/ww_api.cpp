@@ -667,7 +667,7 @@ int ww_autotune(ww_ctx *c, uint64_t lo, uint64_t hi, const uint32_t *bounds, uin
 {
     int rc = use(c);
     if (rc) return rc;
-    static const uint32_t kBounds[] = {0, 1024, 2048, 4096, 8192};
+    static const uint32_t kBounds[] = {0, 2048, 4096, 8192, 16384, 32768, 65535};
     static const int32_t kBlocks[] = {0, 4, 5};
     static const uint32_t kClaims[] = {1, 4, 16, 64};
     if (!bounds || !nb) {

@@ -326,6 +326,30 @@ WW_HD bool spsp2(u64 n, u64 q, u64 one)
     return spsp2_window(n, q, one, 63 - clz64(e), ctz64(e));
 }
 
+/* The deep sieve's start positions in a segment of odd numbers G + 2i (G odd) for one
+   prime ell < 2^16 with m = ell * lcm(2, ord_ell(2)) < 2^32: i_mark in [0, ell) is the
+   first odd multiple of ell, i_resc in [0, m/2) the first member of the pseudoprime
+   class ell mod m. Integer only: G = hi 2^32 + lo is congruent mod m to
+   hi (2^32 mod m) + lo < 2^64 (one MAD), reduced by a 64-bit Barrett step with
+   bar_m = floor(2^64 / m); then 32-bit Barrett steps with bar_ell = floor(2^32 / ell)
+   give G mod ell and i_mark = -G (ell + 1)/2 mod ell. Each Barrett quotient is the true
+   one or one less, so one conditional subtraction finishes each step. */
+WW_HD void deep_offsets(u64 G, u32 ell, u32 m, u32 c32m, u32 bar_ell, u64 bar_m,
+                        u32 *i_mark, u32 *i_resc)
+{
+    u64 B = mad32((u32)(G >> 32), c32m, (u32)G); /* = G mod m, below 2^64 */
+    u64 rm = B - mul_hi(B, bar_m) * m;
+    if (rm >= m) rm -= m;
+    u32 r = (u32)rm;
+    u32 rl = r - (u32)(((u64)r * bar_ell) >> 32) * ell;
+    if (rl >= ell) rl -= ell;
+    u32 x = (rl ? ell - rl : 0u) * ((ell + 1) >> 1); /* < 2^31 */
+    u32 im = x - (u32)(((u64)x * bar_ell) >> 32) * ell;
+    if (im >= ell) im -= ell;
+    *i_mark = im;
+    *i_resc = (ell >= r ? ell - r : ell + m - r) >> 1; /* (ell - G) mod m is even */
+}
+
 /* ---------------------------------------------------------------------------
  * mod p^2 residues (two base-p digits, Montgomery factor 2^64)
  * ------------------------------------------------------------------------- */
@@ -577,16 +601,15 @@ WW_HD m2 wieferich_residue(u64 p, u64 q, m2 one)
    when bit i is 1, and the square's operand is y iff bits i and i+1 of n are equal. The
    ladder starts at k = 0, (V_0, V_1) = (2, 3), which leading zero bits leave unchanged,
    so any top >= the leading bit of n (e.g. common to a wave) gives the same result. */
-WW_HD m2 wallsunsun_residue_w(u64 p, u64 q, m2 one, int top)
+WW_HD m2 wallsunsun_residue_w(u64 p, u64 q, m2 one, double pinv, int top)
 {
     u32 pm5 = mod5(p);
     u64 n = ((pm5 == 2 || pm5 == 3) ? p + 1 : p - 1) >> 1;
     u64 same = ~(n ^ (n >> 1)); /* bit i: bits i and i+1 of n are equal */
     m2 two = m2_add(one, one, p);
     m2 three = m2_add(two, one, p);
-    /* R^2 mod p^2 is the Montgomery form of 2^64: six squarings of 2 */
-    m2 r2 = two;
-    for (int j = 0; j < 6; ++j) r2 = m2_sqr(r2, p, q);
+    /* R^2 mod p^2 is the Montgomery form of 2^64: one times 2^31, 2^31 and 2^2 */
+    m2 r2 = m2_mulpow2(m2_mulpow2(m2_mulpow2(one, 31, p, pinv), 31, p, pinv), 2, p, pinv);
     m2 r2x2 = m2_add(r2, r2, p);
     m2 minus2 = m2_neg(r2x2, p);                /* -2 R^2 */
     m2 minus3 = m2_neg(m2_add(r2x2, r2, p), p); /* -3 R^2 */
@@ -624,7 +647,7 @@ WW_HD int wallsunsun_top(u64 p)
 
 WW_HD m2 wallsunsun_residue(u64 p, u64 q, m2 one)
 {
-    return wallsunsun_residue_w(p, q, one, wallsunsun_top(p));
+    return wallsunsun_residue_w(p, q, one, 1.0 / (double)p, wallsunsun_top(p));
 }
 
 } // namespace ww

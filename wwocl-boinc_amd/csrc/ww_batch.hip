@@ -108,7 +108,7 @@ __global__ __launch_bounds__(64 * kBatchWaves) __attribute__((amdgpu_waves_per_e
                                                              u32 claim, const DeepPrime *deep,
                                                              u32 ndeep)
 {
-    __shared__ uint16_t cand[kBatchWaves][kWaveCandCap];
+    __shared__ __align__(16) uint16_t cand[kBatchWaves][kWaveCandCap];
     __shared__ u64 deep_bm[kBatchWaves][64];
     __shared__ u64 qp[kBatchWaves][kQueueCap];
     __shared__ u32 qs[kBatchWaves][kQueueCap];
@@ -182,11 +182,14 @@ __global__ __launch_bounds__(64 * kBatchWaves) __attribute__((amdgpu_waves_per_e
         /* sieve, then compact candidate positions into LDS */
         u64 bits = sieve_word(group, lane * (u32)kIntsPerWord, sp.hi);
         if (ndeep) {
+            /* the candidate buffer is free until the compaction below: it holds the
+               deep sieve's pseudoprime classes meanwhile */
             u64 *bm = deep_bm[wave];
+            u64 *rm = reinterpret_cast<u64 *>(wc);
             bm[lane] = 0;
+            rm[lane] = 0;
             wave_sync_lds();
-            deep_sieve(bm, group, deep, ndeep, lane);
-            bits &= ~bm[lane];
+            bits &= ~deep_sieve(bm, rm, group, deep, ndeep, lane);
         }
         const u32 cnt = __popcll(bits);
         u32 incl = cnt;

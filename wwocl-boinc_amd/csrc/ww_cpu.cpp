@@ -53,6 +53,16 @@ bool no_factor_to_113(u64 n)
 
 } // namespace
 
+static uint32_t powmod32(uint32_t b, uint32_t e, uint32_t m)
+{
+    uint64_t r = 1 % m, x = b % m;
+    for (; e; e >>= 1) {
+        if (e & 1) r = r * x % m;
+        x = x * x % m;
+    }
+    return (uint32_t)r;
+}
+
 extern "C" {
 
 int ww_cpu_is_prime(uint64_t n)
@@ -103,6 +113,7 @@ int ww_cpu_range(uint64_t low, uint64_t high, uint64_t out[3])
 
 uint32_t ww_deep_table(uint32_t bound, ww_deep_prime *out, uint32_t cap)
 {
+    if (bound > 65535) bound = 65535; /* m = ell * lcm(2, ord) must stay below 2^32 */
     std::vector<uint8_t> comp(bound + 1, 0);
     uint32_t n = 0;
     for (uint32_t i = 2; i <= bound; ++i) {
@@ -110,29 +121,25 @@ uint32_t ww_deep_table(uint32_t bound, ww_deep_prime *out, uint32_t cap)
         for (uint64_t j = (uint64_t)i * i; j <= bound; j += i) comp[j] = 1;
         if (i <= 113) continue;
         if (n < cap && out) {
-            uint32_t ord = 1;
-            for (uint64_t x = 2 % i; x != 1; x = x * 2 % i) ++ord;
-            const int64_t L = (ord % 2 == 0) ? ord : 2 * (int64_t)ord;
-            /* t = l^-1 mod L (extended Euclid); rho = l t is 0 mod l and 1 mod L */
-            int64_t r0 = L, r1 = i % L, t0 = 0, t1 = 1;
-            while (r1) {
-                int64_t q = r0 / r1, tmp = r0 - q * r1;
-                r0 = r1;
-                r1 = tmp;
-                tmp = t0 - q * t1;
-                t0 = t1;
-                t1 = tmp;
+            /* ord_ell(2) divides ell - 1: strip each prime factor while 2^(ord/f) = 1 */
+            uint32_t ord = i - 1, rest = i - 1;
+            for (uint32_t f = 2; rest > 1; ++f) {
+                if (f * f > rest) f = rest;
+                if (rest % f) continue;
+                while (rest % f == 0) rest /= f;
+                while (ord % f == 0 && powmod32(2, ord / f, i) == 1) ord /= f;
             }
-            const uint64_t t = (uint64_t)(((t0 % L) + L) % L);
+            const uint64_t L = (ord % 2 == 0) ? ord : 2 * (uint64_t)ord;
             ww_deep_prime &d = out[n];
             d.ell = i;
-            d.m = (uint32_t)(i * (uint64_t)L);
-            d.rho = (uint32_t)(i * t);
-            d.c32_ell = (uint32_t)((1ull << 32) % i);
+            d.m = (uint32_t)(i * L);
+            /* rho = 0 mod ell and 1 mod L; ell - 1 is a multiple of ord and even, so
+               ell = 1 mod L and rho = ell */
+            d.rho = i;
             d.c32_m = (uint32_t)((1ull << 32) % d.m);
+            d.bar_ell = (uint32_t)(0xffffffffu / i);  /* floor(2^32 / ell), ell odd */
             d.pad = 0;
-            d.inv_ell = 1.0 / (double)i;
-            d.inv_m = 1.0 / (double)d.m;
+            d.bar_m = ~0ull / d.m;                     /* floor(2^64 / m), m no power of 2 */
         }
         ++n;
     }

@@ -97,52 +97,32 @@ __device__ __forceinline__ bool spsp2_wave(u64 n, bool act)
     return prp;
 }
 
-/* x mod m for x < 2^53 with a double reciprocal: the quotient estimate is off by at
-   most one and is corrected exactly. */
-__device__ __forceinline__ u32 mod_small(u64 x, u32 m, double inv)
-{
-    const u64 q = (u64)((double)x * inv);
-    int64_t r = (int64_t)x - (int64_t)(q * m);
-    if (r < 0) r += m;
-    else if (r >= (int64_t)m) r -= m;
-    return (u32)r;
-}
-
-/* x mod m for any 64-bit x and m < 2^26, with c32 = 2^32 mod m. */
-__device__ __forceinline__ u32 mod_u64(u64 x, u32 m, double inv, u32 c32)
-{
-    const u32 rh = mod_small(x >> 32, m, inv);
-    const u32 rl = mod_small((u32)x, m, inv);
-    return mod_small((u64)rh * c32 + rl, m, inv); /* < 2^52 */
-}
-
-/* Marks (bits set) the multiples of the deep primes among group + 2i, i < 4096, in the
-   wave's bitmap bm[64] and then clears each prime's pseudoprime class. Lanes share the
-   primes round-robin. Called by all 64 lanes of a wave; bm must be zeroed before. */
-__device__ __forceinline__ void deep_sieve(u64 *bm, u64 group, const DeepPrime *deep, u32 ndeep,
-                                           u32 lane)
+/* Deep sieve of one wave segment (odd numbers group + 2i, i < 4096): returns this
+   lane's 64-bit word of positions to drop, the multiples of the deep primes minus each
+   prime's base-2 pseudoprime class (which holds every strong pseudoprime that the prime
+   divides, so no list member is dropped). Lanes share the primes round-robin; multiples
+   go to bm, pseudoprime classes to rm (both 64 words of LDS, zeroed by the caller), so
+   each prime's offsets are computed once. Called by all 64 lanes. */
+__device__ __forceinline__ u64 deep_sieve(u64 *bm, u64 *rm, u64 group, const DeepPrime *deep,
+                                          u32 ndeep, u32 lane)
 {
     for (u32 j = lane; j < ndeep; j += 64) {
         const DeepPrime d = deep[j];
-        const u32 r = mod_u64(group, d.ell, d.inv_ell, d.c32_ell);
-        u32 i = mod_small((u64)(d.ell - r) * ((d.ell + 1) >> 1), d.ell, d.inv_ell); /* ell | group+2i */
-        if (group <= d.ell && group + 2ull * i == d.ell) i += d.ell;  /* never ell itself */
+        u32 i, ir;
+        deep_offsets(group, d.ell, d.m, d.c32_m, d.bar_ell, d.bar_m, &i, &ir);
+        if (group <= d.ell && group + 2ull * i == d.ell) i += d.ell; /* never ell itself */
         for (; i < 4096; i += d.ell) atomicOr((unsigned long long *)&bm[i >> 6], 1ull << (i & 63));
+        for (; ir < 4096; ir += d.m >> 1)
+            atomicOr((unsigned long long *)&rm[ir >> 6], 1ull << (ir & 63));
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    for (u32 j = lane; j < ndeep; j += 64) {
-        const DeepPrime d = deep[j];
-        const u32 rm = mod_u64(group, d.m, d.inv_m, d.c32_m);
-        /* group + 2i = rho (mod m), both odd, m even */
-        const u32 diff = d.rho >= rm ? d.rho - rm : d.rho + d.m - rm;
-        for (u32 i = diff >> 1; i < 4096; i += d.m >> 1)
-            atomicAnd((unsigned long long *)&bm[i >> 6], ~(1ull << (i & 63)));
-    }
+    const u64 drop = bm[lane] & ~rm[lane];
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    return drop;
 }
 
 /* Append one record; `tag` goes to the record's pad field (the range slot in the
@@ -204,8 +184,9 @@ __device__ __forceinline__ u64 test_prime_wave(u64 p, bool act, ww_result *res, 
     if (act) {
         const u64 q = inv64(p);
         const m2 one = m2_one(p);
-        const m2 s = wieferich_residue_w(p, q, one, 1.0 / (double)p, top_w);
-        const m2 f = wallsunsun_residue_w(p, q, one, top_f);
+        const double pinv = 1.0 / (double)p;
+        const m2 s = wieferich_residue_w(p, q, one, pinv, top_w);
+        const m2 f = wallsunsun_residue_w(p, q, one, pinv, top_f);
         classify_w(p, s.d0, s.d1, res, cap, count, tag);
         classify_wss(p, f.d1, res, cap, count, tag);
         v = (s.d1 & 0xffffffffu) + (f.d1 & 0xffffffffu);

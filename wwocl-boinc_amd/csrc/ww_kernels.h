@@ -36,14 +36,15 @@ constexpr int kWaveCandCap = 64 * kMaxCandPerWord;                 /* 2304 */
 
 constexpr int kTestThreads = 256;
 
-/* One prime of the deep sieve (127 <= ell <= 8192) with its base-2 pseudoprime class
-   rho mod m, m = ell * lcm(2, ord_ell(2)) < 2^26; c32_* = 2^32 mod *, inv_* = 1 / *.
-   Same layout as ww_deep_prime of the C ABI, which builds the table on the host. */
+/* One prime of the deep sieve (127 <= ell < 2^16) with its base-2 pseudoprime class
+   rho = ell mod m, m = ell * lcm(2, ord_ell(2)) < 2^32, and the reduction constants
+   c32_m = 2^32 mod m, bar_ell = floor(2^32 / ell), bar_m = floor(2^64 / m). Same layout
+   as ww_deep_prime of the C ABI, which builds the table on the host. */
 struct DeepPrime {
-    uint32_t ell, m, rho, c32_ell, c32_m, pad;
-    double inv_ell, inv_m;
+    uint32_t ell, m, rho, c32_m, bar_ell, pad;
+    uint64_t bar_m;
 };
-constexpr uint32_t kMaxDeepBound = 8192;
+constexpr uint32_t kMaxDeepBound = 65535;
 
 hipError_t launch_sieve_prp(uint64_t lo, uint64_t hi, uint64_t *list, uint64_t cap,
                             DevState *st, const DeepPrime *deep, uint32_t ndeep, hipStream_t s);

@@ -20,7 +20,7 @@ __global__ __launch_bounds__(kSieveThreads) void k_sieve_prp(u64 base, u64 hi, u
                                                               u64 cap, DevState *st,
                                                               const DeepPrime *deep, u32 ndeep)
 {
-    __shared__ uint16_t cand[kSieveWaves * kWaveCandCap];
+    __shared__ __align__(16) uint16_t cand[kSieveWaves * kWaveCandCap];
     __shared__ u64 deep_bm[kSieveWaves][64];
     __shared__ u32 s_cnt[kSieveWaves];
     __shared__ u32 s_off;
@@ -35,13 +35,16 @@ __global__ __launch_bounds__(kSieveThreads) void k_sieve_prp(u64 base, u64 hi, u
     /* ---- sieve this lane's 64 odd numbers ---- */
     u64 bits = sieve_word(block_base, lane_pos, hi);
     if (ndeep) {
+        /* the candidate buffer is free until the compaction below: it holds the deep
+           sieve's pseudoprime classes meanwhile */
         u64 *bm = deep_bm[wave];
+        u64 *rm = reinterpret_cast<u64 *>(cand + wave * kWaveCandCap);
         bm[lane] = 0;
+        rm[lane] = 0;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        deep_sieve(bm, wave_base, deep, ndeep, lane);
-        bits &= ~bm[lane];
+        bits &= ~deep_sieve(bm, rm, wave_base, deep, ndeep, lane);
     }
 
     /* ---- wave-local compaction of candidate positions into LDS ---- */

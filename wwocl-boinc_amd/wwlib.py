@@ -21,7 +21,7 @@ WW_E_NODEV = -3
 WW_E_CAPACITY = -4
 WW_MIN_START = 127
 WW_DEFAULT_RESULTS = 15000
-WW_DEFAULT_DEEP_BOUND = 4096
+WW_DEFAULT_DEEP_BOUND = 16384
 
 U64 = ctypes.c_uint64
 U32 = ctypes.c_uint32
@@ -53,8 +53,8 @@ class RangeSum(ctypes.Structure):
 
 
 class DeepPrime(ctypes.Structure):
-    _fields_ = [("ell", U32), ("m", U32), ("rho", U32), ("c32_ell", U32), ("c32_m", U32),
-                ("pad", U32), ("inv_ell", ctypes.c_double), ("inv_m", ctypes.c_double)]
+    _fields_ = [("ell", U32), ("m", U32), ("rho", U32), ("c32_m", U32), ("bar_ell", U32),
+                ("pad", U32), ("bar_m", U64)]
 
 
 class Tuning(ctypes.Structure):
@@ -222,11 +222,13 @@ def cpu_is_prp(n: int) -> bool:
 
 
 def deep_table(bound: int):
-    """[(ell, m, rho)] of the deep sieve up to `bound` (host-side, no device)."""
-    cap = 2048
+    """[(ell, m, rho, c32_m, bar_ell, bar_m)] of the deep sieve up to `bound` (host-side,
+    no device)."""
+    cap = 8192
     buf = (DeepPrime * cap)()
     n = load().ww_deep_table(bound, buf, cap)
-    return [(buf[i].ell, buf[i].m, buf[i].rho, buf[i].c32_ell, buf[i].c32_m) for i in range(n)]
+    return [(buf[i].ell, buf[i].m, buf[i].rho, buf[i].c32_m, buf[i].bar_ell, buf[i].bar_m)
+            for i in range(n)]
 
 
 def cpu_range(low: int, high: int):
