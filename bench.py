@@ -34,9 +34,10 @@ BITS = 64  # every p in the range has bit length 64
 # rocprofv3 --pmc on MI355X for this workload. Algorithmic: the 8-byte list read.
 TRAFFIC_BYTES_PER_ENTRY = 8.00
 # k_batch keeps candidates and PRPs on chip (LDS queues); its HBM traffic per tested
-# prime is the span table lookups and the cursor/slot atomics only (each atomic reads as a
-# 32-byte write), measured the same way: 3.0 MB fetched + 7.9 MB written per step.
-BATCH_TRAFFIC_BYTES_PER_PRP = 0.0555
+# prime is the deep-sieve table, span table lookups and the cursor/slot atomics only (each
+# atomic reads as a 32-byte write), measured the same way at the default deep bound 16384:
+# 11.9 MB fetched + 15.8 MB written per step.
+BATCH_TRAFFIC_BYTES_PER_PRP = 0.141
 
 
 def mul32_path(b: int) -> int:
