@@ -5,7 +5,8 @@ Two independent sources, neither of them the oracle:
     split into base-p digits, plus a pow()-based strong probable-prime test;
   * the reference's own CPU arithmetic (prime.h, wwcpu.cpp) compiled into
     oracle/_ref/libwwref.so (oracle/Makefile, target `ref`): per-prime checksum slices
-    and result flags, and range summaries of the loop of main.cpp:648-697.
+    and result flags, and range summaries of the loop of main.cpp:648-697 (including the
+    full configurations [127, 10^8) and [2^50, 2^50 + 2^32)).
 
     python tests/golden/make_golden.py          (needs oracle/_ref built)
 """
@@ -110,18 +111,26 @@ def main():
         ranges.append({"lo": lo, "hi": hi, "prp_count": prp, "checksum": cks,
                        "cpu_result_count": flags})
 
+    # the full parity configurations: [127, 10^8) and [2^50, 2^50 + 2^32)
+    full = []
+    for name, lo, hi in (("cfg1", 127, 10**8), ("cfg2", 1 << 50, (1 << 50) + (1 << 32))):
+        prp, cks, flags = oracle.ref_range(lo, hi, os.cpu_count() or 1)
+        full.append({"name": name, "lo": lo, "hi": hi, "prp_count": prp, "checksum": cks,
+                     "cpu_result_count": flags})
+
     spsp_composites = [n for n in range(3, 200_001, 2)
                        if strong_prp2(n) and any(n % q == 0 for q in range(3, int(n**0.5) + 1, 2))]
     fixture = {
         "about": "closed forms (Python big ints) and reference CPU path (oracle/_ref)",
         "primes": primes,
         "ranges": ranges,
+        "full_configs": full,
         "spsp2_composites_below_200001": spsp_composites,
         "pi": {"1e6": 78498, "1e7": 664579, "1e8": 5761455},
     }
     with open(os.path.join(HERE, "golden.json"), "w") as f:
         json.dump(fixture, f, indent=0)
-    print(f"wrote {len(primes)} primes, {len(ranges)} ranges")
+    print(f"wrote {len(primes)} primes, {len(ranges)} ranges, {len(full)} full configurations")
 
 
 if __name__ == "__main__":

@@ -4,12 +4,17 @@ Bit-exact everywhere: every quantity on this path is an unsigned integer. Small 
 ragged ranges are compared entry by entry; the known-answer ranges of the reference
 (tests.txt:6-25) and the full-size configurations are checked end to end.
 """
+import json
+import os
 import random
 
 import pytest
 
 import oracle
 import wwlib
+
+GOLD = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                                   "golden.json")))
 
 pytestmark = pytest.mark.gpu
 
@@ -163,3 +168,20 @@ def test_cfg3_full_size_properties(gctx):
         got = gctx.run_range(s, s + (1 << 22))
         prp, cks, _, _ = oracle.range_summary(s, s + (1 << 22), threads=16)
         assert (got.prp_count, got.checksum) == (prp, cks), k
+
+
+@pytest.mark.parametrize("cfg", GOLD["full_configs"], ids=lambda c: c["name"])
+def test_full_configuration_vs_reference_code(gctx, cfg):
+    """[127, 10^8) and [2^50, 2^50 + 2^32) whole, plain and persistent forms, against the
+    reference's own CPU path (compiled from wwcpu.cpp / prime.h, tests/golden)."""
+    lo, hi = cfg["lo"], cfg["hi"]
+    got = gctx.run_range(lo, hi, 1 << 32)
+    print(f"{cfg['name']}: {got.prp_count} PRPs, checksum {got.checksum:016X}, "
+          f"{len(got.results)} records")
+    assert (got.prp_count, got.checksum) == (cfg["prp_count"], cfg["checksum"])
+    assert len(got.results) == cfg["cpu_result_count"]
+    with wwlib.Batch(gctx, [(lo, hi)]) as b:
+        b.launch(graph=True)
+        (prp, cks, recs), = b.read()
+    assert (prp, cks) == (cfg["prp_count"], cfg["checksum"])
+    assert recs == got.results
