@@ -38,6 +38,7 @@ TRAFFIC_BYTES_PER_ENTRY = 8.00
 # atomic reads as a 32-byte write), measured the same way at the default deep bound 16384:
 # 11.9 MB fetched + 15.8 MB written per step.
 BATCH_TRAFFIC_BYTES_PER_PRP = 0.141
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
 
 
 def mul32_path(b: int) -> int:
@@ -280,6 +281,13 @@ def main():
             "executed_per_unit": f"{mul32_path_executed(BITS)} MUL32 per tested prime "
                                  "(78b-117: this build's ladders)",
             "avg_launch_ms": pers["kernel_ms_per_step"], "launches": args.steps}
+        # the same kernel against HBM: measured bytes per launch over the launch time
+        hbm = pers["prps_per_step"] * BATCH_TRAFFIC_BYTES_PER_PRP / kern_s
+        out["roofline_hbm"] = {
+            "bound": "hbm", "kernel": "k_batch", "achieved": hbm / 1e9, "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": hbm / 1e9 / HBM_PEAK_GBS,
+            "traffic": pers["prps_per_step"] * BATCH_TRAFFIC_BYTES_PER_PRP,
+            "note": "not the bound: the kernel keeps candidates and PRPs on chip"}
     if plain is not None:
         tk = plain["tests_kernel"]
         key = "roofline" if pers is None else "roofline_plain_tests_kernel"
