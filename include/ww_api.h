@@ -270,7 +270,7 @@ typedef struct {
     uint32_t tick_segments;     /* 8192-integer segments per launch (0 = 2^17) */
     uint64_t hbm_budget;        /* device bytes admissions may reserve (0 = 64 GiB; capped at 90% of free HBM) */
     uint64_t bytes_per_request; /* reservation of one admitted request (0 = 1 MiB) */
-    uint32_t claim;             /* segments per claim (0 = context default) */
+    uint32_t claim;             /* segments per claim (0 = 1: shortest tick tails) */
     int32_t blocks_per_cu;      /* workgroups per CU (-1 = context default, 0 = as many as fit) */
 } ww_server_config;
 

@@ -325,7 +325,10 @@ int ww_server_create(ww_ctx *ctx, const ww_server_config *cfg, ww_server **out)
         delete s;
         return rc;
     }
-    if (!s->cfg.claim) s->cfg.claim = tuned_claim;
+    /* ticks are short, so a wave claims one segment at a time unless told otherwise:
+       longer claims leave long tails at the end of every tick (latency, not throughput) */
+    (void)tuned_claim;
+    if (!s->cfg.claim) s->cfg.claim = 1;
     int bpc = s->cfg.blocks_per_cu >= 0 ? s->cfg.blocks_per_cu : tuned_blocks;
     if (bpc <= 0) bpc = ww::batch_blocks_per_cu();
     s->grid = cus * (bpc > 0 ? bpc : 4);
