@@ -25,6 +25,9 @@ namespace {
 
 constexpr int kBatchWaves = 4;
 constexpr int kQueueCap = 128; /* < 64 left over + <= 64 appended per PRP round */
+/* candidates of one segment (at most kWaveCandCap) behind fewer than 64 leftovers */
+constexpr int kBatchCandCap = kWaveCandCap + 64;
+static_assert(kBatchCandCap - 256 >= 64, "rescue bitmap must not reach the leftovers");
 
 /* Wave-uniform values as scalars (the compiler cannot see that a lane-0 broadcast is
    uniform, and would keep them in vector registers). */
@@ -108,7 +111,7 @@ __global__ __launch_bounds__(64 * kBatchWaves) __attribute__((amdgpu_waves_per_e
                                                              u32 claim, const DeepPrime *deep,
                                                              u32 ndeep)
 {
-    __shared__ __align__(16) uint16_t cand[kBatchWaves][kWaveCandCap];
+    __shared__ __align__(16) uint16_t cand[kBatchWaves][kBatchCandCap];
     __shared__ u64 deep_bm[kBatchWaves][64];
     __shared__ u64 qp[kBatchWaves][kQueueCap];
     __shared__ u32 qs[kBatchWaves][kQueueCap];
@@ -126,11 +129,17 @@ __global__ __launch_bounds__(64 * kBatchWaves) __attribute__((amdgpu_waves_per_e
     acc.cks = acc.cnt = 0;
 
     /* One wave-uniform state machine with a single site per phase (keeps each phase's
-       registers to itself): drain a full test round, else run the next PRP round of the
-       current segment, else claim the next segment. */
+       registers to itself): drain a full test round, else run the next PRP round, else
+       claim and sieve the next segment.
+       Candidates are 12-bit odd-number positions tagged with the segment they belong to:
+       bit 12 clear = the older segment (base g_old, range slot s_old), set = the newest
+       (g_new, s_new). When fewer than 64 candidates of the newest segment remain, they move
+       to the front as "older" ones and the next segment's candidates are appended behind
+       them, so PRP rounds stay full across segment boundaries. */
     bool done = false;
-    u32 j0 = 0, ncand = 0, slot = 0;
-    u64 group = 0;
+    u32 j0 = 0, ncand = 0, kb = 0; /* kb: first entry of the newest segment */
+    u32 s_old = 0, s_new = 0;
+    u64 g_old = 0, g_new = 0;
     u64 seg_next = 0, seg_end = 0; /* segments claimed but not yet sieved */
     for (;;) {
         if (qn >= 64 || (done && qn > 0)) {
@@ -138,24 +147,27 @@ __global__ __launch_bounds__(64 * kBatchWaves) __attribute__((amdgpu_waves_per_e
                         &st->resultcount);
             continue;
         }
-        if (done) break;
-        if (j0 < ncand) {
+        const u32 rem = j0 < ncand ? ncand - j0 : 0;
+        if (rem >= 64 || (rem > 0 && (done || j0 < kb))) {
             /* strong PRP, 64 candidates per round; survivors join the test queue */
             const u32 j = j0 + lane;
             const bool in = j < ncand;
-            const u64 n = group + 2ull * wc[in ? j : j0];
+            const u32 e = wc[in ? j : j0];
+            const bool newest = (e >> 12) & 1;
+            const u64 n = (newest ? g_new : g_old) + 2ull * (e & 0xfffu);
             const bool prp = spsp2_wave(n, in);
             const u64 bal = __ballot(prp);
             if (prp) {
                 const u32 at = qn + (u32)__popcll(bal & ((1ull << lane) - 1));
                 wq[at] = n;
-                ws[at] = slot;
+                ws[at] = newest ? s_new : s_old;
             }
             qn += (u32)__popcll(bal);
             j0 += 64;
             wave_sync_lds();
             continue;
         }
+        if (done) break;
         if (seg_next >= seg_end) {
             /* claim `claim` consecutive segments: one atomic, and runs of one range */
             u64 c = 0;
@@ -176,20 +188,33 @@ __global__ __launch_bounds__(64 * kBatchWaves) __attribute__((amdgpu_waves_per_e
             else b = m;
         }
         const BatchSpan sp = spans[a];
-        slot = sp.slot;
-        group = (sp.lo | 1) + (sp.seg0 + (s - prefix[a])) * kIntsPerWave;
 
-        /* sieve, then compact candidate positions into LDS */
-        u64 bits = sieve_word(group, lane * (u32)kIntsPerWord, sp.hi);
+        /* the rem (< 64) leftovers, all of the newest segment, become the older ones at
+           the front of the buffer (rem <= j0 unless j0 = 0, so the moves never overlap) */
+        {
+            u32 e = 0;
+            if (lane < rem) e = wc[j0 + lane] & 0xfffu;
+            wave_sync_lds();
+            if (lane < rem) wc[lane] = (uint16_t)e;
+            wave_sync_lds();
+        }
+        g_old = g_new;
+        s_old = s_new;
+        s_new = uniform32(sp.slot);
+        g_new = uniform64((sp.lo | 1) + (sp.seg0 + (s - prefix[a])) * kIntsPerWave);
+
+        /* sieve, then compact candidate positions into LDS behind the leftovers */
+        u64 bits = sieve_word(g_new, lane * (u32)kIntsPerWord, sp.hi);
         if (ndeep) {
-            /* the candidate buffer is free until the compaction below: it holds the
-               deep sieve's pseudoprime classes meanwhile */
+            /* the end of the candidate buffer is free until the compaction below: it holds
+               the deep sieve's pseudoprime classes meanwhile (the leftovers sit at the
+               front) */
             u64 *bm = deep_bm[wave];
-            u64 *rm = reinterpret_cast<u64 *>(wc);
+            u64 *rm = reinterpret_cast<u64 *>(wc + kBatchCandCap - 256);
             bm[lane] = 0;
             rm[lane] = 0;
             wave_sync_lds();
-            bits &= ~deep_sieve(bm, rm, group, deep, ndeep, lane);
+            bits &= ~deep_sieve(bm, rm, g_new, deep, ndeep, lane);
         }
         const u32 cnt = __popcll(bits);
         u32 incl = cnt;
@@ -197,12 +222,13 @@ __global__ __launch_bounds__(64 * kBatchWaves) __attribute__((amdgpu_waves_per_e
             u32 v = __shfl_up(incl, d, 64);
             if (lane >= (u32)d) incl += v;
         }
-        ncand = uniform32(__shfl(incl, 63, 64));
+        kb = rem;
+        ncand = rem + uniform32(__shfl(incl, 63, 64));
         j0 = 0;
-        u32 pos = incl - cnt;
+        u32 pos = rem + incl - cnt;
         while (bits) {
             u32 bit = __builtin_ctzll(bits);
-            wc[pos++] = (uint16_t)(lane * 64 + bit);
+            wc[pos++] = (uint16_t)((1u << 12) | (lane * 64 + bit));
             bits &= bits - 1;
         }
         wave_sync_lds();
