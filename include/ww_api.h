@@ -213,7 +213,8 @@ int ww_batch_create(ww_ctx *ctx, const ww_range *ranges, uint32_t n, uint32_t re
                     ww_batch **out);
 
 /* Launch geometry: workgroups (4 waves each) per CU (0 = as many as fit) and the
-   number of consecutive 8192-integer segments a wave claims at once (0 = 8). */
+   largest number of consecutive 8192-integer segments a wave claims at once (0 = 8);
+   claims shrink toward one segment as the launch's work runs out. */
 int ww_batch_set_grid(ww_batch *b, int blocks_per_cu, uint32_t claim);
 
 /* Stream-ordered search of every range of the batch: clears the batch's sums, then one
@@ -239,7 +240,7 @@ int ww_batch_destroy(ww_batch *b);
 typedef struct {
     uint32_t deep_bound; /* ww_set_deep_bound */
     int32_t blocks_per_cu; /* ww_batch_set_grid; 0 = as many as fit */
-    uint32_t claim;        /* segments per claim */
+    uint32_t claim;        /* largest segments per claim */
     uint32_t pad;
     double ms;             /* kernel time of the sample */
     double prps_per_s;     /* tested primes per second on the sample */

@@ -141,6 +141,7 @@ __global__ __launch_bounds__(64 * kBatchWaves) __attribute__((amdgpu_waves_per_e
     u32 s_old = 0, s_new = 0;
     u64 g_old = 0, g_new = 0;
     u64 seg_next = 0, seg_end = 0; /* segments claimed but not yet sieved */
+    u64 seen = 0;                  /* cursor after this wave's last claim */
     for (;;) {
         if (qn >= 64 || (done && qn > 0)) {
             tests_round(qn >= 64 ? 64 : qn, qn, wq, ws, lane, acc, sums, res, res_cap,
@@ -169,11 +170,18 @@ __global__ __launch_bounds__(64 * kBatchWaves) __attribute__((amdgpu_waves_per_e
         }
         if (done) break;
         if (seg_next >= seg_end) {
-            /* claim `claim` consecutive segments: one atomic, and runs of one range */
+            /* claim consecutive segments with one atomic (runs of one range): up to `claim`
+               of them, tapering to one as the work runs out (remaining segments over four
+               per wave of the grid, as last seen by this wave), so the launch ends with
+               short claims and no long tail */
+            const u64 left = total > seen ? total - seen : 0;
+            u64 k = left / ((u64)gridDim.x * kBatchWaves * 4);
+            k = k < 1 ? 1 : (k > claim ? claim : k);
             u64 c = 0;
-            if (lane == 0) c = atomicAdd((unsigned long long *)&st->cursor, (unsigned long long)claim);
+            if (lane == 0) c = atomicAdd((unsigned long long *)&st->cursor, (unsigned long long)k);
             seg_next = uniform64(__shfl(c, 0, 64));
-            seg_end = seg_next + claim < total ? seg_next + claim : total;
+            seen = seg_next + k;
+            seg_end = seg_next + k < total ? seg_next + k : total;
             if (seg_next >= total) {
                 done = true;
                 continue;
