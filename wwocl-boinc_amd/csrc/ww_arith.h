@@ -29,10 +29,11 @@
 #define WW_HD static inline
 #endif
 
-/* WW_LOOP_LOCAL(x): an empty asm that makes x opaque at the top of a loop body. Loop-invariant
+/* WW_LOOP_LOCAL(x): an empty asm that makes x opaque. At the top of a loop body: loop-invariant
    64-bit operands otherwise have their 32-bit halves hoisted as zero-extended 64-bit values,
    and instruction selection, which sees one block at a time, then multiplies by the known-
-   zero high halves too (8 v_mad_u64_u32 for a 4-MUL32 high product). */
+   zero high halves too (8 v_mad_u64_u32 for a 4-MUL32 high product). On a value like 2^64 - p:
+   keeps "x + (2^64 - p)" (one 64-bit add) from being rewritten as "x - p" (two). */
 #if defined(__HIP_DEVICE_COMPILE__)
 #define WW_LOOP_LOCAL(x) asm volatile("" : "+v"(x))
 #else
@@ -160,6 +161,15 @@ WW_HD u64 add_mod(u64 x, u64 y, u64 p)
 {
     u64 c = (x >= p - y) ? p : 0;
     return x + y - c;
+}
+
+/* 2x mod p for x < p, with np = 2^64 - p: 2x >= p exactly when the shift carries out or
+   the shifted value is >= p, and adding np subtracts p modulo 2^64 (one 64-bit add). */
+WW_HD u64 dbl_mod(u64 x, u64 p, u64 np)
+{
+    u64 d = x << 1;
+    bool ge = (x >> 63) | (d >= p);
+    return d + (ge ? np : 0);
 }
 
 /* x - y mod p for x < p, y <= p. */
@@ -424,12 +434,12 @@ WW_HD m2 m2_redc(u128 t, u64 c_lo, u64 c_hi, u64 c_top, u64 p, u64 q)
    its REDC into the high digit, so only its value mod p matters: it is formed as
    x0 * (2 x1 mod p) + u0 <= (p-1)^2 + 2^64 - 1 < 2^128, whose high word is already below
    p. That drops the 129-bit sum and its conditional subtraction of the general product. */
-WW_HD m2 m2_sqr(m2 x, u64 p, u64 q)
+WW_HD m2 m2_sqr(m2 x, u64 p, u64 q, u64 np)
 {
     u128 t = sqr_wide(x.d0);
     u64 u0 = t.lo * q;
     u64 v1 = mul_hi(p, u0);
-    u128 c = mul_wide(x.d0, add_mod(x.d1, x.d1, p));
+    u128 c = mul_wide(x.d0, dbl_mod(x.d1, p, np));
     u64 tlo;
     bool cy = __builtin_add_overflow(c.lo, u0, &tlo);
     u64 t1p = c.hi + (u64)cy; /* < p */
@@ -462,12 +472,12 @@ WW_HD m2 m2_mul(m2 x, m2 y, u64 p, u64 q)
    k = -c*R^2 mod p^2 this is the Montgomery square minus c, so the ladder's "- 2" costs
    two 32-bit additions instead of a mod p^2 subtraction. Bounds: x0^2 + k0 < p^2 and
    x0*(2x1 mod p) + u0 + k1 < p*2^64, so both high words are below p as in m2_sqr. */
-WW_HD m2 m2_sqr_add(m2 x, m2 k, u64 p, u64 q)
+WW_HD m2 m2_sqr_add(m2 x, m2 k, u64 p, u64 q, u64 np)
 {
     u128 t = sqr_wide_add(x.d0, k.d0);
     u64 u0 = t.lo * q;
     u64 v1 = mul_hi(p, u0);
-    u128 c = mul_wide_add2(x.d0, add_mod(x.d1, x.d1, p), u0, k.d1);
+    u128 c = mul_wide_add2(x.d0, dbl_mod(x.d1, p, np), u0, k.d1);
     u64 up0 = c.lo * q;
     u64 v1p = mul_hi(p, up0);
     bool b;
@@ -481,7 +491,7 @@ WW_HD m2 m2_sqr_add(m2 x, m2 k, u64 p, u64 q)
 /* REDC(x*y + k), the Montgomery product plus a constant as in m2_sqr_add. The cross sum
    x0*y1 + u0 + x1*y0 + k1 is below 2p*2^64 (at most 2p^2 - 3p + 2^64), a 129-bit value
    whose high part one conditional subtraction of p brings below p. */
-WW_HD m2 m2_mul_add(m2 x, m2 y, m2 k, u64 p, u64 q)
+WW_HD m2 m2_mul_add(m2 x, m2 y, m2 k, u64 p, u64 q, u64 np)
 {
     u128 t = mul_wide_add(x.d0, y.d0, k.d0);
     u64 u0 = t.lo * q;
@@ -493,7 +503,7 @@ WW_HD m2 m2_mul_add(m2 x, m2 y, m2 k, u64 p, u64 q)
     bool c2 = __builtin_add_overflow(a.hi, e.hi, &hi);
     bool c3 = __builtin_add_overflow(hi, (u64)c1, &hi2);
     bool big = c2 | c3 | (hi2 >= p);
-    u64 t1p = big ? hi2 - p : hi2;
+    u64 t1p = hi2 + (big ? np : 0); /* minus p, modulo 2^64 */
     u64 up0 = lo * q;
     u64 v1p = mul_hi(p, up0);
     bool b;
@@ -575,10 +585,12 @@ WW_HD m2 wieferich_residue_w(u64 p, u64 q, m2 one, double pinv, int top)
     int len = top + 1;
     int first = len - ((len - 1) / W) * W; /* 1..W bits */
     int pos = top - first + 1;
+    u64 np = 0 - p;
+    WW_LOOP_LOCAL(np); /* opaque, so that adding it is not turned back into subtracting p */
     m2 a = m2_mulpow2(one, (u32)(e >> pos) & ((1u << first) - 1u), p, pinv);
     while (pos > 0) {
         pos -= W;
-        for (int k = 0; k < W; ++k) a = m2_sqr(a, p, q);
+        for (int k = 0; k < W; ++k) a = m2_sqr(a, p, q, np);
         a = m2_mulpow2(a, (u32)(e >> pos) & 31u, p, pinv);
     }
     return m2_get(a, p, q);
@@ -613,6 +625,8 @@ WW_HD m2 wallsunsun_residue_w(u64 p, u64 q, m2 one, double pinv, int top)
     m2 r2x2 = m2_add(r2, r2, p);
     m2 minus2 = m2_neg(r2x2, p);                /* -2 R^2 */
     m2 minus3 = m2_neg(m2_add(r2x2, r2, p), p); /* -3 R^2 */
+    u64 np = 0 - p;
+    WW_LOOP_LOCAL(np); /* opaque, so that adding it is not turned back into subtracting p */
     m2 x = three;                               /* k = 0: bit top+1 is 0, x holds V_1 */
     m2 y = two;                                 /* and y holds V_0 */
     for (int i = top; i >= 0; --i) {
@@ -621,8 +635,8 @@ WW_HD m2 wallsunsun_residue_w(u64 p, u64 q, m2 one, double pinv, int top)
         m2 sin;
         sin.d0 = use_y ? y.d0 : x.d0;
         sin.d1 = use_y ? y.d1 : x.d1;
-        x = m2_mul_add(x, y, minus3, p, q); /* V_{2k+1} = V_k V_{k+1} - 3 */
-        y = m2_sqr_add(sin, minus2, p, q);  /* V_{2k} or V_{2k+2} = V^2 - 2 */
+        x = m2_mul_add(x, y, minus3, p, q, np); /* V_{2k+1} = V_k V_{k+1} - 3 */
+        y = m2_sqr_add(sin, minus2, p, q, np);  /* V_{2k} or V_{2k+2} = V^2 - 2 */
     }
     /* slot x holds V_n iff bit 0 of n is 1 */
     bool odd = n & 1;
