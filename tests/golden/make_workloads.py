@@ -1,0 +1,73 @@
+"""Writes tests/golden/workloads.json: the batched and serving workloads of the benchmark,
+pinned, with the reference's own CPU path (oracle/_ref, compiled from wwcpu.cpp / prime.h)
+on a sample of their ranges.
+
+  * batched: 4096 ranges, splitmix64 seed 0 (wwlib.batch_workload): the SHA-256 of the
+    whole list, its first 64 ranges, and reference summaries of 48 sampled ranges of width
+    at most 2^22;
+  * serving: Poisson arrivals at 100/s for 10 s, seed 1 (wwlib.serving_workload): the
+    SHA-256 of the whole request list, its first 16 requests, and reference summaries of
+    16 sampled requests of width at most 2^24.
+
+    python tests/golden/make_workloads.py          (needs oracle/_ref built)
+"""
+from __future__ import annotations
+
+import hashlib
+import json
+import os
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+sys.path.insert(0, os.path.join(ROOT, "wwocl-boinc_amd"))
+
+import oracle  # noqa: E402  (for the _ref binding only)
+import wwlib  # noqa: E402  (the generators; no device needed)
+
+
+def digest(items) -> str:
+    """SHA-256 of the list written one item per line, fields separated by spaces
+    (floats in repr form)."""
+    text = "\n".join(" ".join(repr(f) for f in it) for it in items)
+    return hashlib.sha256(text.encode()).hexdigest()
+
+
+def summarize(lo: int, hi: int):
+    prp, cks, flags = oracle.ref_range(lo, hi, os.cpu_count() or 1)
+    return {"lo": lo, "hi": hi, "prp_count": prp, "checksum": cks, "cpu_result_count": flags}
+
+
+def main():
+    if oracle.ref_lib() is None:
+        sys.exit("oracle/_ref/libwwref.so is not built (make -C oracle ref)")
+    batch = wwlib.batch_workload(n=4096, seed=0)
+    small = [i for i, (lo, hi) in enumerate(batch) if hi - lo <= (1 << 22)]
+    pick = small[:: max(1, len(small) // 48)][:48]
+    serving = wwlib.serving_workload(rate=100.0, duration_s=10.0, seed=1)
+    ssmall = [i for i, (_, lo, hi) in enumerate(serving) if hi - lo <= (1 << 24)]
+    spick = ssmall[:: max(1, len(ssmall) // 16)][:16]
+    fixture = {
+        "about": "benchmark workloads (splitmix64 generators of wwlib) and the reference "
+                 "CPU path (oracle/_ref) on sampled ranges",
+        "batch": {
+            "n": 4096, "seed": 0, "count": len(batch), "sha256": digest(batch),
+            "first": batch[:64],
+            "reference": [dict(index=i, **summarize(*batch[i])) for i in pick],
+        },
+        "serving": {
+            "rate": 100.0, "duration_s": 10.0, "seed": 1, "count": len(serving),
+            "sha256": digest(serving), "first": serving[:16],
+            "reference": [dict(index=i, **summarize(serving[i][1], serving[i][2]))
+                          for i in spick],
+        },
+    }
+    with open(os.path.join(HERE, "workloads.json"), "w") as f:
+        json.dump(fixture, f, indent=0)
+    print(f"batch: {len(batch)} ranges, {len(pick)} summaries; serving: {len(serving)} "
+          f"requests, {len(spick)} summaries")
+
+
+if __name__ == "__main__":
+    main()

@@ -1,5 +1,7 @@
 """The persistent form (ww_batch_*): per-range results equal the plain form's and the
 oracle's; graph replays equal eager launches; results do not depend on the mix."""
+import json
+import os
 import random
 
 import pytest
@@ -108,3 +110,16 @@ def test_autotune_keeps_results(gctx):
         assert (prp, cks, recs) == (plain.prp_count, plain.checksum, plain.results)
     finally:
         gctx.set_deep_bound(wwlib.WW_DEFAULT_DEEP_BOUND)
+
+
+def test_batch_workload_sample_vs_reference_code(gctx):
+    """48 ranges of the 4096-range benchmark workload, in one persistent launch, against
+    the reference's own CPU path (tests/golden/workloads.json)."""
+    w = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                                    "workloads.json")))["batch"]["reference"]
+    with wwlib.Batch(gctx, [(r["lo"], r["hi"]) for r in w]) as b:
+        b.launch(graph=True)
+        got = b.read()
+    for r, (prp, cks, recs) in zip(w, got):
+        assert (prp, cks, len(recs)) == (r["prp_count"], r["checksum"],
+                                         r["cpu_result_count"]), r["index"]

@@ -1,5 +1,7 @@
 """The serving form (ww_server_*): continuous batching of range requests. Each request's
 checksum, count and records are the same alone, in a mix, and in the plain form."""
+import json
+import os
 import random
 
 import pytest
@@ -49,3 +51,17 @@ def test_serve_admission_respects_slots(gctx):
     for rid, (lo, hi) in ids.items():
         plain = gctx.run_range(lo, hi)
         assert (done[rid]["prp_count"], done[rid]["checksum"]) == (plain.prp_count, plain.checksum)
+
+
+def test_serving_sample_vs_reference_code(gctx):
+    """16 requests of the 100 req/s serving stream, submitted together, against the
+    reference's own CPU path (tests/golden/workloads.json)."""
+    w = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                                    "workloads.json")))["serving"]["reference"]
+    with wwlib.Server(gctx) as srv:
+        done, ids, _ = wwlib.serve(srv, [(0.0, r["lo"], r["hi"]) for r in w], realtime=False)
+    by_range = {ids[rid]: c for rid, c in done.items()}
+    for r in w:
+        c = by_range[(r["lo"], r["hi"])]
+        assert (c["prp_count"], c["checksum"], len(c["records"])) == (
+            r["prp_count"], r["checksum"], r["cpu_result_count"]), r["index"]
