@@ -49,7 +49,9 @@ extern "C" {
 /* Result-buffer size of the reference (numresults, main.cpp:1119). */
 #define WW_DEFAULT_RESULTS 15000u
 /* Default bound of the deep sieve (see ww_set_deep_bound). */
-#define WW_DEFAULT_DEEP_BOUND 16384u
+#define WW_DEFAULT_DEEP_BOUND 2048u
+/* Default bound of the wide pre-sieve (see ww_set_presieve_bound). */
+#define WW_DEFAULT_PRESIEVE_BOUND 1048576u
 
 /* One result record (the four arrays gpuSpecialPrime/gpuResult/gpuRem/gpuQuot of
    main.cpp:1458-1493, as one struct). */
@@ -114,6 +116,17 @@ uint64_t ww_list_bound(uint64_t low, uint64_t high);
    launches issued afterwards. Default WW_DEFAULT_DEEP_BOUND. */
 int ww_set_deep_bound(ww_ctx *ctx, uint32_t bound);
 int ww_get_deep_bound(ww_ctx *ctx, uint32_t *bound);
+
+/* Bound B2 (0 = off, else at most WW_MAX_PRESIEVE_BOUND) of the wide pre-sieve of the
+   persistent form: before each batch launch, one kernel drops, in chunks of 2^21 integers,
+   the multiples of the primes in (deep bound, B2] except their pseudoprime classes (the
+   same exact rule as the deep sieve), so fewer composites reach the strong PRP test.
+   Applies to batches created afterwards (each holds 512 bytes per 8192-integer segment)
+   and to the plain form's chunks (ww_getsegprimes keeps one such buffer per context).
+   No counterpart in the reference. Default WW_DEFAULT_PRESIEVE_BOUND; a larger bound
+   than before rebuilds the prime table on the host. */
+int ww_set_presieve_bound(ww_ctx *ctx, uint32_t bound);
+int ww_get_presieve_bound(ww_ctx *ctx, uint32_t *bound);
 
 /* clearresult (clearresult.cl:36-50): zero checksum, resultcount, totalprimecount
    and primecount_max. Stream-ordered. */
@@ -184,6 +197,21 @@ typedef struct {
    ascending; writes up to cap entries and returns the number written. No counterpart in
    the reference. */
 uint32_t ww_deep_table(uint32_t bound, ww_deep_prime *out, uint32_t cap);
+
+/* One prime of the wide pre-sieve (deep primes beyond the per-segment bound, up to
+   WW_MAX_PRESIEVE_BOUND): ell, m = ell * lcm(2, ord_ell(2)) < 2^45 (its base-2
+   pseudoprime class is ell mod m), and Barrett constants bar_ell = floor(2^64 / ell),
+   bar_m = floor(2^64 / m). No counterpart in the reference. */
+typedef struct {
+    uint32_t ell, pad;
+    uint64_t m, bar_ell, bar_m;
+} ww_wide_prime;
+
+#define WW_MAX_PRESIEVE_BOUND 4194304u
+
+/* Host-side (CPU) table of the primes above < ell <= bound (bound <= WW_MAX_PRESIEVE_BOUND,
+   above >= 113), ascending; writes up to cap entries and returns the number written. */
+uint32_t ww_presieve_table(uint32_t above, uint32_t bound, ww_wide_prime *out, uint32_t cap);
 
 /* The reference CPU path over [low, high) (main.cpp:648-697), on the host with the
    same arithmetic: the startup self-test's comparison target (testKernels). out =

@@ -15,8 +15,10 @@ BOUNDS = [0, 127, 1024, 4096, 8192, 20011, 65535]
 
 @pytest.fixture
 def deep_ctx(gctx):
+    deep, pre = gctx.deep_bound(), gctx.presieve_bound()
     yield gctx
-    gctx.set_deep_bound(wwlib.WW_DEFAULT_DEEP_BOUND)
+    gctx.set_deep_bound(deep)
+    gctx.set_presieve_bound(pre)
 
 
 @pytest.mark.parametrize("bound", BOUNDS)
@@ -53,3 +55,28 @@ def test_batch_at_bound(deep_ctx, bound):
         assert b.read() == got
     for (lo, hi), g in zip(ranges, got):
         assert g == oracle.range_results(lo, hi), (bound, lo, hi)
+
+
+# strong pseudoprimes (base 2) with a factor above 16384, which the wide pre-sieve must keep:
+# 3215031751 = 151 * 751 * 28351, 2152302898747 = 6763 * 10627 * 29947,
+# 3474749660383 = 1303 * 16927 * 157543
+@pytest.mark.parametrize("deep,pre", [(0, 1 << 22), (4096, 1 << 22), (16384, 1 << 20),
+                                      (65535, 1 << 22), (16384, 20011)])
+def test_batch_with_wide_presieve(deep_ctx, deep, pre):
+    ctx = deep_ctx
+    ctx.set_deep_bound(deep)
+    ctx.set_presieve_bound(pre)
+    assert (ctx.deep_bound(), ctx.presieve_bound()) == (deep, pre)
+    for n in (3215031751, 2152302898747, 3474749660383):
+        assert oracle.is_prp(n) and not oracle.is_prime(n)
+    ranges = [(n - 3000, n + 3000) for n in KNOWN_SPSP2] + [(127, 300_000), (127, 4_000_000)]
+    ranges += [(3215031751 - (1 << 21), 3215031751 + (1 << 21))]  # whole pre-sieve chunks
+    ranges += wwlib.batch_workload(n=48, seed=5, wmax=21)
+    ranges += [(1 << 63, (1 << 63) + (1 << 22)), ((1 << 64) - (1 << 22), (1 << 64) - 1)]
+    with wwlib.Batch(ctx, ranges) as b:
+        b.launch(graph=True)
+        got = b.read()
+        b.launch()
+        assert b.read() == got
+    for (lo, hi), g in zip(ranges, got):
+        assert g == oracle.range_results(lo, hi), (deep, pre, lo, hi)

@@ -40,6 +40,17 @@ struct ww_ctx {
     uint32_t ndeep = 0; /* entries in use: primes <= the deep-sieve bound */
     uint32_t deep_bound = 0;
     std::vector<uint32_t> deep_ells;
+    /* wide pre-sieve of the persistent form: primes 127..WW_MAX_PRESIEVE_BOUND, built on
+       first use; the active ones are deep_bound < ell <= presieve_bound */
+    ww::WidePrime *d_wide = nullptr;
+    std::vector<uint32_t> wide_ells;
+    uint32_t wide_built = 0;   /* the table holds the primes up to this bound */
+    uint32_t presieve_bound = 0;
+    /* plain form: pre-sieve words of the current chunk, and its span on the device */
+    uint64_t *d_pre_plain = nullptr;
+    uint64_t pre_plain_segs = 0;
+    ww::BatchSpan *d_span1 = nullptr;
+    uint64_t *d_prefix1 = nullptr;
     int tuned_blocks = 0;      /* defaults for batches created from now on */
     uint32_t tuned_claim = 8;
     uint64_t default_chunk = 0;
@@ -105,6 +116,24 @@ int collect(std::vector<hipEvent_t> &v, double *acc)
    bounded number of events. */
 constexpr size_t kMaxPendingEvents = 512;
 
+/* Active pre-sieve primes for this context: [first, first + count) of the wide table
+   (deep_bound < ell <= presieve_bound), as one number for graph bookkeeping. */
+void wide_span(ww_ctx *c, uint32_t *first, uint32_t *count)
+{
+    *first = *count = 0;
+    if (!c->d_wide || !c->presieve_bound) return;
+    /* above the per-segment deep sieve, and above 1024 in any case: smaller primes have
+       small pseudoprime moduli m, whose classes would overflow the pre-sieve's list */
+    uint32_t lo = c->ndeep ? c->deep_bound : 0;
+    if (lo < ww::kPresieveMinPrime) lo = ww::kPresieveMinPrime;
+    uint32_t a = 0, e = 0;
+    while (a < c->wide_ells.size() && c->wide_ells[a] <= lo) ++a;
+    e = a;
+    while (e < c->wide_ells.size() && c->wide_ells[e] <= c->presieve_bound) ++e;
+    *first = a;
+    *count = e - a;
+}
+
 } // namespace
 
 int ww_set_error(int code, const char *fmt, ...)
@@ -118,9 +147,14 @@ int ww_set_error(int code, const char *fmt, ...)
 
 /* Context internals the serving engine needs (same library). */
 int ww_ctx_internals(ww_ctx *c, int *device, const ww::DeepPrime **deep, uint32_t *ndeep,
-                     int *cus, int *tuned_blocks, uint32_t *tuned_claim)
+                     int *cus, int *tuned_blocks, uint32_t *tuned_claim,
+                     const ww::WidePrime **wide, uint32_t *nwide)
 {
     if (!c) return fail(WW_E_ARG, "null context");
+    uint32_t wfirst, wcount;
+    wide_span(c, &wfirst, &wcount);
+    *wide = wcount ? c->d_wide + wfirst : nullptr;
+    *nwide = wcount;
     *device = c->device;
     *deep = c->d_deep;
     *ndeep = c->ndeep;
@@ -226,6 +260,10 @@ int ww_open(int device, uint64_t list_capacity, uint32_t result_capacity, ww_ctx
     }
     if ((e = hipMemsetAsync(c->d_state, 0, sizeof(DevState), c->stream)) != hipSuccess)
         return bail(fail(WW_E_HIP, "hipMemsetAsync: %s", hipGetErrorString(e)));
+    {
+        const int prc = ww_set_presieve_bound(c, WW_DEFAULT_PRESIEVE_BOUND);
+        if (prc) return bail(prc);
+    }
     if ((e = hipStreamSynchronize(c->stream)) != hipSuccess)
         return bail(fail(WW_E_HIP, "hipStreamSynchronize: %s", hipGetErrorString(e)));
     *out = c;
@@ -240,6 +278,10 @@ int ww_close(ww_ctx *c)
     for (hipEvent_t e : c->ev_sieve) (void)hipEventDestroy(e);
     for (hipEvent_t e : c->ev_tests) (void)hipEventDestroy(e);
     if (c->d_deep) (void)hipFree(c->d_deep);
+    if (c->d_wide) (void)hipFree(c->d_wide);
+    if (c->d_pre_plain) (void)hipFree(c->d_pre_plain);
+    if (c->d_span1) (void)hipFree(c->d_span1);
+    if (c->d_prefix1) (void)hipFree(c->d_prefix1);
     if (c->d_results) (void)hipFree(c->d_results);
     if (c->d_list) (void)hipFree(c->d_list);
     if (c->d_state) (void)hipFree(c->d_state);
@@ -263,6 +305,44 @@ int ww_get_deep_bound(ww_ctx *c, uint32_t *bound)
 {
     if (!c || !bound) return fail(WW_E_ARG, "null argument");
     *bound = c->deep_bound;
+    return WW_OK;
+}
+
+int ww_set_presieve_bound(ww_ctx *c, uint32_t bound)
+{
+    int rc = use(c);
+    if (rc) return rc;
+    if (bound > WW_MAX_PRESIEVE_BOUND)
+        return fail(WW_E_ARG, "pre-sieve bound above %u", WW_MAX_PRESIEVE_BOUND);
+    if (bound > c->wide_built) {
+        /* (re)build the table up to this bound; queued launches may still read the old one */
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        if (c->d_wide) (void)hipFree(c->d_wide);
+        c->d_wide = nullptr;
+        c->wide_ells.clear();
+        c->wide_built = 0;
+        std::vector<ww_wide_prime> t(bound / 8 + 64);
+        t.resize(ww_presieve_table(113, bound, t.data(), (uint32_t)t.size()));
+        static_assert(sizeof(ww_wide_prime) == sizeof(ww::WidePrime), "layout");
+        hipError_t e = hipMalloc(&c->d_wide, t.size() * sizeof(ww::WidePrime));
+        if (e == hipSuccess)
+            e = hipMemcpy(c->d_wide, t.data(), t.size() * sizeof(ww::WidePrime), hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            if (c->d_wide) (void)hipFree(c->d_wide);
+            c->d_wide = nullptr;
+            return fail(WW_E_HIP, "pre-sieve table: %s", hipGetErrorString(e));
+        }
+        for (const ww_wide_prime &d : t) c->wide_ells.push_back(d.ell);
+        c->wide_built = bound;
+    }
+    c->presieve_bound = bound;
+    return WW_OK;
+}
+
+int ww_get_presieve_bound(ww_ctx *c, uint32_t *bound)
+{
+    if (!c || !bound) return fail(WW_E_ARG, "null argument");
+    *bound = c->presieve_bound;
     return WW_OK;
 }
 
@@ -295,8 +375,32 @@ int ww_getsegprimes(ww_ctx *c, uint64_t low, uint64_t high)
     hipEvent_t a, b;
     if ((rc = event_pair(c, c->ev_sieve, &a, &b))) return rc;
     HIP_TRY(hipEventRecord(a, c->stream));
+    uint32_t wfirst, wcount;
+    wide_span(c, &wfirst, &wcount);
+    const uint64_t *pre = nullptr;
+    if (wcount && ww::segments_of(low, high) >= ww::kPresieveMinSegs) {
+        const uint64_t nseg = ww::segments_of(low, high);
+        if (nseg > c->pre_plain_segs) {
+            /* grow the chunk's pre-sieve buffer; queued launches may still read the old one */
+            HIP_TRY(hipStreamSynchronize(c->stream));
+            if (c->d_pre_plain) (void)hipFree(c->d_pre_plain);
+            c->d_pre_plain = nullptr;
+            c->pre_plain_segs = 0;
+            HIP_TRY(hipMalloc(&c->d_pre_plain, nseg * ww::kPresieveWordsPerSeg * sizeof(uint64_t)));
+            c->pre_plain_segs = nseg;
+        }
+        if (!c->d_span1) {
+            HIP_TRY(hipMalloc(&c->d_span1, sizeof(ww::BatchSpan)));
+            HIP_TRY(hipMalloc(&c->d_prefix1, 2 * sizeof(uint64_t)));
+        }
+        const int pgrid = (int)std::min<uint64_t>((nseg + 255) / 256, (uint64_t)c->cus);
+        HIP_TRY(ww::launch_presieve_range(low, high, c->d_span1, c->d_prefix1, c->d_wide + wfirst,
+                                          wcount, c->d_pre_plain, pgrid > 0 ? pgrid : 1,
+                                          c->stream));
+        pre = c->d_pre_plain;
+    }
     HIP_TRY(ww::launch_sieve_prp(low, high, c->d_list, c->list_cap, c->d_state, c->d_deep,
-                                 c->ndeep, c->stream));
+                                 c->ndeep, pre, c->stream));
     HIP_TRY(hipEventRecord(b, c->stream));
     c->launches++;
     if (c->ev_sieve.size() >= kMaxPendingEvents) return collect(c->ev_sieve, &c->ms_sieve);
@@ -479,11 +583,14 @@ struct ww_batch {
     SlotSum *d_sums = nullptr;
     BatchState *d_state = nullptr;
     ww_result *d_res = nullptr;
+    uint64_t *d_pre = nullptr; /* wide pre-sieve words, 64 per segment (or none) */
+    uint64_t nseg = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     int graph_grid = 0;
     uint32_t graph_claim = 0;
     uint32_t graph_ndeep = 0;
+    uint64_t graph_wide = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
 
@@ -498,8 +605,18 @@ int batch_grid(ww_ctx *c, ww_batch *b)
 
 int batch_enqueue(ww_ctx *c, ww_batch *b, int grid)
 {
+    uint32_t first, count;
+    wide_span(c, &first, &count);
+    const uint64_t *pre = nullptr;
+    if (count && b->d_pre) {
+        const uint64_t chunks = (b->nseg + 255) / 256;
+        const int pgrid = (int)std::min<uint64_t>(chunks, (uint64_t)c->cus);
+        HIP_TRY(ww::launch_presieve(b->d_spans, b->d_prefix, b->n, c->d_wide + first, count,
+                                    b->d_pre, pgrid > 0 ? pgrid : 1, c->stream));
+        pre = b->d_pre;
+    }
     HIP_TRY(ww::launch_batch(b->d_spans, b->d_prefix, b->n, b->d_state, b->d_sums, b->n, b->d_res,
-                             b->res_cap, grid, b->claim, c->d_deep, c->ndeep, c->stream));
+                             b->res_cap, grid, b->claim, c->d_deep, c->ndeep, pre, c->stream));
     return WW_OK;
 }
 
@@ -539,6 +656,9 @@ int ww_batch_create(ww_ctx *c, const ww_range *ranges, uint32_t n, uint32_t resu
     if (e == hipSuccess) e = hipMalloc(&b->d_sums, n * sizeof(SlotSum));
     if (e == hipSuccess) e = hipMalloc(&b->d_state, sizeof(BatchState));
     if (e == hipSuccess) e = hipMalloc(&b->d_res, (size_t)result_capacity * sizeof(ww_result));
+    b->nseg = prefix[n];
+    if (e == hipSuccess && c->presieve_bound && b->nseg)
+        e = hipMalloc(&b->d_pre, b->nseg * ww::kPresieveWordsPerSeg * sizeof(uint64_t));
     if (e == hipSuccess)
         e = hipMemcpyAsync(b->d_spans, spans.data(), n * sizeof(BatchSpan), hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess)
@@ -580,8 +700,11 @@ int ww_batch_launch_graph(ww_ctx *c, ww_batch *b)
     if (rc) return rc;
     if (!b) return fail(WW_E_ARG, "null batch");
     int grid = batch_grid(c, b);
+    uint32_t wfirst, wcount;
+    wide_span(c, &wfirst, &wcount);
+    const uint64_t wide = ((uint64_t)wfirst << 32) | wcount;
     if (b->exec && (b->graph_grid != grid || b->graph_claim != b->claim ||
-                    b->graph_ndeep != c->ndeep)) {
+                    b->graph_ndeep != c->ndeep || b->graph_wide != wide)) {
         (void)hipGraphExecDestroy(b->exec);
         (void)hipGraphDestroy(b->graph);
         b->exec = nullptr;
@@ -599,6 +722,7 @@ int ww_batch_launch_graph(ww_ctx *c, ww_batch *b)
         b->graph_grid = grid;
         b->graph_claim = b->claim;
         b->graph_ndeep = c->ndeep;
+        b->graph_wide = wide;
     }
     HIP_TRY(hipEventRecord(b->ev0, c->stream));
     HIP_TRY(hipGraphLaunch(b->exec, c->stream));
@@ -652,6 +776,7 @@ int ww_batch_destroy(ww_batch *b)
     if (b->graph) (void)hipGraphDestroy(b->graph);
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
+    if (b->d_pre) (void)hipFree(b->d_pre);
     if (b->d_res) (void)hipFree(b->d_res);
     if (b->d_state) (void)hipFree(b->d_state);
     if (b->d_sums) (void)hipFree(b->d_sums);
@@ -667,7 +792,7 @@ int ww_autotune(ww_ctx *c, uint64_t lo, uint64_t hi, const uint32_t *bounds, uin
 {
     int rc = use(c);
     if (rc) return rc;
-    static const uint32_t kBounds[] = {0, 2048, 4096, 8192, 16384, 32768, 65535};
+    static const uint32_t kBounds[] = {0, 1024, 2048, 4096, 8192, 16384, 65535};
     static const int32_t kBlocks[] = {0, 4, 5};
     static const uint32_t kClaims[] = {1, 4, 16, 64};
     if (!bounds || !nb) {

@@ -29,22 +29,51 @@ struct BatchState {
 };
 
 /* Number of segments of [lo, hi) (lo | 1 < hi). */
-inline uint64_t segments_of(uint64_t lo, uint64_t hi)
+__host__ __device__ inline uint64_t segments_of(uint64_t lo, uint64_t hi)
 {
     uint64_t base = lo | 1;
     return base >= hi ? 0 : (hi - base + kIntsPerWave - 1) / kIntsPerWave;
 }
 
+/* One prime of the wide pre-sieve; same layout as ww_wide_prime of the C ABI. */
+struct WidePrime {
+    uint32_t ell, pad;
+    uint64_t m, bar_ell, bar_m;
+};
+
+/* Words of the pre-sieve per segment (one per lane of a wave). */
+constexpr uint64_t kPresieveWordsPerSeg = 64;
+/* The pre-sieve skips primes up to this (their pseudoprime classes are too dense) ... */
+constexpr uint32_t kPresieveMinPrime = 1024;
+/* ... and pieces of fewer segments than this, where computing every prime's start
+   positions would cost more than the strong PRP tests it saves (those segments drop
+   nothing extra). */
+constexpr uint64_t kPresieveMinSegs = 16;
+
 int batch_blocks_per_cu();
 
+/* The wide pre-sieve over all segments of the spans: out[64 s + w] = odd numbers to drop
+   in word w of segment s (multiples of tab's primes, minus their pseudoprime classes).
+   No allocation, no memset: safe to capture into a graph. */
+hipError_t launch_presieve(const BatchSpan *spans, const uint64_t *prefix, uint32_t nspans,
+                           const WidePrime *tab, uint32_t ntab, uint64_t *out, int grid,
+                           hipStream_t s);
+
+/* The same for the single range [lo, hi) of a plain-form chunk (segments from lo | 1);
+   span and prefix are two device words of scratch the launch fills in. */
+hipError_t launch_presieve_range(uint64_t lo, uint64_t hi, BatchSpan *span, uint64_t *prefix,
+                                 const WidePrime *tab, uint32_t ntab, uint64_t *out, int grid,
+                                 hipStream_t s);
+
 /* Clears the state and the first `nslots` slot sums (0: none, so sums accumulate over
-   launches), then runs the persistent kernel over all spans; each wave claims `claim`
-   consecutive segments at a time.
+   launches), then runs the persistent kernel over all spans; each wave claims up to
+   `claim` consecutive segments at a time. `pre` (may be null) is the wide pre-sieve of
+   the same spans.
    No allocation, no memset: safe to capture into a graph. */
 hipError_t launch_batch(const BatchSpan *spans, const uint64_t *prefix, uint32_t nspans,
                         BatchState *st, SlotSum *sums, uint32_t nslots, ww_result *res,
                         uint32_t res_cap, int grid, uint32_t claim, const DeepPrime *deep,
-                        uint32_t ndeep, hipStream_t s);
+                        uint32_t ndeep, const uint64_t *pre, hipStream_t s);
 
 } // namespace ww
 

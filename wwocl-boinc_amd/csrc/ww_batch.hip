@@ -109,7 +109,7 @@ __global__ __launch_bounds__(64 * kBatchWaves) __attribute__((amdgpu_waves_per_e
                                                              BatchState *st, SlotSum *sums,
                                                              ww_result *res, u32 res_cap,
                                                              u32 claim, const DeepPrime *deep,
-                                                             u32 ndeep)
+                                                             u32 ndeep, const u64 *pre)
 {
     __shared__ __align__(16) uint16_t cand[kBatchWaves][kBatchCandCap];
     __shared__ u64 deep_bm[kBatchWaves][64];
@@ -213,6 +213,7 @@ __global__ __launch_bounds__(64 * kBatchWaves) __attribute__((amdgpu_waves_per_e
 
         /* sieve, then compact candidate positions into LDS behind the leftovers */
         u64 bits = sieve_word(g_new, lane * (u32)kIntsPerWord, sp.hi);
+        if (pre) bits &= ~pre[s * kPresieveWordsPerSeg + lane];
         if (ndeep) {
             /* the end of the candidate buffer is free until the compaction below: it holds
                the deep sieve's pseudoprime classes meanwhile (the leftovers sit at the
@@ -270,14 +271,15 @@ int batch_blocks_per_cu()
 
 hipError_t launch_batch(const BatchSpan *spans, const u64 *prefix, u32 nspans, BatchState *st,
                         SlotSum *sums, u32 nslots, ww_result *res, u32 res_cap, int grid,
-                        u32 claim, const DeepPrime *deep, u32 ndeep, hipStream_t s)
+                        u32 claim, const DeepPrime *deep, u32 ndeep, const u64 *pre,
+                        hipStream_t s)
 {
     hipLaunchKernelGGL(k_batch_clear, dim3(nslots > 256 ? (nslots + 255) / 256 : 1), dim3(256), 0, s,
                        st, sums, nslots);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_batch, dim3(grid), dim3(64 * kBatchWaves), 0, s, spans, prefix, nspans, st,
-                       sums, res, res_cap, claim < 1 ? 1u : claim, deep, ndeep);
+                       sums, res, res_cap, claim < 1 ? 1u : claim, deep, ndeep, pre);
     return hipGetLastError();
 }
 

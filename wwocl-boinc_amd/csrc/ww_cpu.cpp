@@ -146,4 +146,36 @@ uint32_t ww_deep_table(uint32_t bound, ww_deep_prime *out, uint32_t cap)
     return n < cap ? n : cap;
 }
 
+uint32_t ww_presieve_table(uint32_t above, uint32_t bound, ww_wide_prime *out, uint32_t cap)
+{
+    if (bound > WW_MAX_PRESIEVE_BOUND) bound = WW_MAX_PRESIEVE_BOUND;
+    if (above < 113) above = 113;
+    if (bound <= above) return 0;
+    /* smallest prime factor of every k <= bound, so that ell - 1 factors at once */
+    std::vector<uint32_t> spf(bound + 1, 0);
+    for (uint32_t i = 2; i <= bound; ++i)
+        if (!spf[i])
+            for (uint64_t j = i; j <= bound; j += i)
+                if (!spf[j]) spf[j] = i;
+    uint32_t n = 0;
+    for (uint32_t i = above + 1; i <= bound && n < cap; ++i) {
+        if (spf[i] != i) continue;
+        /* ord_ell(2) divides ell - 1: strip each prime factor while 2^(ord/f) = 1 */
+        uint32_t ord = i - 1, rest = i - 1;
+        while (rest > 1) {
+            const uint32_t f = spf[rest];
+            while (rest % f == 0) rest /= f;
+            while (ord % f == 0 && powmod32(2, ord / f, i) == 1) ord /= f;
+        }
+        const uint64_t L = (ord % 2 == 0) ? ord : 2 * (uint64_t)ord;
+        ww_wide_prime &d = out[n++];
+        d.ell = i;
+        d.pad = 0;
+        d.m = i * L;
+        d.bar_ell = ~0ull / i; /* floor(2^64 / ell), ell odd */
+        d.bar_m = ~0ull / d.m; /* floor(2^64 / m), m no power of 2 */
+    }
+    return n;
+}
+
 } // extern "C"

@@ -18,7 +18,8 @@ namespace ww {
 
 __global__ __launch_bounds__(kSieveThreads) void k_sieve_prp(u64 base, u64 hi, u64 *list,
                                                               u64 cap, DevState *st,
-                                                              const DeepPrime *deep, u32 ndeep)
+                                                              const DeepPrime *deep, u32 ndeep,
+                                                              const u64 *pre)
 {
     __shared__ __align__(16) uint16_t cand[kSieveWaves * kWaveCandCap];
     __shared__ u64 deep_bm[kSieveWaves][64];
@@ -34,6 +35,9 @@ __global__ __launch_bounds__(kSieveThreads) void k_sieve_prp(u64 base, u64 hi, u
 
     /* ---- sieve this lane's 64 odd numbers ---- */
     u64 bits = sieve_word(block_base, lane_pos, hi);
+    /* wide pre-sieve of this wave's segment (segments count from base) */
+    if (pre && wave_base < hi)
+        bits &= ~pre[((u64)blockIdx.x * kSieveWaves + wave) * 64 + lane];
     if (ndeep) {
         /* the candidate buffer is free until the compaction below: it holds the deep
            sieve's pseudoprime classes meanwhile */
@@ -159,14 +163,14 @@ int tests_blocks_per_cu()
 }
 
 hipError_t launch_sieve_prp(u64 lo, u64 hi, u64 *list, u64 cap, DevState *st,
-                            const DeepPrime *deep, u32 ndeep, hipStream_t s)
+                            const DeepPrime *deep, u32 ndeep, const u64 *pre, hipStream_t s)
 {
     u64 base = lo | 1;
     if (base >= hi) return hipSuccess;
     u64 span = hi - base;
     u64 blocks = (span + kIntsPerBlock - 1) / kIntsPerBlock;
     hipLaunchKernelGGL(k_sieve_prp, dim3((unsigned)blocks), dim3(kSieveThreads), 0, s, base, hi,
-                       list, cap, st, deep, ndeep);
+                       list, cap, st, deep, ndeep, pre);
     return hipGetLastError();
 }
 

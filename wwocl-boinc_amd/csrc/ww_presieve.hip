@@ -1,0 +1,144 @@
+/*
+ * ww_presieve.hip -- the wide pre-sieve of the persistent form.
+ *
+ * For every 8192-integer segment of a batch it writes 64 words (one per lane of the wave
+ * that will sieve the segment, bit b of word w = the odd number base + 128 w + 2 b) of
+ * numbers to drop: the odd multiples of the deep primes above the per-segment bound B1 and
+ * up to B2 (at most 2^22), except each prime's base-2 pseudoprime class ell mod
+ * m = ell * lcm(2, ord_ell(2)), which holds every strong pseudoprime the prime divides.
+ * So, as with the per-segment deep sieve, no member of the reference's PRP list is ever
+ * dropped; only composites are, before their strong PRP test.
+ *
+ * Work unit: a chunk of 256 consecutive segments (2^21 integers), split where it crosses
+ * from one range of the batch to the next. One workgroup marks a chunk in a 128 KB LDS
+ * bitmap, so each prime's start positions (two 64-bit Barrett reductions) are computed
+ * once per 2^20 odd numbers instead of once per 4096. Pseudoprime classes are collected
+ * in an LDS list (at most a few hundred per chunk; the table starts above 1024 for that);
+ * if the list overflows, the chunk drops nothing (exact, only slower). Pieces shorter than
+ * 16 segments are not pre-sieved.
+ */
+#include "ww_arith.h"
+#include "ww_batch.h"
+
+namespace ww {
+
+namespace {
+
+constexpr int kPreThreads = 1024;
+constexpr u32 kChunkSegs = 256;
+constexpr u32 kChunkPos = kChunkSegs * 4096; /* odd numbers per chunk */
+constexpr u32 kChunkWords = kChunkPos / 32;
+constexpr u32 kRescueCap = 6144;
+
+/* x mod m for any 64-bit x with bar = floor(2^64 / m): the Barrett quotient is the true
+   one or one less. */
+__device__ __forceinline__ u64 barrett64(u64 x, u64 m, u64 bar)
+{
+    u64 r = x - mul_hi(x, bar) * m;
+    return r >= m ? r - m : r;
+}
+
+__global__ __launch_bounds__(kPreThreads) void k_presieve(const BatchSpan *spans,
+                                                          const u64 *prefix, u32 nspans,
+                                                          const WidePrime *tab, u32 ntab,
+                                                          u64 *out)
+{
+    __shared__ u32 bm[kChunkWords];
+    __shared__ u32 rescue[kRescueCap];
+    __shared__ u32 rcount;
+
+    const u64 total = prefix[nspans];
+    const u64 nchunks = (total + kChunkSegs - 1) / kChunkSegs;
+    for (u64 c = blockIdx.x; c < nchunks; c += gridDim.x) {
+        const u64 s0 = c * kChunkSegs;
+        const u64 s1 = s0 + kChunkSegs < total ? s0 + kChunkSegs : total;
+        for (u32 w = threadIdx.x; w < kChunkWords; w += kPreThreads) bm[w] = 0;
+        if (threadIdx.x == 0) rcount = 0;
+        __syncthreads();
+
+        /* the chunk's pieces, each inside one range (the same values in every thread) */
+        for (u64 s = s0; s < s1;) {
+            u32 a = 0, b = nspans;
+            while (b - a > 1) {
+                const u32 mid = (a + b) >> 1;
+                if (prefix[mid] <= s) a = mid;
+                else b = mid;
+            }
+            const u64 e = prefix[a + 1] < s1 ? prefix[a + 1] : s1;
+            const BatchSpan sp = spans[a];
+            const u64 G = (sp.lo | 1) + (sp.seg0 + (s - prefix[a])) * kIntsPerWave;
+            const u32 off = (u32)(s - s0) * 4096; /* odd-number index of G in the chunk */
+            const u32 len = (u32)(e - s) * 4096;
+            /* a short piece (a small range, or the end of one) is left to the strong PRP
+               test: its words stay zero */
+            const u32 nprimes = e - s >= kPresieveMinSegs ? ntab : 0;
+            for (u32 j = threadIdx.x; j < nprimes; j += kPreThreads) {
+                const WidePrime d = tab[j];
+                const u64 ell = d.ell;
+                /* first odd multiple G + 2i of ell: -G mod ell, plus ell when odd */
+                const u64 r = barrett64(G, ell, d.bar_ell);
+                u64 dd = r ? ell - r : 0;
+                if (dd & 1) dd += ell;
+                u64 i = dd >> 1;
+                if (G <= ell && G + 2 * i == ell) i += ell; /* never ell itself */
+                for (; i < len; i += ell) {
+                    const u32 at = off + (u32)i;
+                    atomicOr(&bm[at >> 5], 1u << (at & 31));
+                }
+                /* members of the class ell mod m: G + 2k = ell (mod m), both odd */
+                const u64 rm = barrett64(G, d.m, d.bar_m);
+                for (u64 k = (ell >= rm ? ell - rm : ell + d.m - rm) >> 1; k < len; k += d.m >> 1) {
+                    const u32 at = atomicAdd(&rcount, 1u);
+                    if (at < kRescueCap) rescue[at] = off + (u32)k;
+                }
+            }
+            s = e;
+        }
+        __syncthreads();
+        const u32 nr = rcount;
+        if (nr <= kRescueCap)
+            for (u32 k = threadIdx.x; k < nr; k += kPreThreads)
+                atomicAnd(&bm[rescue[k] >> 5], ~(1u << (rescue[k] & 31)));
+        __syncthreads();
+        const u64 *bm64 = reinterpret_cast<const u64 *>(bm);
+        const u32 nw = (u32)(s1 - s0) * 64;
+        for (u32 w = threadIdx.x; w < nw; w += kPreThreads)
+            out[s0 * 64 + w] = nr <= kRescueCap ? bm64[w] : 0;
+        __syncthreads();
+    }
+}
+
+/* The single span [lo, hi) of a plain-form chunk, written on the device so that the
+   pre-sieve of a chunk needs no host copy. */
+__global__ void k_set_span(BatchSpan *span, u64 *prefix, u64 lo, u64 hi)
+{
+    span->lo = lo;
+    span->hi = hi;
+    span->seg0 = 0;
+    span->slot = 0;
+    span->pad = 0;
+    prefix[0] = 0;
+    prefix[1] = segments_of(lo, hi);
+}
+
+} // namespace
+
+hipError_t launch_presieve_range(u64 lo, u64 hi, BatchSpan *span, u64 *prefix,
+                                 const WidePrime *tab, u32 ntab, u64 *out, int grid,
+                                 hipStream_t s)
+{
+    hipLaunchKernelGGL(k_set_span, dim3(1), dim3(1), 0, s, span, prefix, lo, hi);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return launch_presieve(span, prefix, 1, tab, ntab, out, grid, s);
+}
+
+hipError_t launch_presieve(const BatchSpan *spans, const u64 *prefix, u32 nspans,
+                           const WidePrime *tab, u32 ntab, u64 *out, int grid, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_presieve, dim3(grid), dim3(kPreThreads), 0, s, spans, prefix, nspans,
+                       tab, ntab, out);
+    return hipGetLastError();
+}
+
+} // namespace ww

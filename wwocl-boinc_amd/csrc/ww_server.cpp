@@ -63,6 +63,7 @@ struct Tick {
     BatchState *d_state = nullptr, *h_state = nullptr;
     ww_result *d_res = nullptr, *h_res = nullptr;
     SlotSum *h_sums = nullptr;
+    uint64_t *d_pre = nullptr; /* wide pre-sieve of the tick's segments (or none) */
     hipEvent_t done = nullptr;
     bool busy = false;
     uint64_t segments = 0;
@@ -81,6 +82,9 @@ struct ww_server {
     SlotSum *d_sums = nullptr;
     const ww::DeepPrime *d_deep = nullptr;
     uint32_t ndeep = 0;
+    const ww::WidePrime *d_wide = nullptr;
+    uint32_t nwide = 0;
+    int cus = 0;
     uint32_t max_active = 0;
     Tick ticks[kRing];
     int ring_next = 0;
@@ -235,9 +239,15 @@ void engine_main(ww_server *s)
                                            hipMemcpyHostToDevice, s->stream), "spans") &&
                   hip_ok(s, hipMemcpyAsync(t.d_prefix, t.h_prefix, (nspans + 1) * sizeof(uint64_t),
                                            hipMemcpyHostToDevice, s->stream), "prefix") &&
+                  (!t.d_pre ||
+                   hip_ok(s, ww::launch_presieve(t.d_spans, t.d_prefix, nspans, s->d_wide, s->nwide,
+                                                 t.d_pre,
+                                                 (int)std::min<uint64_t>((total + 255) / 256,
+                                                                         (uint64_t)(s->cus > 0 ? s->cus : 1)),
+                                                 s->stream), "k_presieve")) &&
                   hip_ok(s, ww::launch_batch(t.d_spans, t.d_prefix, nspans, t.d_state, s->d_sums, 0,
                                              t.d_res, kTickRecords, s->grid, s->cfg.claim, s->d_deep,
-                                             s->ndeep, s->stream), "k_batch") &&
+                                             s->ndeep, t.d_pre, s->stream), "k_batch") &&
                   hip_ok(s, hipMemcpyAsync(t.h_state, t.d_state, sizeof(BatchState),
                                            hipMemcpyDeviceToHost, s->stream), "state") &&
                   hip_ok(s, hipMemcpyAsync(t.h_res, t.d_res, kTickCopy * sizeof(ww_result),
@@ -263,6 +273,7 @@ void free_tick(Tick &t)
     if (t.d_prefix) (void)hipFree(t.d_prefix);
     if (t.d_state) (void)hipFree(t.d_state);
     if (t.d_res) (void)hipFree(t.d_res);
+    if (t.d_pre) (void)hipFree(t.d_pre);
     if (t.h_spans) (void)hipHostFree(t.h_spans);
     if (t.h_prefix) (void)hipHostFree(t.h_prefix);
     if (t.h_state) (void)hipHostFree(t.h_state);
@@ -292,7 +303,8 @@ void destroy(ww_server *s)
 
 /* internal accessors of the context (ww_api.cpp) */
 int ww_ctx_internals(ww_ctx *c, int *device, const ww::DeepPrime **deep, uint32_t *ndeep,
-                     int *cus, int *tuned_blocks, uint32_t *tuned_claim);
+                     int *cus, int *tuned_blocks, uint32_t *tuned_claim,
+                     const ww::WidePrime **wide, uint32_t *nwide);
 int ww_set_error(int code, const char *fmt, ...);
 
 extern "C" {
@@ -320,7 +332,8 @@ int ww_server_create(ww_ctx *ctx, const ww_server_config *cfg, ww_server **out)
     int cus = 0, tuned_blocks = 0;
     uint32_t tuned_claim = 8;
     int rc = ww_ctx_internals(ctx, &s->device, &s->d_deep, &s->ndeep, &cus, &tuned_blocks,
-                              &tuned_claim);
+                              &tuned_claim, &s->d_wide, &s->nwide);
+    s->cus = cus;
     if (rc) {
         delete s;
         return rc;
@@ -351,6 +364,9 @@ int ww_server_create(ww_ctx *ctx, const ww_server_config *cfg, ww_server **out)
         if (e == hipSuccess) e = hipMalloc(&t.d_prefix, (s->cfg.slots + 1) * sizeof(uint64_t));
         if (e == hipSuccess) e = hipMalloc(&t.d_state, sizeof(BatchState));
         if (e == hipSuccess) e = hipMalloc(&t.d_res, kTickRecords * sizeof(ww_result));
+        if (e == hipSuccess && s->nwide)
+            e = hipMalloc(&t.d_pre, (uint64_t)s->cfg.tick_segments * ww::kPresieveWordsPerSeg *
+                                        sizeof(uint64_t));
         if (e == hipSuccess) e = hipHostMalloc(&t.h_spans, s->cfg.slots * sizeof(BatchSpan), 0);
         if (e == hipSuccess) e = hipHostMalloc(&t.h_prefix, (s->cfg.slots + 1) * sizeof(uint64_t), 0);
         if (e == hipSuccess) e = hipHostMalloc(&t.h_state, sizeof(BatchState), 0);

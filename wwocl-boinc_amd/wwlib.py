@@ -21,7 +21,9 @@ WW_E_NODEV = -3
 WW_E_CAPACITY = -4
 WW_MIN_START = 127
 WW_DEFAULT_RESULTS = 15000
-WW_DEFAULT_DEEP_BOUND = 16384
+WW_DEFAULT_DEEP_BOUND = 2048
+WW_DEFAULT_PRESIEVE_BOUND = 1048576
+WW_MAX_PRESIEVE_BOUND = 4194304
 
 U64 = ctypes.c_uint64
 U32 = ctypes.c_uint32
@@ -55,6 +57,10 @@ class RangeSum(ctypes.Structure):
 class DeepPrime(ctypes.Structure):
     _fields_ = [("ell", U32), ("m", U32), ("rho", U32), ("c32_m", U32), ("bar_ell", U32),
                 ("pad", U32), ("bar_m", U64)]
+
+
+class WidePrime(ctypes.Structure):
+    _fields_ = [("ell", U32), ("pad", U32), ("m", U64), ("bar_ell", U64), ("bar_m", U64)]
 
 
 class Tuning(ctypes.Structure):
@@ -105,6 +111,8 @@ _SIGNATURES = [
     ("ww_list_bound", U64, [U64, U64]),
     ("ww_set_deep_bound", ctypes.c_int, [ctypes.c_void_p, U32]),
     ("ww_get_deep_bound", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(U32)]),
+    ("ww_set_presieve_bound", ctypes.c_int, [ctypes.c_void_p, U32]),
+    ("ww_get_presieve_bound", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(U32)]),
     ("ww_clear_results", ctypes.c_int, [ctypes.c_void_p]),
     ("ww_clear_prp_count", ctypes.c_int, [ctypes.c_void_p]),
     ("ww_getsegprimes", ctypes.c_int, [ctypes.c_void_p, U64, U64]),
@@ -153,6 +161,7 @@ _SIGNATURES = [
     ("ww_cpu_is_prp", ctypes.c_int, [U64]),
     ("ww_cpu_range", ctypes.c_int, [U64, U64, ctypes.POINTER(U64)]),
     ("ww_deep_table", U32, [U32, ctypes.POINTER(DeepPrime), U32]),
+    ("ww_presieve_table", U32, [U32, U32, ctypes.POINTER(WidePrime), U32]),
 ]
 
 EXPORTED = [s[0] for s in _SIGNATURES]
@@ -231,6 +240,14 @@ def deep_table(bound: int):
             for i in range(n)]
 
 
+def presieve_table(above: int, bound: int):
+    """[(ell, m, bar_ell, bar_m)] of the wide pre-sieve, above < ell <= bound (host-side)."""
+    cap = 300000
+    buf = (WidePrime * cap)()
+    n = load().ww_presieve_table(above, bound, buf, cap)
+    return [(buf[i].ell, buf[i].m, buf[i].bar_ell, buf[i].bar_m) for i in range(n)]
+
+
 def cpu_range(low: int, high: int):
     out = (U64 * 3)()
     _check(load().ww_cpu_range(low, high, out))
@@ -284,6 +301,14 @@ class Context:
     def deep_bound(self) -> int:
         v = U32()
         _check(self.lib.ww_get_deep_bound(self.h, ctypes.byref(v)))
+        return v.value
+
+    def set_presieve_bound(self, bound: int):
+        _check(self.lib.ww_set_presieve_bound(self.h, bound))
+
+    def presieve_bound(self) -> int:
+        v = U32()
+        _check(self.lib.ww_get_presieve_bound(self.h, ctypes.byref(v)))
         return v.value
 
     def autotune(self, lo: int, hi: int, bounds=None, blocks=None, claims=None):
