@@ -2,8 +2,8 @@
 
 One step = the whole range [2^63, 2^63 + 2^33) (196,705,536 tested primes) through the HIP
 path, inputs generated on the device, the checksum accumulated on the device. The headline
-`value` is the persistent form (one graph-launched k_batch: sieve, PRP, Wieferich and
-Wall-Sun-Sun in one kernel); the plain form (per 2^32-wide chunk: clear the list, sieve +
+`value` is the persistent form (the wide pre-sieve k_presieve, then one graph-launched
+k_batch: sieve, PRP, Wieferich and Wall-Sun-Sun in one kernel); the plain form (per 2^32-wide chunk: clear the list, sieve +
 PRP, then the test kernel) runs beside it and must give the same step checksum. Also: the
 4096-range batched workload, the autotuner, the serving form's latency, and the reference's
 own CPU path on a bounded sample. Prints one JSON line.
@@ -33,11 +33,13 @@ BITS = 64  # every p in the range has bit length 64
 # launch (factors from a 1 GiB copy kernel) over the entries of the launch, measured with
 # rocprofv3 --pmc on MI355X for this workload. Algorithmic: the 8-byte list read.
 TRAFFIC_BYTES_PER_ENTRY = 8.00
-# k_batch keeps candidates and PRPs on chip (LDS queues); its HBM traffic per tested
-# prime is the deep-sieve table, span table lookups and the cursor/slot atomics only (each
-# atomic reads as a 32-byte write), measured the same way at the default deep bound 16384:
-# 11.9 MB fetched + 15.8 MB written per step.
-BATCH_TRAFFIC_BYTES_PER_PRP = 0.141
+# k_batch keeps candidates and PRPs on chip (LDS queues). Its algorithmic HBM traffic is the
+# wide pre-sieve's 64 words (512 B) per 8192-integer segment, read once; the rest is the
+# deep-sieve table, span lookups and the cursor/slot atomics (each atomic reads as a 32-byte
+# write). Measured the same way at the defaults (deep bound 2048, pre-sieve 2^20):
+# 572.1 MB fetched + 17.8 MB written per step.
+BATCH_TRAFFIC_BYTES_PER_PRP = 2.999
+PRESIEVE_BYTES_PER_SEGMENT = 512
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
 
 
@@ -119,7 +121,7 @@ def run_plain(ctx, steps, warmup):
 
 
 def run_persistent(ctx, steps, warmup):
-    """Persistent form: the whole range in one graph-launched k_batch."""
+    """Persistent form: the whole range in one graph-launched k_presieve + k_batch."""
     with wwlib.Batch(ctx, [(LO, HI)]) as b:
         res = None
         for _ in range(max(warmup, 1)):
@@ -127,10 +129,13 @@ def run_persistent(ctx, steps, warmup):
             res = b.read()[0]
         ctx.finish()
         t0 = time.perf_counter()
-        kms = 0.0
+        kms = pre_ms = search_ms = 0.0
         for _ in range(steps):
             b.launch(graph=True)
             kms += b.time_ms()
+            pre, search = b.kernel_times_ms()
+            pre_ms += pre
+            search_ms += search
         ctx.finish()
         elapsed = time.perf_counter() - t0
         last = b.read()[0]
@@ -138,7 +143,8 @@ def run_persistent(ctx, steps, warmup):
     prps, cks = res[0], res[1]
     return {"value": prps * steps / elapsed, "ms_per_step": elapsed / steps * 1e3,
             "prps_per_step": prps, "step_checksum": f"{cks:016X}",
-            "kernel_ms_per_step": kms / steps}
+            "kernel_ms_per_step": kms / steps,
+            "kernels_ms_per_step": {"k_presieve": pre_ms / steps, "k_batch": search_ms / steps}}
 
 
 def run_batched_workload(ctx):
@@ -212,13 +218,13 @@ def main():
                     help="length of each serving run (0 = skip serving)")
     ap.add_argument("--serve-rates", default="20,50,100", help="request rates (per second)")
     ap.add_argument("--no-tune", action="store_true",
-                    help="skip the autotuner (defaults: deep bound 16384, full occupancy)")
+                    help="skip the autotuner (library defaults: deep bound 2048, full occupancy)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="also launch one plain copy kernel of BYTES (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
-              "(ranges shard trivially across GPUs, see DESIGN.md)", file=sys.stderr)
+              "(ranges shard trivially across GPUs: one sub-range per GPU, checksums summed)", file=sys.stderr)
         sys.exit(2)
 
     info = wwlib.device_info(0)
@@ -260,7 +266,8 @@ def main():
         "vs_baseline": None,
         "dtype": "u64",
         "data": "synthetic: the number line [2^63, 2^63+2^33), generated on the device",
-        "config": {"workload": ("persistent form (one graph-launched k_batch)" if pers else
+        "config": {"workload": ("persistent form (k_presieve + one k_batch, graph-launched)"
+                                if pers else
                                 "plain form (2^32-wide chunks)") +
                                ", range [2^63, 2^63+2^33)",
                    "range": [LO, HI], "prps_per_step": head["prps_per_step"],
@@ -268,26 +275,28 @@ def main():
                    "arch": info.arch.decode(), "cus": info.compute_units},
     }
     if pers is not None:
-        kern_s = pers["kernel_ms_per_step"] * 1e-3
+        kern_s = pers["kernels_ms_per_step"]["k_batch"] * 1e-3
         ach = pers["prps_per_step"] * mul32_path(BITS) / kern_s
         out["roofline"] = {
             "bound": "valu", "kernel": "k_batch", "achieved": ach / 1e12, "peak": peak / 1e12,
             "unit": "TMUL32/s", "frac": ach / peak,
             "traffic": pers["prps_per_step"] * BATCH_TRAFFIC_BYTES_PER_PRP,
-            "traffic_algorithmic": 0,
+            "traffic_algorithmic": ((HI - LO) // 8192) * PRESIEVE_BYTES_PER_SEGMENT,
             "per_unit": f"{mul32_path(BITS)} MUL32 per tested prime (A(b) = 99b-123 at b=64: "
                         "PRP + Wieferich + Wall-Sun-Sun in one kernel, counted as the "
                         "reference's algorithm spends them)",
             "executed_per_unit": f"{mul32_path_executed(BITS)} MUL32 per tested prime "
                                  "(78b-117: this build's ladders)",
-            "avg_launch_ms": pers["kernel_ms_per_step"], "launches": args.steps}
+            "avg_launch_ms": pers["kernels_ms_per_step"]["k_batch"], "launches": args.steps,
+            "step_kernels_ms": pers["kernels_ms_per_step"]}
         # the same kernel against HBM: measured bytes per launch over the launch time
         hbm = pers["prps_per_step"] * BATCH_TRAFFIC_BYTES_PER_PRP / kern_s
         out["roofline_hbm"] = {
             "bound": "hbm", "kernel": "k_batch", "achieved": hbm / 1e9, "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": hbm / 1e9 / HBM_PEAK_GBS,
             "traffic": pers["prps_per_step"] * BATCH_TRAFFIC_BYTES_PER_PRP,
-            "note": "not the bound: the kernel keeps candidates and PRPs on chip"}
+            "note": "not the bound: the kernel keeps candidates and PRPs on chip and reads "
+                    "only the pre-sieve words (512 B per 8192 integers) from HBM"}
     if plain is not None:
         tk = plain["tests_kernel"]
         key = "roofline" if pers is None else "roofline_plain_tests_kernel"

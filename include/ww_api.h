@@ -245,12 +245,14 @@ int ww_batch_create(ww_ctx *ctx, const ww_range *ranges, uint32_t n, uint32_t re
    claims shrink toward one segment as the launch's work runs out. */
 int ww_batch_set_grid(ww_batch *b, int blocks_per_cu, uint32_t claim);
 
-/* Stream-ordered search of every range of the batch: clears the batch's sums, then one
-   persistent kernel whose waves claim 8192-integer segments from a shared cursor.
+/* Stream-ordered search of every range of the batch: the wide pre-sieve of its segments
+   (when the context's pre-sieve bound is set), then one persistent kernel that clears the
+   batch's sums and whose waves claim 8192-integer segments from a shared cursor.
    Allocates nothing and records no memset, so it may run under graph capture. */
 int ww_batch_launch(ww_ctx *ctx, ww_batch *b);
 
-/* The same launch captured once into a HIP graph on first use and replayed. */
+/* The same launch captured once into HIP graphs on first use (one for the pre-sieve, one
+   for the search) and replayed. */
 int ww_batch_launch_graph(ww_ctx *ctx, ww_batch *b);
 
 /* Blocking read: per-range sums (n entries, in table order) and up to `cap` records
@@ -261,6 +263,10 @@ int ww_batch_read(ww_ctx *ctx, ww_batch *b, ww_range_sum *sums, ww_result *recs,
 
 /* Kernel milliseconds of the last completed batch launch (HIP events on the stream). */
 int ww_batch_time(ww_ctx *ctx, ww_batch *b, double *ms);
+
+/* The same launch split: the pre-sieve kernel alone and the persistent search kernel
+   alone (their sum is ww_batch_time). Either pointer may be null. */
+int ww_batch_kernel_times(ww_ctx *ctx, ww_batch *b, double *presieve_ms, double *search_ms);
 
 int ww_batch_destroy(ww_batch *b);
 

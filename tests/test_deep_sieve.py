@@ -46,6 +46,42 @@ def test_table_to_the_largest_bound():
         assert bar_ell == (1 << 32) // ell and bar_m == (1 << 64) // m
 
 
+def order2_factored(ell):
+    """ord_ell(2) from the prime factors of ell - 1 (for primes too large to step)."""
+    o, n, q = ell - 1, ell - 1, 2
+    while q * q <= n:
+        if n % q == 0:
+            while n % q == 0:
+                n //= q
+            while o % q == 0 and pow(2, o // q, ell) == 1:
+                o //= q
+        q += 1
+    if n > 1 and pow(2, o // n, ell) == 1:
+        o //= n
+    return o
+
+
+def test_presieve_table_entries():
+    """ww_presieve_table: every prime in (above, bound], with m = ell * lcm(2, ord_ell(2))
+    and the 64-bit Barrett reciprocals of ell and m."""
+    t = wwlib.presieve_table(2048, 1 << 20)
+    assert len(t) == 82025 - 309  # pi(2^20) - pi(2048)
+    assert t[0][0] == 2053 and t[-1][0] == 1048573
+    assert all(a[0] < b[0] for a, b in zip(t, t[1:]))
+    rng = random.Random(6)
+    for ell, m, bar_ell, bar_m in t[:40] + rng.sample(t, 400) + t[-5:]:
+        assert oracle.is_prime(ell)
+        o = order2_factored(ell)
+        if ell < 20000:
+            assert o == order2(ell)
+        assert pow(2, o, ell) == 1
+        assert m == ell * (o if o % 2 == 0 else 2 * o)
+        assert bar_ell == (1 << 64) // ell and bar_m == (1 << 64) // m
+    top = wwlib.presieve_table(1 << 20, 1 << 22)
+    assert len(top) == 295947 - 82025 and top[-1][0] == 4194301  # pi(2^22) - pi(2^20)
+    assert [e for e, *_ in wwlib.presieve_table(2048, 4096)] == [e for e, *_ in t if e <= 4096]
+
+
 def survivors(lo, hi, bound):
     """Odd n in [lo, hi) with no factor 3..113 left by the deep sieve to `bound`
     (every multiple of a deep prime marked, then its class rho mod m cleared)."""

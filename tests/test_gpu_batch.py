@@ -91,9 +91,19 @@ def test_batch_single_range_equals_plain_at_full_size(gctx):
         b.launch()
         (prp, cks, recs), = b.read()
         ms = b.time_ms()
+        pre, search = b.kernel_times_ms()
+        b.launch(graph=True)
+        assert b.read() == [(prp, cks, recs)]
+        gms = b.time_ms()
+        gpre, gsearch = b.kernel_times_ms()
     plain = gctx.run_range(lo, hi, 1 << 32)
-    print(f"persistent cfg3: {prp} PRPs in {ms:.1f} ms = {prp / ms * 1e3:.3e} /s")
+    print(f"persistent 2^33 at 2^63: {prp} PRPs in {ms:.1f} ms = {prp / ms * 1e3:.3e} /s "
+          f"(pre-sieve {pre:.2f} ms, search {search:.2f} ms)")
     assert (prp, cks, recs) == (plain.prp_count, plain.checksum, plain.results)
+    # the split times cover the whole launch, eager and graph-launched
+    for total, a, s in ((ms, pre, search), (gms, gpre, gsearch)):
+        assert a > 0 and s > 0
+        assert abs(a + s - total) <= 0.05 + 0.01 * total
 
 
 def test_autotune_keeps_results(gctx):

@@ -11,6 +11,10 @@ from test_deep_sieve import KNOWN_SPSP2
 pytestmark = pytest.mark.gpu
 
 BOUNDS = [0, 127, 1024, 4096, 8192, 20011, 65535]
+# strong pseudoprimes (base 2) with a factor above 16384, which the wide pre-sieve must keep:
+# 3215031751 = 151 * 751 * 28351, 2152302898747 = 6763 * 10627 * 29947,
+# 3474749660383 = 1303 * 16927 * 157543
+WIDE_SPSP2 = (3215031751, 2152302898747, 3474749660383)
 
 
 @pytest.fixture
@@ -26,7 +30,10 @@ def test_list_and_ranges_at_bound(deep_ctx, bound):
     ctx = deep_ctx
     ctx.set_deep_bound(bound)
     assert ctx.deep_bound() == bound
-    for lo, hi in [(127, 3_000_000)] + [(n - 5000, n + 5000) for n in KNOWN_SPSP2[7:]]:
+    # windows of 32 segments around WIDE_SPSP2, so that the plain form pre-sieves them too
+    windows = [(n - 5000, n + 5000) for n in KNOWN_SPSP2[7:]]
+    windows += [(n - (1 << 17), n + (1 << 17)) for n in WIDE_SPSP2]
+    for lo, hi in [(127, 3_000_000)] + windows:
         ctx.clear_results()
         ctx.clear_prp_count()
         ctx.getsegprimes(lo, hi)
@@ -57,9 +64,6 @@ def test_batch_at_bound(deep_ctx, bound):
         assert g == oracle.range_results(lo, hi), (bound, lo, hi)
 
 
-# strong pseudoprimes (base 2) with a factor above 16384, which the wide pre-sieve must keep:
-# 3215031751 = 151 * 751 * 28351, 2152302898747 = 6763 * 10627 * 29947,
-# 3474749660383 = 1303 * 16927 * 157543
 @pytest.mark.parametrize("deep,pre", [(0, 1 << 22), (4096, 1 << 22), (16384, 1 << 20),
                                       (65535, 1 << 22), (16384, 20011)])
 def test_batch_with_wide_presieve(deep_ctx, deep, pre):
@@ -67,10 +71,13 @@ def test_batch_with_wide_presieve(deep_ctx, deep, pre):
     ctx.set_deep_bound(deep)
     ctx.set_presieve_bound(pre)
     assert (ctx.deep_bound(), ctx.presieve_bound()) == (deep, pre)
-    for n in (3215031751, 2152302898747, 3474749660383):
+    for n in WIDE_SPSP2:
         assert oracle.is_prp(n) and not oracle.is_prime(n)
     ranges = [(n - 3000, n + 3000) for n in KNOWN_SPSP2] + [(127, 300_000), (127, 4_000_000)]
     ranges += [(3215031751 - (1 << 21), 3215031751 + (1 << 21))]  # whole pre-sieve chunks
+    # 16 and 15 segments: the shortest piece that is pre-sieved, and one that is not
+    ranges += [(n - (1 << 16), n + (1 << 16)) for n in WIDE_SPSP2]
+    ranges += [(n - 61440, n + 61440) for n in WIDE_SPSP2]
     ranges += wwlib.batch_workload(n=48, seed=5, wmax=21)
     ranges += [(1 << 63, (1 << 63) + (1 << 22)), ((1 << 64) - (1 << 22), (1 << 64) - 1)]
     with wwlib.Batch(ctx, ranges) as b:

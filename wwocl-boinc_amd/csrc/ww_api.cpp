@@ -585,13 +585,13 @@ struct ww_batch {
     ww_result *d_res = nullptr;
     uint64_t *d_pre = nullptr; /* wide pre-sieve words, 64 per segment (or none) */
     uint64_t nseg = 0;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph = nullptr, graph_pre = nullptr;
+    hipGraphExec_t exec = nullptr, exec_pre = nullptr;
     int graph_grid = 0;
     uint32_t graph_claim = 0;
     uint32_t graph_ndeep = 0;
     uint64_t graph_wide = 0;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev0 = nullptr, evm = nullptr, ev1 = nullptr;
 };
 
 namespace {
@@ -603,21 +603,59 @@ int batch_grid(ww_ctx *c, ww_batch *b)
     return c->cus * nb;
 }
 
-int batch_enqueue(ww_ctx *c, ww_batch *b, int grid)
+/* Whether this launch pre-sieves (the batch has the buffer and the context active
+   pre-sieve primes). */
+bool batch_presieves(ww_ctx *c, ww_batch *b)
 {
     uint32_t first, count;
     wide_span(c, &first, &count);
-    const uint64_t *pre = nullptr;
-    if (count && b->d_pre) {
-        const uint64_t chunks = (b->nseg + 255) / 256;
-        const int pgrid = (int)std::min<uint64_t>(chunks, (uint64_t)c->cus);
-        HIP_TRY(ww::launch_presieve(b->d_spans, b->d_prefix, b->n, c->d_wide + first, count,
-                                    b->d_pre, pgrid > 0 ? pgrid : 1, c->stream));
-        pre = b->d_pre;
-    }
-    HIP_TRY(ww::launch_batch(b->d_spans, b->d_prefix, b->n, b->d_state, b->d_sums, b->n, b->d_res,
-                             b->res_cap, grid, b->claim, c->d_deep, c->ndeep, pre, c->stream));
+    return count && b->d_pre;
+}
+
+int batch_enqueue_presieve(ww_ctx *c, ww_batch *b)
+{
+    uint32_t first, count;
+    wide_span(c, &first, &count);
+    const uint64_t chunks = (b->nseg + 255) / 256;
+    const int pgrid = (int)std::min<uint64_t>(chunks, (uint64_t)c->cus);
+    HIP_TRY(ww::launch_presieve(b->d_spans, b->d_prefix, b->n, c->d_wide + first, count, b->d_pre,
+                                pgrid > 0 ? pgrid : 1, c->stream));
     return WW_OK;
+}
+
+int batch_enqueue_search(ww_ctx *c, ww_batch *b, int grid, bool pre)
+{
+    HIP_TRY(ww::launch_batch(b->d_spans, b->d_prefix, b->n, b->d_state, b->d_sums, b->n, b->d_res,
+                             b->res_cap, grid, b->claim, c->d_deep, c->ndeep,
+                             pre ? b->d_pre : nullptr, c->stream));
+    return WW_OK;
+}
+
+/* Capture `enqueue` on the context's stream into an instantiated graph. */
+template <class F> int capture(ww_ctx *c, hipGraph_t *graph, hipGraphExec_t *exec, F enqueue)
+{
+    HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue();
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return fail(WW_E_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    *graph = g;
+    HIP_TRY(hipGraphInstantiate(exec, g, nullptr, nullptr, 0));
+    return WW_OK;
+}
+
+void drop_graphs(ww_batch *b)
+{
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    if (b->exec_pre) (void)hipGraphExecDestroy(b->exec_pre);
+    if (b->graph_pre) (void)hipGraphDestroy(b->graph_pre);
+    b->exec = b->exec_pre = nullptr;
+    b->graph = b->graph_pre = nullptr;
 }
 
 } // namespace
@@ -665,6 +703,7 @@ int ww_batch_create(ww_ctx *c, const ww_range *ranges, uint32_t n, uint32_t resu
         e = hipMemcpyAsync(b->d_prefix, prefix.data(), (n + 1) * sizeof(uint64_t), hipMemcpyHostToDevice,
                            c->stream);
     if (e == hipSuccess) e = hipEventCreate(&b->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&b->evm);
     if (e == hipSuccess) e = hipEventCreate(&b->ev1);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) {
@@ -688,8 +727,11 @@ int ww_batch_launch(ww_ctx *c, ww_batch *b)
     int rc = use(c);
     if (rc) return rc;
     if (!b) return fail(WW_E_ARG, "null batch");
+    const bool pre = batch_presieves(c, b);
     HIP_TRY(hipEventRecord(b->ev0, c->stream));
-    if ((rc = batch_enqueue(c, b, batch_grid(c, b)))) return rc;
+    if (pre && (rc = batch_enqueue_presieve(c, b))) return rc;
+    HIP_TRY(hipEventRecord(b->evm, c->stream));
+    if ((rc = batch_enqueue_search(c, b, batch_grid(c, b), pre))) return rc;
     HIP_TRY(hipEventRecord(b->ev1, c->stream));
     return WW_OK;
 }
@@ -703,28 +745,30 @@ int ww_batch_launch_graph(ww_ctx *c, ww_batch *b)
     uint32_t wfirst, wcount;
     wide_span(c, &wfirst, &wcount);
     const uint64_t wide = ((uint64_t)wfirst << 32) | wcount;
+    const bool pre = batch_presieves(c, b);
     if (b->exec && (b->graph_grid != grid || b->graph_claim != b->claim ||
-                    b->graph_ndeep != c->ndeep || b->graph_wide != wide)) {
-        (void)hipGraphExecDestroy(b->exec);
-        (void)hipGraphDestroy(b->graph);
-        b->exec = nullptr;
-        b->graph = nullptr;
-    }
+                    b->graph_ndeep != c->ndeep || b->graph_wide != wide))
+        drop_graphs(b);
     if (!b->exec) {
-        HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        rc = batch_enqueue(c, b, grid);
-        hipGraph_t g = nullptr;
-        hipError_t e = hipStreamEndCapture(c->stream, &g);
-        if (rc) return rc;
-        if (e != hipSuccess) return fail(WW_E_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
-        b->graph = g;
-        HIP_TRY(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+        if (pre && (rc = capture(c, &b->graph_pre, &b->exec_pre,
+                                 [&] { return batch_enqueue_presieve(c, b); }))) {
+            drop_graphs(b);
+            return rc;
+        }
+        if ((rc = capture(c, &b->graph, &b->exec,
+                          [&] { return batch_enqueue_search(c, b, grid, pre); }))) {
+            drop_graphs(b);
+            return rc;
+        }
         b->graph_grid = grid;
         b->graph_claim = b->claim;
         b->graph_ndeep = c->ndeep;
         b->graph_wide = wide;
     }
+    /* two graphs, so that the events between them time each kernel on its own */
     HIP_TRY(hipEventRecord(b->ev0, c->stream));
+    if (b->exec_pre) HIP_TRY(hipGraphLaunch(b->exec_pre, c->stream));
+    HIP_TRY(hipEventRecord(b->evm, c->stream));
     HIP_TRY(hipGraphLaunch(b->exec, c->stream));
     HIP_TRY(hipEventRecord(b->ev1, c->stream));
     return WW_OK;
@@ -767,14 +811,28 @@ int ww_batch_time(ww_ctx *c, ww_batch *b, double *ms)
     return WW_OK;
 }
 
+int ww_batch_kernel_times(ww_ctx *c, ww_batch *b, double *presieve_ms, double *search_ms)
+{
+    int rc = use(c);
+    if (rc) return rc;
+    if (!b) return fail(WW_E_ARG, "null batch");
+    float t0 = 0, t1 = 0;
+    HIP_TRY(hipEventSynchronize(b->ev1));
+    HIP_TRY(hipEventElapsedTime(&t0, b->ev0, b->evm));
+    HIP_TRY(hipEventElapsedTime(&t1, b->evm, b->ev1));
+    if (presieve_ms) *presieve_ms = t0;
+    if (search_ms) *search_ms = t1;
+    return WW_OK;
+}
+
 int ww_batch_destroy(ww_batch *b)
 {
     if (!b) return WW_OK;
     (void)hipSetDevice(b->device);
     (void)hipDeviceSynchronize();
-    if (b->exec) (void)hipGraphExecDestroy(b->exec);
-    if (b->graph) (void)hipGraphDestroy(b->graph);
+    drop_graphs(b);
     if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->evm) (void)hipEventDestroy(b->evm);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
     if (b->d_pre) (void)hipFree(b->d_pre);
     if (b->d_res) (void)hipFree(b->d_res);

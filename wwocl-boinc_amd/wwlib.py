@@ -140,6 +140,9 @@ _SIGNATURES = [
                                      ctypes.POINTER(U32)]),
     ("ww_batch_time", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p,
                                      ctypes.POINTER(ctypes.c_double)]),
+    ("ww_batch_kernel_times", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p,
+                                             ctypes.POINTER(ctypes.c_double),
+                                             ctypes.POINTER(ctypes.c_double)]),
     ("ww_batch_destroy", ctypes.c_int, [ctypes.c_void_p]),
     ("ww_autotune", ctypes.c_int, [ctypes.c_void_p, U64, U64, ctypes.POINTER(U32), U32,
                                    ctypes.POINTER(ctypes.c_int32), U32, ctypes.POINTER(U32), U32,
@@ -425,6 +428,13 @@ class Batch:
         v = ctypes.c_double()
         _check(self.lib.ww_batch_time(self.ctx.h, self.h, ctypes.byref(v)))
         return v.value
+
+    def kernel_times_ms(self):
+        """(pre-sieve kernel, search kernel) milliseconds of the last launch."""
+        pre, search = ctypes.c_double(), ctypes.c_double()
+        _check(self.lib.ww_batch_kernel_times(self.ctx.h, self.h, ctypes.byref(pre),
+                                              ctypes.byref(search)))
+        return pre.value, search.value
 
     def read(self):
         """[(prp_count, checksum, sorted records)] per range, in table order."""
