@@ -67,7 +67,8 @@ int or_strong_prp(uint64_t base, uint64_t n)
     uint64_t nm1 = n - 1;
     int t = __builtin_ctzll(nm1);
     uint64_t d = n >> t;                 /* (n-1)/2^t: the shift drops the low 1 bit */
-    uint64_t bit = 0x8000000000000000ull >> (__builtin_clzll(d) + 1);
+    int lead = 63 - __builtin_clzll(d);  /* bits below the leading one of d */
+    uint64_t bit = lead ? 1ull << (lead - 1) : 0;  /* d = 1 (n = 2^t + 1): no bits */
 
     uint64_t mb = or_mont_mul(base, r2, n, q);
     uint64_t a = mb;

@@ -8,7 +8,7 @@ import oracle
 import wwlib
 
 KNOWN_SPSP2 = [42799, 49141, 88357, 90751, 104653, 130561, 196093,
-               1373653, 25326001, 3215031751, 2152302898747, 3474749660383]
+               1373653, 25326001, 3215031751, 4294967297, 2152302898747, 3474749660383]
 
 
 def order2(ell):

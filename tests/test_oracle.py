@@ -98,6 +98,34 @@ def test_spsp2_membership():
         assert pow(2, n - 1, n) == 1 and not oracle.is_prp(n)
 
 
+def strong_prp_definition(b, n):
+    """Miller-Rabin to base b from its definition, with Python's pow."""
+    d, t = n - 1, 0
+    while d % 2 == 0:
+        d, t = d // 2, t + 1
+    x = pow(b, d, n)
+    if x in (1, n - 1):
+        return True
+    for _ in range(t - 1):
+        x = x * x % n
+        if x == n - 1:
+            return True
+    return False
+
+
+def test_strong_prp_when_n_minus_1_is_a_power_of_two():
+    """n = 2^t + 1, where the odd part of n - 1 is 1 (no exponent bits below the leading
+    one): Fermat primes, 2^32 + 1 = 641 * 6700417 (a strong pseudoprime to base 2) and
+    composites that are not."""
+    for t in range(2, 64):
+        n = (1 << t) + 1
+        for b in (2, 3, 5):
+            if n % b:
+                assert oracle.strong_prp(b, n) == strong_prp_definition(b, n), (b, t)
+    assert oracle.strong_prp(2, (1 << 32) + 1) and oracle.is_prp((1 << 32) + 1)
+    assert not oracle.strong_prp(2, (1 << 40) + 1)
+
+
 def test_ref_strong_prp_agrees_on_random_inputs():
     r = oracle.ref_lib()
     if r is None:

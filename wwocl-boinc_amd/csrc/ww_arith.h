@@ -148,11 +148,43 @@ WW_HD u128 sqr_wide_add(u64 a, u64 c)
  * backend reuses the carry-out of v_sub_co/v_add_co instead of comparing again.
  * ------------------------------------------------------------------------- */
 
-/* x - y, and whether it borrowed. */
+/* x - y, and whether it borrowed. Written as two 32-bit subtractions: the backend then
+   takes the borrow from v_subb_co's carry-out, where the 64-bit builtin gets a second
+   64-bit compare (the high word borrows when x_hi < y_hi, or when they are equal and the
+   low word borrowed). */
 WW_HD u64 sub_b(u64 x, u64 y, bool *borrow)
 {
-    u64 r;
-    *borrow = __builtin_sub_overflow(x, y, &r);
+    u32 rl, t, rh;
+    bool b1 = __builtin_sub_overflow((u32)x, (u32)y, &rl);
+    bool b2 = __builtin_sub_overflow((u32)(x >> 32), (u32)(y >> 32), &t);
+    bool b3 = __builtin_sub_overflow(t, (u32)b1, &rh);
+    *borrow = b2 | b3;
+    return ((u64)rh << 32) | rl;
+}
+
+/* a + b + c for a carry c in {0, 1}, and the carry out (one v_addc_co on the device). */
+WW_HD u32 addc32(u32 a, u32 b, bool *c)
+{
+    u32 t, r;
+    bool c1 = __builtin_add_overflow(a, b, &t);
+    bool c2 = __builtin_add_overflow(t, (u32)*c, &r);
+    *c = c1 | c2;
+    return r;
+}
+
+/* a + e as four 32-bit words, with the carry out of bit 127: a v_add_co / v_addc_co chain
+   whose last carry is the result, instead of 64-bit adds each followed by a compare. */
+WW_HD u128 add128_c(u128 a, u128 e, bool *carry)
+{
+    bool c = false;
+    u32 r0 = addc32((u32)a.lo, (u32)e.lo, &c);
+    u32 r1 = addc32((u32)(a.lo >> 32), (u32)(e.lo >> 32), &c);
+    u32 r2 = addc32((u32)a.hi, (u32)e.hi, &c);
+    u32 r3 = addc32((u32)(a.hi >> 32), (u32)(e.hi >> 32), &c);
+    *carry = c;
+    u128 r;
+    r.lo = ((u64)r1 << 32) | r0;
+    r.hi = ((u64)r3 << 32) | r2;
     return r;
 }
 
@@ -178,6 +210,26 @@ WW_HD u64 sub_mod(u64 x, u64 y, u64 p)
     bool b;
     u64 r = sub_b(x, y, &b);
     return b ? r + p : r;
+}
+
+/* a - b - c for a borrow c in {0, 1}, and the borrow out (one v_subb_co on the device). */
+WW_HD u32 subb32(u32 a, u32 b, bool *c)
+{
+    u32 t, r;
+    bool b1 = __builtin_sub_overflow(a, b, &t);
+    bool b2 = __builtin_sub_overflow(t, (u32)*c, &r);
+    *c = b1 | b2;
+    return r;
+}
+
+/* x - (y + c) mod p for x < p, y < p and a borrow c in {0, 1}: the borrow enters the
+   subtraction's carry chain instead of being added to y first. */
+WW_HD u64 sub_mod_c(u64 x, u64 y, bool c, u64 p)
+{
+    u32 r0 = subb32((u32)x, (u32)y, &c);
+    u32 r1 = subb32((u32)(x >> 32), (u32)(y >> 32), &c);
+    u64 r = ((u64)r1 << 32) | r0;
+    return c ? r + p : r;
 }
 
 /* p^-1 mod 2^64 for odd p: Newton from the 5-bit seed (3p) xor 2, three 32-bit steps
@@ -395,7 +447,7 @@ WW_HD m2 m2_sub(m2 x, m2 y, u64 p)
     bool b = x.d0 < y.d0;
     m2 r;
     r.d0 = x.d0 - y.d0 + (b ? p : 0);
-    r.d1 = sub_mod(x.d1, y.d1 + (b ? 1 : 0), p);
+    r.d1 = sub_mod_c(x.d1, y.d1, b, p);
     return r;
 }
 
@@ -426,7 +478,7 @@ WW_HD m2 m2_redc(u128 t, u64 c_lo, u64 c_hi, u64 c_top, u64 p, u64 q)
     bool b = t.hi < v1;
     m2 r;
     r.d0 = t.hi - v1 + (b ? p : 0);
-    r.d1 = sub_mod(t1p, v1p + (b ? 1 : 0), p);
+    r.d1 = sub_mod_c(t1p, v1p, b, p);
     return r;
 }
 
@@ -439,17 +491,14 @@ WW_HD m2 m2_sqr(m2 x, u64 p, u64 q, u64 np)
     u128 t = sqr_wide(x.d0);
     u64 u0 = t.lo * q;
     u64 v1 = mul_hi(p, u0);
-    u128 c = mul_wide(x.d0, dbl_mod(x.d1, p, np));
-    u64 tlo;
-    bool cy = __builtin_add_overflow(c.lo, u0, &tlo);
-    u64 t1p = c.hi + (u64)cy; /* < p */
-    u64 up0 = tlo * q;
+    u128 c = mul_wide_add(x.d0, dbl_mod(x.d1, p, np), u0); /* high word < p */
+    u64 up0 = c.lo * q;
     u64 v1p = mul_hi(p, up0);
     bool b;
     u64 z0 = sub_b(t.hi, v1, &b);
     m2 r;
     r.d0 = b ? z0 + p : z0;
-    r.d1 = sub_mod(t1p, v1p + (u64)b, p);
+    r.d1 = sub_mod_c(c.hi, v1p, b, p);
     return r;
 }
 
@@ -484,7 +533,7 @@ WW_HD m2 m2_sqr_add(m2 x, m2 k, u64 p, u64 q, u64 np)
     u64 z0 = sub_b(t.hi, v1, &b);
     m2 r;
     r.d0 = b ? z0 + p : z0;
-    r.d1 = sub_mod(c.hi, v1p + (u64)b, p);
+    r.d1 = sub_mod_c(c.hi, v1p, b, p);
     return r;
 }
 
@@ -498,19 +547,17 @@ WW_HD m2 m2_mul_add(m2 x, m2 y, m2 k, u64 p, u64 q, u64 np)
     u64 v1 = mul_hi(p, u0);
     u128 a = mul_wide_add(x.d0, y.d1, u0);
     u128 e = mul_wide_add(x.d1, y.d0, k.d1);
-    u64 lo, hi, hi2;
-    bool c1 = __builtin_add_overflow(a.lo, e.lo, &lo);
-    bool c2 = __builtin_add_overflow(a.hi, e.hi, &hi);
-    bool c3 = __builtin_add_overflow(hi, (u64)c1, &hi2);
-    bool big = c2 | c3 | (hi2 >= p);
-    u64 t1p = hi2 + (big ? np : 0); /* minus p, modulo 2^64 */
-    u64 up0 = lo * q;
+    bool top;
+    u128 sum = add128_c(a, e, &top);
+    bool big = top | (sum.hi >= p);
+    u64 t1p = sum.hi + (big ? np : 0); /* minus p, modulo 2^64 */
+    u64 up0 = sum.lo * q;
     u64 v1p = mul_hi(p, up0);
     bool b;
     u64 z0 = sub_b(t.hi, v1, &b);
     m2 r;
     r.d0 = b ? z0 + p : z0;
-    r.d1 = sub_mod(t1p, v1p + (u64)b, p);
+    r.d1 = sub_mod_c(t1p, v1p, b, p);
     return r;
 }
 
@@ -537,7 +584,7 @@ WW_HD m2 m2_get(m2 x, u64 p, u64 q)
     bool b = v1 != 0;
     m2 r;
     r.d0 = b ? p - v1 : 0;
-    r.d1 = sub_mod(t1p, v1p + (b ? 1 : 0), p);
+    r.d1 = sub_mod_c(t1p, v1p, b, p);
     return r;
 }
 
@@ -612,7 +659,7 @@ WW_HD m2 wieferich_residue(u64 p, u64 q, m2 one)
    square of the operand the bit picks goes to y, so after bit i slot x holds V_k exactly
    when bit i is 1, and the square's operand is y iff bits i and i+1 of n are equal. The
    ladder starts at k = 0, (V_0, V_1) = (2, 3), which leading zero bits leave unchanged,
-   so any top >= the leading bit of n (e.g. common to a wave) gives the same result. */
+   so any top in [leading bit of n, 63] (e.g. common to a wave) gives the same result. */
 WW_HD m2 wallsunsun_residue_w(u64 p, u64 q, m2 one, double pinv, int top)
 {
     u32 pm5 = mod5(p);
@@ -629,9 +676,11 @@ WW_HD m2 wallsunsun_residue_w(u64 p, u64 q, m2 one, double pinv, int top)
     WW_LOOP_LOCAL(np); /* opaque, so that adding it is not turned back into subtracting p */
     m2 x = three;                               /* k = 0: bit top+1 is 0, x holds V_1 */
     m2 y = two;                                 /* and y holds V_0 */
+    u64 sh = same << (63 - top);                /* bit i of same at the sign, i = top */
     for (int i = top; i >= 0; --i) {
         WW_LOOP_LOCAL(p);
-        bool use_y = (same >> i) & 1;
+        bool use_y = (int32_t)(sh >> 32) < 0;
+        sh <<= 1;
         m2 sin;
         sin.d0 = use_y ? y.d0 : x.d0;
         sin.d1 = use_y ? y.d1 : x.d1;
