@@ -676,16 +676,21 @@ WW_HD m2 wallsunsun_residue_w(u64 p, u64 q, m2 one, double pinv, int top)
     WW_LOOP_LOCAL(np); /* opaque, so that adding it is not turned back into subtracting p */
     m2 x = three;                               /* k = 0: bit top+1 is 0, x holds V_1 */
     m2 y = two;                                 /* and y holds V_0 */
-    u64 sh = same << (63 - top);                /* bit i of same at the sign, i = top */
-    for (int i = top; i >= 0; --i) {
-        WW_LOOP_LOCAL(p);
-        bool use_y = (int32_t)(sh >> 32) < 0;
-        sh <<= 1;
-        m2 sin;
-        sin.d0 = use_y ? y.d0 : x.d0;
-        sin.d1 = use_y ? y.d1 : x.d1;
-        x = m2_mul_add(x, y, minus3, p, q, np); /* V_{2k+1} = V_k V_{k+1} - 3 */
-        y = m2_sqr_add(sin, minus2, p, q, np);  /* V_{2k} or V_{2k+2} = V^2 - 2 */
+    /* The bits of `same` are read one 32-bit word at a time, high word first when top
+       >= 32: the word is doubled each step and the carry out is the bit (one
+       v_add_co_u32 on the device gives both). */
+    for (int half = top >= 32 ? 1 : 0; half >= 0; --half) {
+        const int lead = half ? top - 32 : (top >= 32 ? 31 : top); /* first bit, in the word */
+        u32 w = (half ? (u32)(same >> 32) : (u32)same) << (31 - lead);
+        for (int j = lead; j >= 0; --j) {
+            WW_LOOP_LOCAL(p);
+            bool use_y = __builtin_add_overflow(w, w, &w);
+            m2 sin;
+            sin.d0 = use_y ? y.d0 : x.d0;
+            sin.d1 = use_y ? y.d1 : x.d1;
+            x = m2_mul_add(x, y, minus3, p, q, np); /* V_{2k+1} = V_k V_{k+1} - 3 */
+            y = m2_sqr_add(sin, minus2, p, q, np);  /* V_{2k} or V_{2k+2} = V^2 - 2 */
+        }
     }
     /* slot x holds V_n iff bit 0 of n is 1 */
     bool odd = n & 1;

@@ -63,6 +63,8 @@ static uint32_t powmod32(uint32_t b, uint32_t e, uint32_t m)
     return (uint32_t)r;
 }
 
+int ww_set_error(int code, const char *fmt, ...); /* ww_api.cpp */
+
 extern "C" {
 
 int ww_cpu_is_prime(uint64_t n)
@@ -79,6 +81,46 @@ int ww_cpu_is_prp(uint64_t n)
 {
     if ((n & 1) == 0) return 0;
     return no_factor_to_113(n) && strong_prp(2, n);
+}
+
+int ww_cpu_residues(const uint64_t *p, uint64_t n, int32_t top_w, int32_t top_f, uint64_t *out)
+{
+    if (n && (!p || !out)) return ww_set_error(WW_E_ARG, "null buffer");
+    for (uint64_t i = 0; i < n; ++i) {
+        const u64 pi = p[i];
+        if (pi < 127 || (pi & 1) == 0 || mod5(pi) == 0)
+            return ww_set_error(WW_E_ARG, "p[%llu] = %llu is not odd, >= 127 and prime to 5",
+                                (unsigned long long)i, (unsigned long long)pi);
+        const int lw = 63 - clz64(pi >> 1), lf = wallsunsun_top(pi);
+        const int tw = top_w < 0 ? lw : top_w, tf = top_f < 0 ? lf : top_f;
+        if (tw < lw || tw > 63 || tf < lf || tf > 63)
+            return ww_set_error(WW_E_ARG, "ladder bounds %d, %d outside [%d, 63], [%d, 63]", tw, tf,
+                                lw, lf);
+        const u64 q = inv64(pi);
+        const m2 one = m2_one(pi);
+        const double pinv = 1.0 / (double)pi;
+        const m2 s = wieferich_residue_w(pi, q, one, pinv, tw);
+        const m2 f = wallsunsun_residue_w(pi, q, one, pinv, tf);
+        out[4 * i] = s.d0;
+        out[4 * i + 1] = s.d1;
+        out[4 * i + 2] = f.d0;
+        out[4 * i + 3] = f.d1;
+    }
+    return WW_OK;
+}
+
+int ww_cpu_spsp2(uint64_t n, int32_t top, int32_t tmax)
+{
+    if (n < 3 || (n & 1) == 0) return ww_set_error(WW_E_ARG, "n = %llu is not odd and > 1",
+                                                   (unsigned long long)n);
+    const u64 e = n - 1;
+    const int lead = 63 - clz64(e), t = ctz64(e);
+    if (top < 0) top = lead;
+    if (tmax < 0) tmax = t;
+    if (top < lead || top > 63 || tmax < t || tmax > top)
+        return ww_set_error(WW_E_ARG, "bounds top %d, tmax %d outside [%d, 63], [%d, top]", top,
+                            tmax, lead, t);
+    return spsp2_window(n, inv64(n), mont_one(n), top, tmax) ? 1 : 0;
 }
 
 int ww_cpu_range(uint64_t low, uint64_t high, uint64_t out[3])

@@ -162,6 +162,9 @@ _SIGNATURES = [
     ("ww_copy_probe", ctypes.c_int, [ctypes.c_int, U64, ctypes.POINTER(ctypes.c_double)]),
     ("ww_cpu_is_prime", ctypes.c_int, [U64]),
     ("ww_cpu_is_prp", ctypes.c_int, [U64]),
+    ("ww_cpu_residues", ctypes.c_int, [ctypes.POINTER(U64), U64, ctypes.c_int32,
+                                       ctypes.c_int32, ctypes.POINTER(U64)]),
+    ("ww_cpu_spsp2", ctypes.c_int, [U64, ctypes.c_int32, ctypes.c_int32]),
     ("ww_cpu_range", ctypes.c_int, [U64, U64, ctypes.POINTER(U64)]),
     ("ww_deep_table", U32, [U32, ctypes.POINTER(DeepPrime), U32]),
     ("ww_presieve_table", U32, [U32, U32, ctypes.POINTER(WidePrime), U32]),
@@ -249,6 +252,24 @@ def presieve_table(above: int, bound: int):
     buf = (WidePrime * cap)()
     n = load().ww_presieve_table(above, bound, buf, cap)
     return [(buf[i].ell, buf[i].m, buf[i].bar_ell, buf[i].bar_m) for i in range(n)]
+
+
+def cpu_residues(ps, top_w: int = -1, top_f: int = -1):
+    """Host build of the device ladders: [(s0, s1, c20, c21)] per p, from the given
+    leading-bit bounds (-1 = each p's own)."""
+    n = len(ps)
+    pin = (U64 * max(n, 1))(*ps)
+    out = (U64 * (4 * max(n, 1)))()
+    _check(load().ww_cpu_residues(pin, n, top_w, top_f, out))
+    return [tuple(out[4 * i:4 * i + 4]) for i in range(n)]
+
+
+def cpu_spsp2(n: int, top: int = -1, tmax: int = -1) -> bool:
+    """Host build of the device's windowed strong PRP test with the given bounds."""
+    r = load().ww_cpu_spsp2(n, top, tmax)
+    if r < 0:
+        _check(r)
+    return bool(r)
 
 
 def cpu_range(low: int, high: int):
