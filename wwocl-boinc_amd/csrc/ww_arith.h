@@ -304,47 +304,46 @@ WW_HD u64 mont_mul(u64 a, u64 b, u64 n, u64 q)
     return t.hi < mp ? r + n : r;
 }
 
-/* y * 2^v mod n for y < n and v <= 31. The quotient floor(y 2^v / n) < 2^31 is estimated
-   in double precision from ninv ~ 1/n: the estimate's relative error is below 2^-51, so
-   its absolute error is below 2^-20 and the estimate is the quotient or one off; the
-   remainder x - qe*n then lies in [-n, 2n) and one signed correction makes it exact. */
-WW_HD u64 mulpow2_mod(u64 y, u32 v, u64 n, double ninv)
+/* A 64-bit integer as the nearest double: both 32-bit halves convert exactly and one FMA
+   rounds their sum once. */
+WW_HD double u64_to_double(u64 y)
 {
-    u64 lo = y << v;
-    u32 hi = (u32)((y >> 1) >> (63 - v)); /* y 2^v = hi:lo, hi < 2^v */
-    u32 qe = (u32)ldexp((double)y * ninv, (int)v);
-    u64 m0 = (u64)qe * (u32)n;
-    u64 m1 = mad32(qe, (u32)(n >> 32), m0 >> 32);
-    u64 qn_lo = (m1 << 32) | (u32)m0;
-    u32 qn_hi = (u32)(m1 >> 32);
-    bool b;
-    u64 r = sub_b(lo, qn_lo, &b);
-    int32_t rh = (int32_t)(hi - qn_hi - (u32)b); /* -1, 0 or 1 */
-    bool big = (rh > 0) | (r >= n);
-    u64 adj = rh < 0 ? n : (big ? 0 - n : 0);
-    return r + adj;
+    return fma((double)(u32)(y >> 32), 4294967296.0, (double)(u32)y);
 }
 
 /* y * 2^v + c = quot * n + r for y < n, v <= 31 and c < 2^v; returns r and the quotient
-   (below 2^v). The estimate is that of mulpow2_mod with c folded into one rounding. */
+   (below 2^v). With x = y 2^v + c < n 2^31, x and 1/n in double precision carry a
+   relative error below 2^-50.6 together (three roundings of x, two of 1/n, the FMA's
+   own), an absolute error below 2^-19.6 at quotients below 2^31, so
+   floor(x * (1/n) + 2^-18) lies in (x/n, x/n + 1): it is the quotient or one more. The
+   remainder x - qe n then lies in [-n, n), and when the 96-bit subtraction borrows, one
+   addition of n and a decrement of the quotient finish it. */
 WW_HD u64 mulpow2_add_qr(u64 y, u32 v, u32 c, u64 n, double ninv, u32 *quot)
 {
-    u64 lo;
-    bool cy = __builtin_add_overflow(y << v, (u64)c, &lo);
-    u32 hi = (u32)((y >> 1) >> (63 - v)) + (u32)cy;
-    u32 qe = (u32)((ldexp((double)y, (int)v) + (double)c) * ninv);
+    u32 qe = (u32)fma(ldexp(u64_to_double(y), (int)v) + (double)c, ninv, 0x1p-18);
+    /* x = hi:lo */
+    u64 sh = y << v;
+    bool cy = false;
+    u32 lo0 = addc32((u32)sh, c, &cy);
+    u32 lo1 = addc32((u32)(sh >> 32), 0u, &cy);
+    u32 hi = addc32(((u32)(y >> 32) >> 1) >> (31 - v), 0u, &cy); /* y >> (64 - v) */
+    /* qe n = m1:m0 (96 bits) */
     u64 m0 = (u64)qe * (u32)n;
     u64 m1 = mad32(qe, (u32)(n >> 32), m0 >> 32);
-    u64 qn_lo = (m1 << 32) | (u32)m0;
-    u32 qn_hi = (u32)(m1 >> 32);
-    bool b;
-    u64 r = sub_b(lo, qn_lo, &b);
-    int32_t rh = (int32_t)(hi - qn_hi - (u32)b); /* -1, 0 or 1 */
-    bool neg = rh < 0;
-    bool big = !neg & ((rh > 0) | (r >= n)); /* a negative r also reads as >= n */
-    *quot = qe + (u32)big - (u32)neg;
-    u64 adj = neg ? n : (big ? 0 - n : 0);
-    return r + adj;
+    bool b = false;
+    u32 r0 = subb32(lo0, (u32)m0, &b);
+    u32 r1 = subb32(lo1, (u32)m1, &b);
+    (void)subb32(hi, (u32)(m1 >> 32), &b); /* b: x < qe n */
+    u64 r = ((u64)r1 << 32) | r0;
+    *quot = qe - (u32)b;
+    return b ? r + n : r;
+}
+
+/* y * 2^v mod n for y < n and v <= 31. */
+WW_HD u64 mulpow2_mod(u64 y, u32 v, u64 n, double ninv)
+{
+    u32 quot;
+    return mulpow2_add_qr(y, v, 0, n, ninv, &quot);
 }
 
 /* Base-2 strong probable-prime test of odd n > 1 (getsegprimes.cl:118-164,
