@@ -79,12 +79,12 @@ __global__ __launch_bounds__(kPreThreads) void k_presieve(const BatchSpan *spans
                 const u64 r = barrett64(G, ell, d.bar_ell);
                 u64 dd = r ? ell - r : 0;
                 if (dd & 1) dd += ell;
-                u64 i = dd >> 1;
-                if (G <= ell && G + 2 * i == ell) i += ell; /* never ell itself */
-                for (; i < len; i += ell) {
-                    const u32 at = off + (u32)i;
+                u32 i = (u32)(dd >> 1); /* < ell < 2^22 */
+                if (G <= ell && G + 2 * (u64)i == ell) i += (u32)ell; /* never ell itself */
+                /* positions in the chunk, below 2^20 + 2^23: 32-bit arithmetic */
+                const u32 end = off + len;
+                for (u32 at = off + i; at < end; at += (u32)ell)
                     atomicOr(&bm[at >> 5], 1u << (at & 31));
-                }
                 /* members of the class ell mod m: G + 2k = ell (mod m), both odd */
                 const u64 rm = barrett64(G, d.m, d.bar_m);
                 for (u64 k = (ell >= rm ? ell - rm : ell + d.m - rm) >> 1; k < len; k += d.m >> 1) {
