@@ -104,7 +104,12 @@ __device__ __forceinline__ void tests_round(u32 n, u32 &qn, u64 *wq, u32 *ws, u3
     }
 }
 
-__global__ __launch_bounds__(64 * kBatchWaves) __attribute__((amdgpu_waves_per_eu(6))) void k_batch(const BatchSpan *spans,
+#ifndef WW_BATCH_WAVES_PER_EU
+#define WW_BATCH_WAVES_PER_EU 5 /* 96 VGPRs: five 4-wave workgroups per CU */
+#endif
+
+__global__ __launch_bounds__(64 * kBatchWaves)
+__attribute__((amdgpu_waves_per_eu(WW_BATCH_WAVES_PER_EU))) void k_batch(const BatchSpan *spans,
                                                              const u64 *prefix, u32 nspans,
                                                              BatchState *st, SlotSum *sums,
                                                              ww_result *res, u32 res_cap,
