@@ -218,7 +218,8 @@ def main():
                     help="length of each serving run (0 = skip serving)")
     ap.add_argument("--serve-rates", default="20,50,100", help="request rates (per second)")
     ap.add_argument("--no-tune", action="store_true",
-                    help="skip the autotuner (library defaults: deep bound 2048, full occupancy)")
+                    help="skip the autotuner (library defaults: deep bound 2048, pre-sieve "
+                         "bound 2^20, full occupancy)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="also launch one plain copy kernel of BYTES (counter calibration)")
     args = ap.parse_args()
@@ -240,7 +241,11 @@ def main():
                       "seconds": time.perf_counter() - t0,
                       "by_deep_bound": {str(b): max(t["prps_per_s"] for t in trials
                                                     if t["deep_bound"] == b)
-                                        for b in sorted({t["deep_bound"] for t in trials})}}
+                                        for b in sorted({t["deep_bound"] for t in trials})},
+                      "by_presieve_bound": {
+                          str(b): max(t["prps_per_s"] for t in trials
+                                      if t["presieve_bound"] == b)
+                          for b in sorted({t["presieve_bound"] for t in trials})}}
         if args.form in ("both", "persistent"):
             pers = run_persistent(ctx, args.steps, args.warmup)
         if args.form in ("both", "plain"):

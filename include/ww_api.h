@@ -282,23 +282,26 @@ int ww_batch_destroy(ww_batch *b);
 
 /* One configuration of the persistent form and its measured speed. */
 typedef struct {
-    uint32_t deep_bound; /* ww_set_deep_bound */
-    int32_t blocks_per_cu; /* ww_batch_set_grid; 0 = as many as fit */
-    uint32_t claim;        /* largest segments per claim */
-    uint32_t pad;
-    double ms;             /* kernel time of the sample */
-    double prps_per_s;     /* tested primes per second on the sample */
+    uint32_t deep_bound;     /* ww_set_deep_bound */
+    int32_t blocks_per_cu;   /* ww_batch_set_grid; 0 = as many as fit */
+    uint32_t claim;          /* largest segments per claim */
+    uint32_t presieve_bound; /* ww_set_presieve_bound */
+    double ms;               /* kernel time of the sample (pre-sieve + search) */
+    double prps_per_s;       /* tested primes per second on the sample */
 } ww_tuning;
 
 /* Autotuner replacing the reference's work-size profiling (profileGPU,
    main.cpp:959-1107): times the persistent form on the sample range [lo, hi) for every
-   combination of the given deep-sieve bounds, workgroups per CU and claim sizes (null
-   lists = built-in defaults), keeps the fastest as this context's deep bound and as the
-   default geometry of batches created afterwards, and returns it in *best. `trials`
-   (may be null) receives up to cap measured configurations; *ntrials their count. */
+   combination of the given deep-sieve bounds, workgroups per CU and claim sizes at the
+   context's pre-sieve bound, then each given pre-sieve bound at the best of those (null
+   lists = built-in defaults). Keeps the fastest as this context's deep and pre-sieve
+   bounds and as the default geometry of batches created afterwards, and returns it in
+   *best. `trials` (may be null) receives up to cap measured configurations; *ntrials
+   their count. Results do not depend on any of these settings. */
 int ww_autotune(ww_ctx *ctx, uint64_t lo, uint64_t hi, const uint32_t *bounds, uint32_t nb,
                 const int32_t *blocks, uint32_t nbl, const uint32_t *claims, uint32_t nc,
-                ww_tuning *best, ww_tuning *trials, uint32_t cap, uint32_t *ntrials);
+                const uint32_t *presieve, uint32_t np, ww_tuning *best, ww_tuning *trials,
+                uint32_t cap, uint32_t *ntrials);
 
 /* ---------------------------------------------------------------------------
  * Serving form: continuous batching of range requests. No counterpart in the

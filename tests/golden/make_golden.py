@@ -113,7 +113,7 @@ def main():
 
     # the full parity configurations: [127, 10^8) and [2^50, 2^50 + 2^32)
     full = []
-    for name, lo, hi in (("cfg1", 127, 10**8), ("cfg2", 1 << 50, (1 << 50) + (1 << 32))):
+    for name, lo, hi in (("127_to_1e8", 127, 10**8), ("2p50_w2p32", 1 << 50, (1 << 50) + (1 << 32))):
         prp, cks, flags = oracle.ref_range(lo, hi, os.cpu_count() or 1)
         full.append({"name": name, "lo": lo, "hi": hi, "prp_count": prp, "checksum": cks,
                      "cpu_result_count": flags})
@@ -124,7 +124,7 @@ def main():
         "about": "closed forms (Python big ints) and reference CPU path (oracle/_ref)",
         "primes": primes,
         "ranges": ranges,
-        "full_configs": full,
+        "full_ranges": full,
         "spsp2_composites_below_200001": spsp_composites,
         "pi": {"1e6": 78498, "1e7": 664579, "1e8": 5761455},
     }

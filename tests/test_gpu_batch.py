@@ -39,7 +39,7 @@ def test_batch_ragged_mix_vs_oracle(gctx):
 
 
 def test_batch_workload_equals_plain_per_range(gctx):
-    """4096 ranges (seed 0). Per-range checksums and counts bit-identical to
+    """The 4096-range workload (seed 0). Per-range checksums and counts bit-identical to
     plain per-range launches; 64 of the narrow ones also against the oracle."""
     ranges = wwlib.batch_workload()
     with wwlib.Batch(gctx, ranges) as b:
@@ -110,16 +110,22 @@ def test_autotune_keeps_results(gctx):
     lo, hi = (1 << 62) + 12345, (1 << 62) + 12345 + (1 << 24)
     plain = gctx.run_range(lo, hi)
     try:
-        best, trials = gctx.autotune(lo, hi, bounds=[0, 4096], blocks=[0, 4], claims=[1, 16])
-        assert len(trials) == 8
+        pre0 = gctx.presieve_bound()
+        best, trials = gctx.autotune(lo, hi, bounds=[0, 4096], blocks=[0, 4], claims=[1, 16],
+                                     presieve=[0, pre0, 1 << 19])
+        assert len(trials) == 8 + (2 if pre0 not in (0, 1 << 19) else 1)
+        assert all(t["presieve_bound"] == pre0 for t in trials[:8])
+        assert {t["presieve_bound"] for t in trials[8:]} == {0, 1 << 19} - {pre0}
         assert best["ms"] == min(t["ms"] for t in trials)
         assert gctx.deep_bound() == best["deep_bound"]
+        assert gctx.presieve_bound() == best["presieve_bound"]
         with wwlib.Batch(gctx, [(lo, hi)]) as b:
             b.launch()
             (prp, cks, recs), = b.read()
         assert (prp, cks, recs) == (plain.prp_count, plain.checksum, plain.results)
     finally:
         gctx.set_deep_bound(wwlib.WW_DEFAULT_DEEP_BOUND)
+        gctx.set_presieve_bound(wwlib.WW_DEFAULT_PRESIEVE_BOUND)
 
 
 def test_batch_workload_sample_vs_reference_code(gctx):

@@ -72,11 +72,11 @@ RANGES = [
     (127, 128), (127, 131), (128, 129), (1000, 1001), (1001, 1003),
     (2_000_000, 2_000_061), (5_000_001, 5_050_000),
     ((1 << 32) - (1 << 20), (1 << 32) + (1 << 20)),  # two-branch m2p_one boundary
-    (1 << 50, (1 << 50) + (1 << 22)),          # cfg2 slice
+    (1 << 50, (1 << 50) + (1 << 22)),          # 2^50 slice
     (T3_P - 1000, T3_P + 1000),                # Wieferich special instance (-1 -3 p)
     (T4_P, T4_P + 5000),                       # WSS special instance at START (inclusive)
     ((1 << 62) - 12345, (1 << 62) + 54321),
-    (1 << 63, (1 << 63) + (1 << 22)),          # cfg3 slice
+    (1 << 63, (1 << 63) + (1 << 22)),          # 2^63 slice
     ((1 << 64) - (1 << 22), (1 << 64) - 1),    # top of the domain
     ((1 << 64) - 100, (1 << 64) - 1),
 ]
@@ -152,7 +152,7 @@ def test_known_answers(gctx, name, lo, hi, lines, cks):
         assert primes_only == lines
 
 
-def test_cfg3_full_size_properties(gctx):
+def test_2p33_range_at_2p63_properties(gctx):
     """[2^63, 2^63 + 2^33): checksum additivity over halves, and 16 sampled 2^22-wide
     slices bit-exact against the oracle."""
     lo, hi = 1 << 63, (1 << 63) + (1 << 33)
@@ -162,7 +162,7 @@ def test_cfg3_full_size_properties(gctx):
     b = gctx.run_range(mid, hi, 1 << 32)
     assert whole.prp_count == a.prp_count + b.prp_count
     assert whole.checksum == (a.checksum + b.checksum) % (1 << 64)
-    print(f"cfg3: {whole.prp_count} PRPs, checksum {whole.checksum:016X}")
+    print(f"[2^63, 2^63+2^33): {whole.prp_count} PRPs, checksum {whole.checksum:016X}")
     for k in range(16):
         s = lo + k * (1 << 29)
         got = gctx.run_range(s, s + (1 << 22))
@@ -170,18 +170,18 @@ def test_cfg3_full_size_properties(gctx):
         assert (got.prp_count, got.checksum) == (prp, cks), k
 
 
-@pytest.mark.parametrize("cfg", GOLD["full_configs"], ids=lambda c: c["name"])
-def test_full_configuration_vs_reference_code(gctx, cfg):
+@pytest.mark.parametrize("whole", GOLD["full_ranges"], ids=lambda c: c["name"])
+def test_whole_range_vs_reference_code(gctx, whole):
     """[127, 10^8) and [2^50, 2^50 + 2^32) whole, plain and persistent forms, against the
     reference's own CPU path (compiled from wwcpu.cpp / prime.h, tests/golden)."""
-    lo, hi = cfg["lo"], cfg["hi"]
+    lo, hi = whole["lo"], whole["hi"]
     got = gctx.run_range(lo, hi, 1 << 32)
-    print(f"{cfg['name']}: {got.prp_count} PRPs, checksum {got.checksum:016X}, "
+    print(f"{whole['name']}: {got.prp_count} PRPs, checksum {got.checksum:016X}, "
           f"{len(got.results)} records")
-    assert (got.prp_count, got.checksum) == (cfg["prp_count"], cfg["checksum"])
-    assert len(got.results) == cfg["cpu_result_count"]
+    assert (got.prp_count, got.checksum) == (whole["prp_count"], whole["checksum"])
+    assert len(got.results) == whole["cpu_result_count"]
     with wwlib.Batch(gctx, [(lo, hi)]) as b:
         b.launch(graph=True)
         (prp, cks, recs), = b.read()
-    assert (prp, cks) == (cfg["prp_count"], cfg["checksum"])
+    assert (prp, cks) == (whole["prp_count"], whole["checksum"])
     assert recs == got.results

@@ -156,7 +156,7 @@ def test_sieve_prime_counts():
         assert got == want
 
 
-def test_cfg1_prime_count():
+def test_prime_count_127_to_1e8():
     """[127, 1e8): the list holds pi(1e8) - pi(126) = 5761425 primes plus the base-2
     strong pseudoprimes with no factor <= 113."""
     lst = oracle.prp_list(127, 10**8)

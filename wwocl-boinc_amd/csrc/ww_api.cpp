@@ -846,13 +846,15 @@ int ww_batch_destroy(ww_batch *b)
 
 int ww_autotune(ww_ctx *c, uint64_t lo, uint64_t hi, const uint32_t *bounds, uint32_t nb,
                 const int32_t *blocks, uint32_t nbl, const uint32_t *claims, uint32_t nc,
-                ww_tuning *best, ww_tuning *trials, uint32_t cap, uint32_t *ntrials)
+                const uint32_t *presieve, uint32_t np, ww_tuning *best, ww_tuning *trials,
+                uint32_t cap, uint32_t *ntrials)
 {
     int rc = use(c);
     if (rc) return rc;
     static const uint32_t kBounds[] = {0, 1024, 2048, 4096, 8192, 16384, 65535};
     static const int32_t kBlocks[] = {0, 4, 5};
     static const uint32_t kClaims[] = {1, 4, 16, 64};
+    static const uint32_t kPresieve[] = {0, 1u << 18, 1u << 19, 1u << 20, 1u << 21};
     if (!bounds || !nb) {
         bounds = kBounds;
         nb = sizeof kBounds / sizeof *kBounds;
@@ -865,47 +867,76 @@ int ww_autotune(ww_ctx *c, uint64_t lo, uint64_t hi, const uint32_t *bounds, uin
         claims = kClaims;
         nc = sizeof kClaims / sizeof *kClaims;
     }
+    if (!presieve || !np) {
+        presieve = kPresieve;
+        np = sizeof kPresieve / sizeof *kPresieve;
+    }
+    for (uint32_t i = 0; i < np; ++i)
+        if (presieve[i] > WW_MAX_PRESIEVE_BOUND)
+            return fail(WW_E_ARG, "pre-sieve bound %u above %u", presieve[i], WW_MAX_PRESIEVE_BOUND);
+    const uint32_t saved_bound = c->deep_bound, saved_pre = c->presieve_bound;
+    /* the sample batch holds pre-sieve words only if the pre-sieve is on when it is made */
+    if (!saved_pre && (rc = ww_set_presieve_bound(c, WW_DEFAULT_PRESIEVE_BOUND))) return rc;
     ww_range r = {lo, hi};
     ww_batch *b = nullptr;
-    if ((rc = ww_batch_create(c, &r, 1, 1u << 16, &b))) return rc;
-    const uint32_t saved_bound = c->deep_bound;
+    if ((rc = ww_batch_create(c, &r, 1, 1u << 16, &b))) {
+        ww_set_presieve_bound(c, saved_pre);
+        return rc;
+    }
+    if ((rc = ww_set_presieve_bound(c, saved_pre))) {
+        ww_batch_destroy(b);
+        return rc;
+    }
     ww_tuning top;
     memset(&top, 0, sizeof top);
     top.ms = 1e300;
     uint32_t n = 0;
+    /* one timed configuration (the first launch warms) */
+    auto trial = [&](uint32_t deep, int32_t bpc, uint32_t claim) -> int {
+        int e;
+        if ((e = ww_batch_set_grid(b, bpc, claim))) return e;
+        double ms = 0, t = 0;
+        ww_range_sum sum;
+        for (int rep = 0; rep < 2; ++rep) {
+            if ((e = ww_batch_launch(c, b))) return e;
+            if ((e = ww_batch_time(c, b, &t))) return e;
+            ms = t;
+        }
+        if ((e = ww_batch_read(c, b, &sum, nullptr, 0, nullptr, nullptr))) return e;
+        ww_tuning tr;
+        tr.deep_bound = deep;
+        tr.blocks_per_cu = bpc;
+        tr.claim = claim;
+        tr.presieve_bound = c->presieve_bound;
+        tr.ms = ms;
+        tr.prps_per_s = ms > 0 ? sum.prp_count / (ms * 1e-3) : 0;
+        if (trials && n < cap) trials[n] = tr;
+        ++n;
+        if (ms < top.ms) top = tr;
+        return WW_OK;
+    };
+    /* deep bound x geometry at the context's pre-sieve bound, then the pre-sieve bound at
+       the best of those */
     for (uint32_t i = 0; i < nb && rc == WW_OK; ++i) {
         if ((rc = ww_set_deep_bound(c, bounds[i]))) break;
-        for (uint32_t j = 0; j < nbl && rc == WW_OK; ++j) {
-            for (uint32_t k = 0; k < nc && rc == WW_OK; ++k) {
-                if ((rc = ww_batch_set_grid(b, blocks[j], claims[k]))) break;
-                double ms = 0, t = 0;
-                ww_range_sum sum;
-                for (int rep = 0; rep < 2 && rc == WW_OK; ++rep) { /* first run warms */
-                    if ((rc = ww_batch_launch(c, b))) break;
-                    if ((rc = ww_batch_time(c, b, &t))) break;
-                    ms = t;
-                }
-                if (rc) break;
-                if ((rc = ww_batch_read(c, b, &sum, nullptr, 0, nullptr, nullptr))) break;
-                ww_tuning tr;
-                tr.deep_bound = bounds[i];
-                tr.blocks_per_cu = blocks[j];
-                tr.claim = claims[k];
-                tr.pad = 0;
-                tr.ms = ms;
-                tr.prps_per_s = ms > 0 ? sum.prp_count / (ms * 1e-3) : 0;
-                if (trials && n < cap) trials[n] = tr;
-                ++n;
-                if (ms < top.ms) top = tr;
-            }
-        }
+        for (uint32_t j = 0; j < nbl && rc == WW_OK; ++j)
+            for (uint32_t k = 0; k < nc && rc == WW_OK; ++k) rc = trial(bounds[i], blocks[j], claims[k]);
+    }
+    if (rc == WW_OK) rc = ww_set_deep_bound(c, top.deep_bound);
+    const ww_tuning stage1 = top;
+    for (uint32_t i = 0; i < np && rc == WW_OK; ++i) {
+        if (presieve[i] == stage1.presieve_bound) continue; /* measured already */
+        if ((rc = ww_set_presieve_bound(c, presieve[i]))) break;
+        rc = trial(stage1.deep_bound, stage1.blocks_per_cu, stage1.claim);
     }
     ww_batch_destroy(b);
     if (rc) {
         ww_set_deep_bound(c, saved_bound);
+        ww_set_presieve_bound(c, saved_pre);
         return rc;
     }
     ww_set_deep_bound(c, top.deep_bound);
+    ww_set_presieve_bound(c, top.presieve_bound);
     c->tuned_blocks = top.blocks_per_cu;
     c->tuned_claim = top.claim;
     if (best) *best = top;

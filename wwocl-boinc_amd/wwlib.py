@@ -65,11 +65,13 @@ class WidePrime(ctypes.Structure):
 
 class Tuning(ctypes.Structure):
     _fields_ = [("deep_bound", U32), ("blocks_per_cu", ctypes.c_int32), ("claim", U32),
-                ("pad", U32), ("ms", ctypes.c_double), ("prps_per_s", ctypes.c_double)]
+                ("presieve_bound", U32), ("ms", ctypes.c_double),
+                ("prps_per_s", ctypes.c_double)]
 
     def as_dict(self):
         return {"deep_bound": self.deep_bound, "blocks_per_cu": self.blocks_per_cu,
-                "claim": self.claim, "ms": self.ms, "prps_per_s": self.prps_per_s}
+                "claim": self.claim, "presieve_bound": self.presieve_bound, "ms": self.ms,
+                "prps_per_s": self.prps_per_s}
 
 
 class ServerConfig(ctypes.Structure):
@@ -146,8 +148,8 @@ _SIGNATURES = [
     ("ww_batch_destroy", ctypes.c_int, [ctypes.c_void_p]),
     ("ww_autotune", ctypes.c_int, [ctypes.c_void_p, U64, U64, ctypes.POINTER(U32), U32,
                                    ctypes.POINTER(ctypes.c_int32), U32, ctypes.POINTER(U32), U32,
-                                   ctypes.POINTER(Tuning), ctypes.POINTER(Tuning), U32,
-                                   ctypes.POINTER(U32)]),
+                                   ctypes.POINTER(U32), U32, ctypes.POINTER(Tuning),
+                                   ctypes.POINTER(Tuning), U32, ctypes.POINTER(U32)]),
     ("ww_server_create", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ServerConfig),
                                         ctypes.POINTER(ctypes.c_void_p)]),
     ("ww_server_submit", ctypes.c_int, [ctypes.c_void_p, U64, U64, ctypes.POINTER(U64)]),
@@ -335,19 +337,20 @@ class Context:
         _check(self.lib.ww_get_presieve_bound(self.h, ctypes.byref(v)))
         return v.value
 
-    def autotune(self, lo: int, hi: int, bounds=None, blocks=None, claims=None):
+    def autotune(self, lo: int, hi: int, bounds=None, blocks=None, claims=None, presieve=None):
         """Tune the persistent form on [lo, hi); returns (best, [trials]) as dicts."""
         def arr(t, v):
             return ((t * len(v))(*v), len(v)) if v else (None, 0)
         b, nb = arr(U32, bounds)
         bl, nbl = arr(ctypes.c_int32, blocks)
         c, nc = arr(U32, claims)
+        ps, nps = arr(U32, presieve)
         best = Tuning()
         cap = 256
         trials = (Tuning * cap)()
         n = U32()
-        _check(self.lib.ww_autotune(self.h, lo, hi, b, nb, bl, nbl, c, nc, ctypes.byref(best),
-                                    trials, cap, ctypes.byref(n)))
+        _check(self.lib.ww_autotune(self.h, lo, hi, b, nb, bl, nbl, c, nc, ps, nps,
+                                    ctypes.byref(best), trials, cap, ctypes.byref(n)))
         return best.as_dict(), [trials[i].as_dict() for i in range(min(n.value, cap))]
 
     # reference kernel-level entry points
