@@ -87,3 +87,26 @@ def test_batch_with_wide_presieve(deep_ctx, deep, pre):
         assert b.read() == got
     for (lo, hi), g in zip(ranges, got):
         assert g == oracle.range_results(lo, hi), (deep, pre, lo, hi)
+
+
+def test_graphs_and_servers_survive_a_presieve_table_rebuild():
+    """Raising the pre-sieve bound past what the context has built replaces its device
+    table. A batch's captured graph is captured again, and a running server keeps its own
+    copy of the tables, so both still give the oracle's results afterwards."""
+    ranges = [(3215031751 - (1 << 21), 3215031751 + (1 << 21)),
+              ((1 << 63) + 12345, (1 << 63) + 12345 + (1 << 22))]
+    want = [oracle.range_results(lo, hi) for lo, hi in ranges]
+    with wwlib.Context(0, list_capacity=1 << 20) as ctx:
+        assert ctx.presieve_bound() == wwlib.WW_DEFAULT_PRESIEVE_BOUND < wwlib.WW_MAX_PRESIEVE_BOUND
+        with wwlib.Batch(ctx, ranges) as b, wwlib.Server(ctx, tick_segments=1024) as srv:
+            b.launch(graph=True)
+            assert b.read() == want
+            ctx.set_presieve_bound(wwlib.WW_MAX_PRESIEVE_BOUND)  # builds a larger table
+            ctx.set_presieve_bound(wwlib.WW_DEFAULT_PRESIEVE_BOUND)
+            b.launch(graph=True)
+            assert b.read() == want
+            done, ids, _ = wwlib.serve(srv, [(0.0, lo, hi) for lo, hi in ranges],
+                                       realtime=False)
+        for rid, (lo, hi) in ids.items():
+            c = done[rid]
+            assert (c["prp_count"], c["checksum"], c["records"]) == want[ranges.index((lo, hi))]

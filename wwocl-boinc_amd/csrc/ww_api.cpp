@@ -591,6 +591,7 @@ struct ww_batch {
     uint32_t graph_claim = 0;
     uint32_t graph_ndeep = 0;
     uint64_t graph_wide = 0;
+    const void *graph_wide_table = nullptr; /* the table the graph reads (rebuilds move it) */
     hipEvent_t ev0 = nullptr, evm = nullptr, ev1 = nullptr;
 };
 
@@ -747,7 +748,8 @@ int ww_batch_launch_graph(ww_ctx *c, ww_batch *b)
     const uint64_t wide = ((uint64_t)wfirst << 32) | wcount;
     const bool pre = batch_presieves(c, b);
     if (b->exec && (b->graph_grid != grid || b->graph_claim != b->claim ||
-                    b->graph_ndeep != c->ndeep || b->graph_wide != wide))
+                    b->graph_ndeep != c->ndeep || b->graph_wide != wide ||
+                    b->graph_wide_table != c->d_wide))
         drop_graphs(b);
     if (!b->exec) {
         if (pre && (rc = capture(c, &b->graph_pre, &b->exec_pre,
@@ -764,6 +766,7 @@ int ww_batch_launch_graph(ww_ctx *c, ww_batch *b)
         b->graph_claim = b->claim;
         b->graph_ndeep = c->ndeep;
         b->graph_wide = wide;
+        b->graph_wide_table = c->d_wide;
     }
     /* two graphs, so that the events between them time each kernel on its own */
     HIP_TRY(hipEventRecord(b->ev0, c->stream));

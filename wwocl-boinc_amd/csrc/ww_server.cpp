@@ -80,9 +80,11 @@ struct ww_server {
     int grid = 0;
     hipStream_t stream = nullptr;
     SlotSum *d_sums = nullptr;
-    const ww::DeepPrime *d_deep = nullptr;
+    /* the server's own copies of the context's active sieve tables, so that later changes
+       to the context's bounds (which may free its tables) never reach a running engine */
+    ww::DeepPrime *d_deep = nullptr;
     uint32_t ndeep = 0;
-    const ww::WidePrime *d_wide = nullptr;
+    ww::WidePrime *d_wide = nullptr;
     uint32_t nwide = 0;
     int cus = 0;
     uint32_t max_active = 0;
@@ -295,6 +297,8 @@ void destroy(ww_server *s)
     if (s->stream) (void)hipStreamSynchronize(s->stream);
     for (Tick &t : s->ticks) free_tick(t);
     if (s->d_sums) (void)hipFree(s->d_sums);
+    if (s->d_deep) (void)hipFree(s->d_deep);
+    if (s->d_wide) (void)hipFree(s->d_wide);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
 }
@@ -331,8 +335,10 @@ int ww_server_create(ww_ctx *ctx, const ww_server_config *cfg, ww_server **out)
     }
     int cus = 0, tuned_blocks = 0;
     uint32_t tuned_claim = 8;
-    int rc = ww_ctx_internals(ctx, &s->device, &s->d_deep, &s->ndeep, &cus, &tuned_blocks,
-                              &tuned_claim, &s->d_wide, &s->nwide);
+    const ww::DeepPrime *ctx_deep = nullptr;
+    const ww::WidePrime *ctx_wide = nullptr;
+    int rc = ww_ctx_internals(ctx, &s->device, &ctx_deep, &s->ndeep, &cus, &tuned_blocks,
+                              &tuned_claim, &ctx_wide, &s->nwide);
     s->cus = cus;
     if (rc) {
         delete s;
@@ -356,6 +362,12 @@ int ww_server_create(ww_ctx *ctx, const ww_server_config *cfg, ww_server **out)
     for (uint32_t i = s->cfg.slots; i > 0; --i) s->free_slots.push_back(i - 1);
 
     hipError_t e = hipSetDevice(s->device);
+    if (e == hipSuccess && s->ndeep) e = hipMalloc(&s->d_deep, s->ndeep * sizeof(ww::DeepPrime));
+    if (e == hipSuccess && s->ndeep)
+        e = hipMemcpy(s->d_deep, ctx_deep, s->ndeep * sizeof(ww::DeepPrime), hipMemcpyDeviceToDevice);
+    if (e == hipSuccess && s->nwide) e = hipMalloc(&s->d_wide, s->nwide * sizeof(ww::WidePrime));
+    if (e == hipSuccess && s->nwide)
+        e = hipMemcpy(s->d_wide, ctx_wide, s->nwide * sizeof(ww::WidePrime), hipMemcpyDeviceToDevice);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMalloc(&s->d_sums, s->cfg.slots * sizeof(SlotSum));
     if (e == hipSuccess) e = hipMemsetAsync(s->d_sums, 0, s->cfg.slots * sizeof(SlotSum), s->stream);
