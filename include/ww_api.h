@@ -99,7 +99,8 @@ int ww_device_info(int device, ww_device_info_t *info);
    cleared. */
 int ww_open(int device, uint64_t list_capacity, uint32_t result_capacity, ww_ctx **out);
 
-/* Release the context and its device memory (cleanup, main.cpp:69-90). */
+/* Release the context and its device memory (cleanup, main.cpp:69-90). Destroy its
+   batches and servers first. */
 int ww_close(ww_ctx *ctx);
 
 /* Upper bound on the list length for one range [low, high): an explicit
