@@ -154,6 +154,15 @@ int ww_wieferich_wallsunsun(ww_ctx *ctx);
    host does not wait between chunks. */
 int ww_search_range(ww_ctx *ctx, uint64_t low, uint64_t high, uint64_t chunk);
 
+/* One chunk [low, high) (width at most 2^34) through the persistent kernel instead of
+   getsegprimes + wieferich/wallsunsun: the same checksum, tested count and records are
+   added to the context's counters and results (what ww_read_counters and
+   ww_read_results return), but no PRP list is written (its count stays as it is, so
+   ww_read_prps and the tests kernel do not apply to such a chunk). Stream-ordered; its
+   time counts as test-kernel time in ww_kernel_times. No counterpart in the reference:
+   a faster body for its chunk loop (main.cpp:1683-1741) when the list is not needed. */
+int ww_search_fused(ww_ctx *ctx, uint64_t low, uint64_t high);
+
 /* Wait for all queued work (sclFinish / waitOnEvent, simpleCL.c, main.cpp:329-377). */
 int ww_finish(ww_ctx *ctx);
 

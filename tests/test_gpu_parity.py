@@ -185,3 +185,33 @@ def test_whole_range_vs_reference_code(gctx, whole):
         (prp, cks, recs), = b.read()
     assert (prp, cks) == (whole["prp_count"], whole["checksum"])
     assert recs == got.results
+
+
+def test_fused_chunks_vs_oracle(gctx):
+    """ww_search_fused: the persistent kernel as the body of the chunk loop adds the same
+    counters and records to the context as the two-kernel form, over the edge ranges, in
+    uneven chunks, and mixed with two-kernel chunks between clears."""
+    for lo, hi in RANGES:
+        got = gctx.run_range_fused(lo, hi)
+        assert (got.prp_count, got.checksum, got.results) == oracle.range_results(lo, hi), (lo, hi)
+    lo, hi = (1 << 40) + 17, (1 << 40) + 17 + 3_000_000
+    want = oracle.range_results(lo, hi)
+    got = gctx.run_range_fused(lo, hi, chunk=1_000_003)
+    assert (got.prp_count, got.checksum, got.results) == want
+    gctx.clear_results()
+    gctx.clear_prp_count()
+    gctx.search_fused(lo, lo + 1_500_000)
+    gctx.search_range(lo + 1_500_000, hi, 1 << 32)
+    gctx.finish()
+    c = gctx.counters()
+    assert (c.totalprimecount, c.checksum) == want[:2]
+    assert wwlib.sort_results(gctx.results()) == want[2]
+
+
+def test_fused_chunks_at_full_size(gctx):
+    lo, hi = 1 << 63, (1 << 63) + (1 << 33)
+    fused = gctx.run_range_fused(lo, hi)
+    plain = gctx.run_range(lo, hi, 1 << 32)
+    assert (fused.prp_count, fused.checksum, fused.results) == \
+        (plain.prp_count, plain.checksum, plain.results)
+    assert f"{fused.checksum:016X}" == "0BB982E18153E292"

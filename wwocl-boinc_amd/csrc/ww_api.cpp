@@ -51,6 +51,10 @@ struct ww_ctx {
     uint64_t pre_plain_segs = 0;
     ww::BatchSpan *d_span1 = nullptr;
     uint64_t *d_prefix1 = nullptr;
+    /* fused chunks (ww_search_fused): the persistent kernel's state, sum and records */
+    ww::BatchState *d_fstate = nullptr;
+    SlotSum *d_fsum = nullptr;
+    ww_result *d_fres = nullptr;
     int tuned_blocks = 0;      /* defaults for batches created from now on */
     uint32_t tuned_claim = 8;
     uint64_t default_chunk = 0;
@@ -132,6 +136,49 @@ void wide_span(ww_ctx *c, uint32_t *first, uint32_t *count)
     while (e < c->wide_ells.size() && c->wide_ells[e] <= c->presieve_bound) ++e;
     *first = a;
     *count = e - a;
+}
+
+} // namespace
+
+namespace {
+
+/* The one-span table of a single chunk [low, high) on the device, and its wide pre-sieve
+   when the pre-sieve is on and the chunk is long enough: *pre = the words, else null. */
+int chunk_presieve(ww_ctx *c, uint64_t low, uint64_t high, const uint64_t **pre)
+{
+    *pre = nullptr;
+    hipError_t e;
+    if (!c->d_span1) {
+        if ((e = hipMalloc(&c->d_span1, sizeof(ww::BatchSpan))) != hipSuccess ||
+            (e = hipMalloc(&c->d_prefix1, 2 * sizeof(uint64_t))) != hipSuccess)
+            return fail(WW_E_HIP, "chunk span: %s", hipGetErrorString(e));
+    }
+    uint32_t wfirst, wcount;
+    wide_span(c, &wfirst, &wcount);
+    const uint64_t nseg = ww::segments_of(low, high);
+    if (!wcount || nseg < ww::kPresieveMinSegs) {
+        if ((e = ww::launch_set_span(low, high, c->d_span1, c->d_prefix1, c->stream)) != hipSuccess)
+            return fail(WW_E_HIP, "chunk span: %s", hipGetErrorString(e));
+        return WW_OK;
+    }
+    if (nseg > c->pre_plain_segs) {
+        /* grow the chunk's pre-sieve buffer; queued launches may still read the old one */
+        if ((e = hipStreamSynchronize(c->stream)) != hipSuccess)
+            return fail(WW_E_HIP, "pre-sieve buffer: %s", hipGetErrorString(e));
+        if (c->d_pre_plain) (void)hipFree(c->d_pre_plain);
+        c->d_pre_plain = nullptr;
+        c->pre_plain_segs = 0;
+        if ((e = hipMalloc(&c->d_pre_plain, nseg * ww::kPresieveWordsPerSeg * sizeof(uint64_t))) !=
+            hipSuccess)
+            return fail(WW_E_HIP, "pre-sieve buffer: %s", hipGetErrorString(e));
+        c->pre_plain_segs = nseg;
+    }
+    const int pgrid = (int)std::min<uint64_t>((nseg + 255) / 256, (uint64_t)c->cus);
+    if ((e = ww::launch_presieve_range(low, high, c->d_span1, c->d_prefix1, c->d_wide + wfirst, wcount,
+                                       c->d_pre_plain, pgrid > 0 ? pgrid : 1, c->stream)) != hipSuccess)
+        return fail(WW_E_HIP, "pre-sieve: %s", hipGetErrorString(e));
+    *pre = c->d_pre_plain;
+    return WW_OK;
 }
 
 } // namespace
@@ -279,6 +326,9 @@ int ww_close(ww_ctx *c)
     for (hipEvent_t e : c->ev_tests) (void)hipEventDestroy(e);
     if (c->d_deep) (void)hipFree(c->d_deep);
     if (c->d_wide) (void)hipFree(c->d_wide);
+    if (c->d_fstate) (void)hipFree(c->d_fstate);
+    if (c->d_fsum) (void)hipFree(c->d_fsum);
+    if (c->d_fres) (void)hipFree(c->d_fres);
     if (c->d_pre_plain) (void)hipFree(c->d_pre_plain);
     if (c->d_span1) (void)hipFree(c->d_span1);
     if (c->d_prefix1) (void)hipFree(c->d_prefix1);
@@ -375,30 +425,8 @@ int ww_getsegprimes(ww_ctx *c, uint64_t low, uint64_t high)
     hipEvent_t a, b;
     if ((rc = event_pair(c, c->ev_sieve, &a, &b))) return rc;
     HIP_TRY(hipEventRecord(a, c->stream));
-    uint32_t wfirst, wcount;
-    wide_span(c, &wfirst, &wcount);
     const uint64_t *pre = nullptr;
-    if (wcount && ww::segments_of(low, high) >= ww::kPresieveMinSegs) {
-        const uint64_t nseg = ww::segments_of(low, high);
-        if (nseg > c->pre_plain_segs) {
-            /* grow the chunk's pre-sieve buffer; queued launches may still read the old one */
-            HIP_TRY(hipStreamSynchronize(c->stream));
-            if (c->d_pre_plain) (void)hipFree(c->d_pre_plain);
-            c->d_pre_plain = nullptr;
-            c->pre_plain_segs = 0;
-            HIP_TRY(hipMalloc(&c->d_pre_plain, nseg * ww::kPresieveWordsPerSeg * sizeof(uint64_t)));
-            c->pre_plain_segs = nseg;
-        }
-        if (!c->d_span1) {
-            HIP_TRY(hipMalloc(&c->d_span1, sizeof(ww::BatchSpan)));
-            HIP_TRY(hipMalloc(&c->d_prefix1, 2 * sizeof(uint64_t)));
-        }
-        const int pgrid = (int)std::min<uint64_t>((nseg + 255) / 256, (uint64_t)c->cus);
-        HIP_TRY(ww::launch_presieve_range(low, high, c->d_span1, c->d_prefix1, c->d_wide + wfirst,
-                                          wcount, c->d_pre_plain, pgrid > 0 ? pgrid : 1,
-                                          c->stream));
-        pre = c->d_pre_plain;
-    }
+    if ((rc = chunk_presieve(c, low, high, &pre))) return rc;
     HIP_TRY(ww::launch_sieve_prp(low, high, c->d_list, c->list_cap, c->d_state, c->d_deep,
                                  c->ndeep, pre, c->stream));
     HIP_TRY(hipEventRecord(b, c->stream));
@@ -440,6 +468,37 @@ int ww_search_range(ww_ctx *c, uint64_t low, uint64_t high, uint64_t chunk)
         if ((rc = ww_wieferich_wallsunsun(c))) return rc;
         s = e;
     }
+    return WW_OK;
+}
+
+int ww_search_fused(ww_ctx *c, uint64_t low, uint64_t high)
+{
+    int rc = use(c);
+    if (rc) return rc;
+    if (low < WW_MIN_START) return fail(WW_E_ARG, "low %llu below %llu", (unsigned long long)low, WW_MIN_START);
+    if (high <= low) return fail(WW_E_ARG, "empty range");
+    if (high - low > (1ull << 34)) return fail(WW_E_ARG, "range wider than 2^34");
+    if (!c->d_fstate) {
+        hipError_t e = hipMalloc(&c->d_fstate, sizeof(ww::BatchState));
+        if (e == hipSuccess) e = hipMalloc(&c->d_fsum, sizeof(SlotSum));
+        if (e == hipSuccess) e = hipMalloc(&c->d_fres, (size_t)c->result_cap * sizeof(ww_result));
+        if (e != hipSuccess) return fail(WW_E_HIP, "ww_search_fused: %s", hipGetErrorString(e));
+    }
+    hipEvent_t a, b;
+    if ((rc = event_pair(c, c->ev_tests, &a, &b))) return rc;
+    HIP_TRY(hipEventRecord(a, c->stream));
+    const uint64_t *pre = nullptr;
+    if ((rc = chunk_presieve(c, low, high, &pre))) return rc;
+    int nb = c->tuned_blocks > 0 ? c->tuned_blocks : ww::batch_blocks_per_cu();
+    if (nb <= 0) nb = 4;
+    HIP_TRY(ww::launch_batch(c->d_span1, c->d_prefix1, 1, c->d_fstate, c->d_fsum, 1, c->d_fres,
+                             c->result_cap, c->cus * nb, c->tuned_claim, c->d_deep, c->ndeep, pre,
+                             c->stream));
+    HIP_TRY(ww::launch_fold(c->d_fstate, c->d_fsum, c->d_fres, c->result_cap, c->d_state,
+                            c->d_results, c->result_cap, c->stream));
+    HIP_TRY(hipEventRecord(b, c->stream));
+    c->launches++;
+    if (c->ev_tests.size() >= kMaxPendingEvents) return collect(c->ev_tests, &c->ms_tests);
     return WW_OK;
 }
 

@@ -59,8 +59,12 @@ hipError_t launch_presieve(const BatchSpan *spans, const uint64_t *prefix, uint3
                            const WidePrime *tab, uint32_t ntab, uint64_t *out, int grid,
                            hipStream_t s);
 
-/* The same for the single range [lo, hi) of a plain-form chunk (segments from lo | 1);
-   span and prefix are two device words of scratch the launch fills in. */
+/* Writes the one-span table of the range [lo, hi) (segments from lo | 1) into span and
+   prefix (device scratch), so that a single range needs no host copy. */
+hipError_t launch_set_span(uint64_t lo, uint64_t hi, BatchSpan *span, uint64_t *prefix,
+                           hipStream_t s);
+
+/* launch_set_span, then launch_presieve over that one span (a plain-form chunk). */
 hipError_t launch_presieve_range(uint64_t lo, uint64_t hi, BatchSpan *span, uint64_t *prefix,
                                  const WidePrime *tab, uint32_t ntab, uint64_t *out, int grid,
                                  hipStream_t s);
@@ -74,6 +78,12 @@ hipError_t launch_batch(const BatchSpan *spans, const uint64_t *prefix, uint32_t
                         BatchState *st, SlotSum *sums, uint32_t nslots, ww_result *res,
                         uint32_t res_cap, int grid, uint32_t claim, const DeepPrime *deep,
                         uint32_t ndeep, const uint64_t *pre, hipStream_t s);
+
+/* Adds one single-range persistent launch to the plain form's counters and records: its
+   checksum and tested count to the totals, and its records (up to bcap) after the ones
+   already held (pad cleared). The PRP list and its count are left alone. */
+hipError_t launch_fold(const BatchState *bst, const SlotSum *sum, const ww_result *bres,
+                       uint32_t bcap, DevState *ds, ww_result *res, uint32_t cap, hipStream_t s);
 
 } // namespace ww
 

@@ -264,7 +264,38 @@ __global__ void k_batch_clear(BatchState *st, SlotSum *sums, u32 nslots)
     }
 }
 
+/* One workgroup: counters by thread 0, then the records copied in parallel. */
+__global__ void k_fold(const BatchState *bst, const SlotSum *sum, const ww_result *bres, u32 bcap,
+                       DevState *ds, ww_result *res, u32 cap)
+{
+    __shared__ u32 base, n;
+    if (threadIdx.x == 0) {
+        const u32 made = bst->resultcount;
+        n = made < bcap ? made : bcap;
+        base = ds->resultcount;
+        ds->resultcount = base + made;
+        ds->checksum += sum->checksum;
+        ds->totalprimecount += sum->count;
+    }
+    __syncthreads();
+    for (u32 i = threadIdx.x; i < n; i += blockDim.x) {
+        const u32 at = base + i;
+        if (at < cap) {
+            ww_result r = bres[i];
+            r.pad = 0;
+            res[at] = r;
+        }
+    }
+}
+
 } // namespace
+
+hipError_t launch_fold(const BatchState *bst, const SlotSum *sum, const ww_result *bres,
+                       u32 bcap, DevState *ds, ww_result *res, u32 cap, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fold, dim3(1), dim3(256), 0, s, bst, sum, bres, bcap, ds, res, cap);
+    return hipGetLastError();
+}
 
 int batch_blocks_per_cu()
 {

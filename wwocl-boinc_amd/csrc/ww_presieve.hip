@@ -123,12 +123,17 @@ __global__ void k_set_span(BatchSpan *span, u64 *prefix, u64 lo, u64 hi)
 
 } // namespace
 
+hipError_t launch_set_span(u64 lo, u64 hi, BatchSpan *span, u64 *prefix, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_set_span, dim3(1), dim3(1), 0, s, span, prefix, lo, hi);
+    return hipGetLastError();
+}
+
 hipError_t launch_presieve_range(u64 lo, u64 hi, BatchSpan *span, u64 *prefix,
                                  const WidePrime *tab, u32 ntab, u64 *out, int grid,
                                  hipStream_t s)
 {
-    hipLaunchKernelGGL(k_set_span, dim3(1), dim3(1), 0, s, span, prefix, lo, hi);
-    hipError_t e = hipGetLastError();
+    hipError_t e = launch_set_span(lo, hi, span, prefix, s);
     if (e != hipSuccess) return e;
     return launch_presieve(span, prefix, 1, tab, ntab, out, grid, s);
 }

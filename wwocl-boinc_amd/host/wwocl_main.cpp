@@ -200,6 +200,7 @@ struct Run {
     ww_ctx *ctx = nullptr;
     uint64_t list_cap = 0;
     int debug = 0;
+    bool fused = true; /* chunks through the persistent kernel (no PRP list) */
 };
 
 /* checkPRPS (main.cpp:451-523): every list entry is prime or a base-2 PRP with no factor
@@ -256,6 +257,18 @@ void test_kernels(Run &run, uint64_t start, uint64_t top, uint64_t chunk)
         die("ERROR: gpu / cpu result count mismatch %u / %u\n", k.resultcount, (uint32_t)cpu[2]);
     say("GPU Wieferich/WallSunSun kernel tests passed. (%" PRIu64 " checksum, %u results)\n",
         k.checksum, k.resultcount);
+    if (run.fused) {
+        /* the chunk body the search uses must agree with the CPU path too */
+        check(ww_clear_results(run.ctx), "clear results");
+        check(ww_search_fused(run.ctx, start, stop), "persistent kernel");
+        ww_counters f;
+        check(ww_read_counters(run.ctx, &f), "read counters");
+        if (f.totalprimecount != cpu[0] || f.checksum != cpu[1] || f.resultcount != cpu[2])
+            die("ERROR: persistent kernel / cpu mismatch: %" PRIu64 " / %" PRIu64 " PRPs, checksum %"
+                PRIu64 " / %" PRIu64 ", %u / %u results\n",
+                f.totalprimecount, cpu[0], f.checksum, cpu[1], f.resultcount, (uint32_t)cpu[2]);
+        if (run.debug) printf("GPU persistent kernel test passed.\n");
+    }
 }
 
 /* getResults (main.cpp:768-956). */
@@ -378,6 +391,7 @@ int main(int argc, char *argv[])
 
     Run run;
     run.debug = debug;
+    run.fused = !test_prps;
     run.list_cap = ww_list_bound(1ull << 32, (1ull << 32) + chunk); /* worst case: low p */
     if (run.list_cap < ww_list_bound(WW_MIN_START, WW_MIN_START + chunk))
         run.list_cap = ww_list_bound(WW_MIN_START, WW_MIN_START + chunk);
@@ -399,10 +413,14 @@ int main(int argc, char *argv[])
             fflush(stdout);
             last_progress = now;
         }
-        check(ww_clear_prp_count(run.ctx), "clear PRP count");
-        check(ww_getsegprimes(run.ctx, start, stop), "getsegprimes");
-        if (test_prps) printf("GPU primes match CPU (%u PRPs)\n", check_prps(run, start, stop));
-        check(ww_wieferich_wallsunsun(run.ctx), "wieferich/wallsunsun");
+        if (run.fused) {
+            check(ww_search_fused(run.ctx, start, stop), "persistent kernel");
+        } else { /* -testprimes needs the PRP list of the two-kernel form */
+            check(ww_clear_prp_count(run.ctx), "clear PRP count");
+            check(ww_getsegprimes(run.ctx, start, stop), "getsegprimes");
+            if (test_prps) printf("GPU primes match CPU (%u PRPs)\n", check_prps(run, start, stop));
+            check(ww_wieferich_wallsunsun(run.ctx), "wieferich/wallsunsun");
+        }
         if (++in_flight == 3) { /* bound the queue depth as main.cpp:1685-1691 */
             check(ww_finish(run.ctx), "finish");
             in_flight = 0;

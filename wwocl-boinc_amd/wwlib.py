@@ -167,6 +167,7 @@ _SIGNATURES = [
     ("ww_cpu_residues", ctypes.c_int, [ctypes.POINTER(U64), U64, ctypes.c_int32,
                                        ctypes.c_int32, ctypes.POINTER(U64)]),
     ("ww_cpu_spsp2", ctypes.c_int, [U64, ctypes.c_int32, ctypes.c_int32]),
+    ("ww_search_fused", ctypes.c_int, [ctypes.c_void_p, U64, U64]),
     ("ww_cpu_range", ctypes.c_int, [U64, U64, ctypes.POINTER(U64)]),
     ("ww_deep_table", U32, [U32, ctypes.POINTER(DeepPrime), U32]),
     ("ww_presieve_table", U32, [U32, U32, ctypes.POINTER(WidePrime), U32]),
@@ -360,6 +361,9 @@ class Context:
     def clear_prp_count(self):
         _check(self.lib.ww_clear_prp_count(self.h))
 
+    def search_fused(self, low: int, high: int):
+        _check(self.lib.ww_search_fused(self.h, low, high))
+
     def getsegprimes(self, low: int, high: int):
         _check(self.lib.ww_getsegprimes(self.h, low, high))
 
@@ -417,6 +421,21 @@ class Context:
         return [bool(v) for v in out]
 
     # convenience: one whole range, counters cleared first
+    def run_range_fused(self, low: int, high: int, chunk: int = 1 << 32) -> RangeResult:
+        """run_range with each chunk through ww_search_fused (no PRP list)."""
+        self.clear_results()
+        self.clear_prp_count()
+        s = low
+        while s < high:
+            e = min(high, s + chunk)
+            self.search_fused(s, e)
+            s = e
+        self.finish()
+        c = self.counters()
+        if c.resultcount > self.result_capacity:
+            raise WWError(WW_E_CAPACITY, f"{c.resultcount} results overflowed the buffer")
+        return RangeResult(c.totalprimecount, c.checksum, sort_results(self.results()))
+
     def run_range(self, low: int, high: int, chunk: int = 0) -> RangeResult:
         self.clear_results()
         self.clear_prp_count()
