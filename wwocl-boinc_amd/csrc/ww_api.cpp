@@ -234,6 +234,11 @@ int ww_device_info(int device, ww_device_info_t *info)
     snprintf(info->name, sizeof info->name, "%s", p.name);
     if (!info->name[0]) (void)hipDeviceGetName(info->name, (int)sizeof info->name, device);
     snprintf(info->arch, sizeof info->arch, "%s", p.gcnArchName);
+    if (!info->name[0]) { /* some runtimes report no marketing name: use the target */
+        const char *colon = strchr(info->arch, ':');
+        const int n = colon ? (int)(colon - info->arch) : (int)strlen(info->arch);
+        snprintf(info->name, sizeof info->name, "AMD GPU %.*s", n, info->arch);
+    }
     info->compute_units = p.multiProcessorCount;
     info->clock_khz = p.clockRate;
     info->global_mem = (uint64_t)p.totalGlobalMem;
