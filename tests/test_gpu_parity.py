@@ -215,3 +215,28 @@ def test_fused_chunks_at_full_size(gctx):
     assert (fused.prp_count, fused.checksum, fused.results) == \
         (plain.prp_count, plain.checksum, plain.results)
     assert f"{fused.checksum:016X}" == "0BB982E18153E292"
+
+
+def test_random_ranges_all_forms_vs_oracle(gctx):
+    """200 random ranges across the 64-bit domain (widths 1 to 2^20 above 2^20, up to 2^12
+    below it, where nearly every prime yields records): the two-kernel form, fused chunks
+    and one persistent batch all give the oracle's PRP count, checksum and records."""
+    rng = random.Random(31)
+    ranges = []
+    while len(ranges) < 200:
+        bits = rng.randint(7, 64)
+        lo = max(127, rng.getrandbits(bits))
+        width = rng.randint(1, 1 << (20 if bits > 20 else 12))
+        hi = min(lo + width, (1 << 64) - 1)
+        if hi > lo:
+            ranges.append((lo, hi))
+    with wwlib.Batch(gctx, ranges) as b:
+        b.launch(graph=True)
+        batched = b.read()
+    for (lo, hi), bat in zip(ranges, batched):
+        want = oracle.range_results(lo, hi)
+        plain = gctx.run_range(lo, hi)
+        fused = gctx.run_range_fused(lo, hi)
+        assert (plain.prp_count, plain.checksum, plain.results) == want, (lo, hi)
+        assert (fused.prp_count, fused.checksum, fused.results) == want, (lo, hi)
+        assert bat == want, (lo, hi)
