@@ -19,15 +19,20 @@ __device__ constexpr u64 stride_pattern(u32 L)
     return m;
 }
 
-/* Marks the bits of `comp` whose odd number w0 + 2i is divisible by L, given
-   base_mod = (word group base) mod L and lane_off = w0 - (word group base) < 2^32. */
+/* Marks the bits of `comp` whose odd number w0 + 2i is divisible by L, for a lane word at
+   w0 = group + 2 half_off (half_off < 2^15) and gneg = (-group / 2) mod L (wave-uniform).
+   L | w0 + 2i exactly when i = (gneg - half_off) mod L. x = gneg + C - half_off with C a
+   multiple of L above 2^15 stays below 2^16, where the quotient mulhi(x, ceil(2^32 / L))
+   is exact (its error is below x / 2^32 < 1/L), so the residue costs three VALU
+   instructions per prime and lane. */
 template <u32 L>
-__device__ __forceinline__ void sieve_prime(u64 &comp, u32 base_mod, u32 lane_off)
+__device__ __forceinline__ void sieve_prime(u64 &comp, u32 gneg, u32 half_off)
 {
-    constexpr u32 inv2 = (L + 1) / 2; /* 2^-1 mod L */
     constexpr u64 pat = stride_pattern(L);
-    u32 r = (base_mod + lane_off % L) % L;  /* w0 mod L */
-    u32 i0 = ((L - r) * inv2) % L;          /* first i with L | w0 + 2i */
+    constexpr u32 C = L * ((1u << 15) / L + 1);
+    constexpr u32 M = (u32)(((1ull << 32) + L - 1) / L);
+    const u32 x = gneg + C - half_off;
+    const u32 i0 = x - __umulhi(x, M) * L; /* first i with L | w0 + 2i */
     if (L < 64) {
         comp |= pat << i0;
     } else {
@@ -42,18 +47,18 @@ __device__ __forceinline__ void sieve_prime(u64 &comp, u32 base_mod, u32 lane_of
     X(53) X(59) X(61) X(67) X(71) X(73) X(79) X(83) X(89) X(97) X(101) X(103) X(107) \
     X(109) X(113)
 
-/* Candidate bits of the word of odd numbers w0 + 2i (i < 64), w0 = group + lane_off:
-   no factor in 3..113, and w0 + 2i < hi. Words that wrapped past 2^64 (w0 < group)
-   yield nothing. `group` is uniform across the wave, so its residues are scalar work. */
+/* Candidate bits of the word of odd numbers w0 + 2i (i < 64), w0 = group + lane_off
+   (lane_off even, below 2^16): no factor in 3..113, and w0 + 2i < hi. Words that wrapped
+   past 2^64 (w0 < group) yield nothing. `group` is uniform across the wave, so its
+   residues are scalar work. */
 __device__ __forceinline__ u64 sieve_word(u64 group, u32 lane_off, u64 hi)
 {
-    /* An empty asm makes lane_off opaque, so a persistent caller recomputes the per-lane
-       residues lane_off % L (a few instructions each) instead of hoisting all 29 of them
-       out of its loop into live registers. */
-    asm volatile("" : "+v"(lane_off));
     const u64 w0 = group + lane_off;
+    const u32 half_off = lane_off >> 1;
     u64 comp = 0;
-#define WW_SIEVE_ONE(L) sieve_prime<L>(comp, (u32)(group % L), lane_off);
+    /* gneg = (-group) * 2^-1 mod L, with 2^-1 = (L + 1) / 2 */
+#define WW_SIEVE_ONE(L)                                                                  \
+    sieve_prime<L>(comp, (u32)(((L - group % L) % L) * ((L + 1) / 2) % L), half_off);
     WW_SIEVE_PRIMES(WW_SIEVE_ONE)
 #undef WW_SIEVE_ONE
     u64 valid = 0;
