@@ -149,9 +149,12 @@ int chunk_presieve(ww_ctx *c, uint64_t low, uint64_t high, const uint64_t **pre)
     *pre = nullptr;
     hipError_t e;
     if (!c->d_span1) {
-        if ((e = hipMalloc(&c->d_span1, sizeof(ww::BatchSpan))) != hipSuccess ||
-            (e = hipMalloc(&c->d_prefix1, 2 * sizeof(uint64_t))) != hipSuccess)
+        if ((e = hipMalloc(&c->d_prefix1, 2 * sizeof(uint64_t))) != hipSuccess ||
+            (e = hipMalloc(&c->d_span1, sizeof(ww::BatchSpan))) != hipSuccess) { /* span last */
+            if (c->d_prefix1) (void)hipFree(c->d_prefix1);
+            c->d_prefix1 = nullptr;
             return fail(WW_E_HIP, "chunk span: %s", hipGetErrorString(e));
+        }
     }
     uint32_t wfirst, wcount;
     wide_span(c, &wfirst, &wcount);
@@ -385,6 +388,7 @@ int ww_set_presieve_bound(ww_ctx *c, uint32_t bound)
         if (e != hipSuccess) {
             if (c->d_wide) (void)hipFree(c->d_wide);
             c->d_wide = nullptr;
+            c->presieve_bound = 0; /* no table: the pre-sieve is off */
             return fail(WW_E_HIP, "pre-sieve table: %s", hipGetErrorString(e));
         }
         for (const ww_wide_prime &d : t) c->wide_ells.push_back(d.ell);
@@ -484,10 +488,17 @@ int ww_search_fused(ww_ctx *c, uint64_t low, uint64_t high)
     if (high <= low) return fail(WW_E_ARG, "empty range");
     if (high - low > (1ull << 34)) return fail(WW_E_ARG, "range wider than 2^34");
     if (!c->d_fstate) {
-        hipError_t e = hipMalloc(&c->d_fstate, sizeof(ww::BatchState));
-        if (e == hipSuccess) e = hipMalloc(&c->d_fsum, sizeof(SlotSum));
+        hipError_t e = hipMalloc(&c->d_fsum, sizeof(SlotSum));
         if (e == hipSuccess) e = hipMalloc(&c->d_fres, (size_t)c->result_cap * sizeof(ww_result));
-        if (e != hipSuccess) return fail(WW_E_HIP, "ww_search_fused: %s", hipGetErrorString(e));
+        if (e == hipSuccess) e = hipMalloc(&c->d_fstate, sizeof(ww::BatchState)); /* last: marks all made */
+        if (e != hipSuccess) {
+            if (c->d_fsum) (void)hipFree(c->d_fsum);
+            if (c->d_fres) (void)hipFree(c->d_fres);
+            c->d_fsum = nullptr;
+            c->d_fres = nullptr;
+            c->d_fstate = nullptr;
+            return fail(WW_E_HIP, "ww_search_fused: %s", hipGetErrorString(e));
+        }
     }
     hipEvent_t a, b;
     if ((rc = event_pair(c, c->ev_tests, &a, &b))) return rc;
