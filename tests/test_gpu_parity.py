@@ -77,6 +77,7 @@ RANGES = [
     (T4_P, T4_P + 5000),                       # WSS special instance at START (inclusive)
     ((1 << 62) - 12345, (1 << 62) + 54321),
     (1 << 63, (1 << 63) + (1 << 22)),          # 2^63 slice
+    ((1 << 63) - (1 << 20), (1 << 63) + (1 << 20)),  # 2^64 mod p: quotient 2 below, 1 above
     ((1 << 64) - (1 << 22), (1 << 64) - 1),    # top of the domain
     ((1 << 64) - 100, (1 << 64) - 1),
 ]
