@@ -91,6 +91,24 @@ def test_start_clamps_to_127_and_exponent_form(tmp_path):
     assert "Starting search at: 127\nStopping search at: 60000000000" in out
 
 
+def test_device_choice_as_boinc_gets_it(tmp_path):
+    """boinc_get_opencl_ids (main.cpp:1288-1308): without init_data.xml the program is
+    standalone and takes device 0; init_data.xml's <gpu_device_num> or a --device argument
+    names another. Devices past the last one are refused."""
+    code, out, _ = run(tmp_path, 127, 1000, "--device", 7)
+    assert code != 0 and "device 7 of" in out and "init_data.xml not found" not in out
+    (tmp_path / "init_data.xml").write_text(
+        "<app_init_data>\n<gpu_device_num>5</gpu_device_num>\n</app_init_data>\n")
+    code, out, _ = run(tmp_path, 127, 1000)
+    assert code != 0 and "device 5 of" in out and "init_data.xml not found" not in out
+    (tmp_path / "init_data.xml").unlink()
+    (tmp_path / "results-WW.txt").unlink()
+    if no_gpu():
+        code, out, _ = run(tmp_path, 127, 1000)
+        assert "init_data.xml not found, using device 0." in out
+        assert code != 0 and "device 0 of 0" in out
+
+
 # ----------------------------------------------------------------------------- GPU
 
 
@@ -111,8 +129,12 @@ def expected_lines(lo, hi):
 
 @pytest.mark.gpu
 def test_host_small_range_matches_oracle(tmp_path):
+    # a BOINC client's init_data.xml names the device (the box's only one)
+    (tmp_path / "init_data.xml").write_text("<app_init_data><gpu_device_num>0</gpu_device_num>"
+                                            "</app_init_data>\n")
     rc, out, err = run(tmp_path, 1, "3e6")
     assert rc == 0, out + err
+    assert "init_data.xml not found" not in out
     assert "GPU Wieferich/WallSunSun kernel tests passed" in out
     text = results(tmp_path).splitlines()
     prp, cks, _ = oracle.range_results(127, 3_000_000)

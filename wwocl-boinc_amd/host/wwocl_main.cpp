@@ -12,10 +12,13 @@
  *   chunk loop, checkpoint every > 60 s             main.cpp:1683-1795
  *   result validation, sorting, line formats        main.cpp:768-956
  *   final checksum line                             main.cpp:1758-1782
- * BOINC is absent, so the program always runs in standalone mode (messages go to both
- * stdout and stderr where the reference prints both in standalone mode). The device
- * work-size probe (main.cpp:959-1107) is replaced by a fixed chunk size: the checksum
- * does not depend on chunking.
+ * BOINC's library is absent: the program keeps the standalone behaviour (messages go to
+ * both stdout and stderr where the reference prints both in standalone mode, progress as
+ * "Tests done" lines) and takes the device the way boinc_get_opencl_ids does
+ * (main.cpp:1288-1308): --device N on the command line, else <gpu_device_num> or
+ * <gpu_opencl_dev_index> from init_data.xml, else device 0 with the reference's
+ * "init_data.xml not found" message. The device work-size probe (main.cpp:959-1107) is
+ * replaced by a fixed chunk size: the checksum does not depend on chunking.
  */
 #include <algorithm>
 #include <cinttypes>
@@ -313,6 +316,30 @@ void get_results(Run &run, int ckpt_time, State &st)
     ++st.pcnt;
 }
 
+/* The device a BOINC client assigned, as boinc_get_opencl_ids finds it (main.cpp:1288-1308):
+   "--device N" (older clients pass it on the command line), else the device number in
+   init_data.xml, else standalone mode: device 0. */
+int pick_device(int argc, char *argv[])
+{
+    for (int i = 3; i + 1 < argc; ++i)
+        if (strcmp(argv[i], "--device") == 0) return atoi(argv[i + 1]);
+    FILE *f = fopen("init_data.xml", "r");
+    if (!f) {
+        printf("init_data.xml not found, using device 0.\n");
+        return 0;
+    }
+    std::string xml;
+    char buf[4096];
+    size_t n;
+    while ((n = fread(buf, 1, sizeof buf, f)) > 0) xml.append(buf, n);
+    fclose(f);
+    for (const char *tag : {"<gpu_device_num>", "<gpu_opencl_dev_index>"}) {
+        size_t at = xml.find(tag);
+        if (at != std::string::npos) return atoi(xml.c_str() + at + strlen(tag));
+    }
+    return 0;
+}
+
 void usage(const char *prog)
 {
     printf("Usage: %s START END -testprimes -compute -debug -ws #\nWhere 127 <= START < END < 2^64\n", prog);
@@ -384,8 +411,12 @@ int main(int argc, char *argv[])
     uint64_t chunk = 1ull << 32;
     if (!compute && ws) chunk = std::max<uint64_t>(1ull << 20, chunk / 100 * (uint64_t)ws);
 
+    const int device = pick_device(argc, argv);
+    int ndev = 0;
+    if (ww_device_count(&ndev) != WW_OK || device < 0 || device >= ndev)
+        die("Error: boinc_get_opencl_ids() failed: device %d of %d\n", device, ndev);
     ww_device_info_t info;
-    check(ww_device_info(0, &info), "device info");
+    check(ww_device_info(device, &info), "device info");
     say("GPU Info:\n  Name: \t\t%s\n  Arch: \t\t%s\n  Compute Units: \t%d\n", info.name, info.arch,
         info.compute_units);
 
@@ -395,7 +426,7 @@ int main(int argc, char *argv[])
     run.list_cap = ww_list_bound(1ull << 32, (1ull << 32) + chunk); /* worst case: low p */
     if (run.list_cap < ww_list_bound(WW_MIN_START, WW_MIN_START + chunk))
         run.list_cap = ww_list_bound(WW_MIN_START, WW_MIN_START + chunk);
-    check(ww_open(0, run.list_cap, kNumResults, &run.ctx), "ww_open");
+    check(ww_open(device, run.list_cap, kNumResults, &run.ctx), "ww_open");
     if (debug) printf("chunk size: %" PRIu64 " integers, list capacity %" PRIu64 "\n", chunk, run.list_cap);
 
     test_kernels(run, st.current, top, chunk);
