@@ -105,7 +105,7 @@ __device__ __forceinline__ void tests_round(u32 n, u32 &qn, u64 *wq, u32 *ws, u3
 }
 
 #ifndef WW_BATCH_WAVES_PER_EU
-#define WW_BATCH_WAVES_PER_EU 6 /* 80 VGPRs: six 4-wave workgroups per CU */
+#define WW_BATCH_WAVES_PER_EU 5 /* 96 VGPRs: five 4-wave workgroups per CU */
 #endif
 
 __global__ __launch_bounds__(64 * kBatchWaves)
