@@ -27,6 +27,44 @@ def test_library_exports_every_declared_symbol():
     assert sorted(wwlib.EXPORTED) == declared
 
 
+C_CALLER = r"""
+#include <stdio.h>
+#include "ww_api.h"
+/* a C caller of the ABI: host-side entry points only (no device needed) */
+int main(void)
+{
+    uint64_t out[3], r[4];
+    uint64_t p = 1093;
+    if (ww_cpu_range(127, 100000, out) != WW_OK) return 1;
+    if (ww_cpu_residues(&p, 1, -1, -1, r) != WW_OK) return 2;
+    if (ww_cpu_spsp2(4294967297ull, -1, -1) != 1) return 3;
+    if (ww_list_bound(1ull << 40, (1ull << 40) + 1000000) == 0) return 4;
+    printf("%llu %llu %llu %llu\n", (unsigned long long)out[0], (unsigned long long)out[1],
+           (unsigned long long)r[0], (unsigned long long)r[1]);
+    return 0;
+}
+"""
+
+
+def test_header_is_plain_c_and_links(tmp_path):
+    """include/ww_api.h compiles as C11 (plain C types only) and a C program links and runs
+    against libww.so."""
+    import subprocess
+    src = tmp_path / "caller.c"
+    src.write_text(C_CALLER)
+    exe = tmp_path / "caller"
+    lib_dir = os.path.join(ROOT, "wwocl-boinc_amd")
+    r = subprocess.run(["gcc", "-std=c11", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"),
+                        "-o", str(exe), str(src), "-L", lib_dir, "-lww", "-Wl,-rpath," + lib_dir],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    r = subprocess.run([str(exe)], capture_output=True, text=True)
+    assert r.returncode == 0, (r.returncode, r.stderr)
+    prp, cks = oracle.range_summary(127, 100000)[:2]
+    s0, s1 = oracle.wieferich(1093)
+    assert r.stdout.split() == [str(prp), str(cks), str(s0), str(s1)]
+
+
 def test_no_device_is_reported_cleanly():
     if wwlib.device_count() > 0:
         pytest.skip("a GPU is visible")
