@@ -536,6 +536,25 @@ WW_HD m2 m2_sqr_add(m2 x, m2 k, u64 p, u64 q, u64 np)
     return r;
 }
 
+/* v*y / 2^64 mod p^2 for one ordinary digit v < p: the product with a zero high digit.
+   As in m2_sqr, the cross term v*y1 + u0 <= (p-1)^2 + 2^64 - 1 has a high word below p.
+   With y = R^2 mod p^2 (R = 2^64) this is v's Montgomery form, v*R mod p^2. */
+WW_HD m2 m2_mul_digit(u64 v, m2 y, u64 p, u64 q)
+{
+    u128 t = mul_wide(v, y.d0);
+    u64 u0 = t.lo * q;
+    u64 v1 = mul_hi(p, u0);
+    u128 c = mul_wide_add(v, y.d1, u0);
+    u64 up0 = c.lo * q;
+    u64 v1p = mul_hi(p, up0);
+    bool b;
+    u64 z0 = sub_b(t.hi, v1, &b);
+    m2 r;
+    r.d0 = b ? z0 + p : z0;
+    r.d1 = sub_mod_c(c.hi, v1p, b, p);
+    return r;
+}
+
 /* REDC(x*y + k), the Montgomery product plus a constant as in m2_sqr_add. The cross sum
    x0*y1 + u0 + x1*y0 + k1 is below 2p*2^64 (at most 2p^2 - 3p + 2^64), a 129-bit value
    whose high part one conditional subtraction of p brings below p. */
@@ -647,6 +666,20 @@ WW_HD m2 wieferich_residue(u64 p, u64 q, m2 one)
     return wieferich_residue_w(p, q, one, 1.0 / (double)p, 63 - clz64(p >> 1));
 }
 
+/* V_k(3, 1) = L_2k for k = 0..32: V_0 = 2, V_1 = 3, V_k+1 = 3 V_k - V_k-1. The last,
+   L_64 = 23725150497407, is below 2^45. */
+constexpr int kWssTableBits = 5;
+constexpr u64 kLucasV31[(1 << kWssTableBits) + 1] = {
+    2ull, 3ull, 7ull, 18ull, 47ull, 123ull, 322ull, 843ull, 2207ull, 5778ull, 15127ull,
+    39603ull, 103682ull, 271443ull, 710647ull, 1860498ull, 4870847ull, 12752043ull,
+    33385282ull, 87403803ull, 228826127ull, 599074578ull, 1568397607ull, 4106118243ull,
+    10749957122ull, 28143753123ull, 73681302247ull, 192900153618ull, 505019158607ull,
+    1322157322203ull, 3461452808002ull, 9062201101803ull, 23725150497407ull};
+
+/* Whether the Wall-Sun-Sun ladder for p may start from kLucasV31: every entry is then an
+   ordinary digit below p. */
+WW_HD bool wallsunsun_table_ok(u64 p) { return p > kLucasV31[1 << kWssTableBits]; }
+
 /* F_{p-(p/5)} mod p^2 as digits (c20, c21) (wallsunsun.cl:242-300, wwcpu.cpp:311-372).
    m = p - (p/5) is even, and F_{2n} = U_n(3, 1), the Lucas sequence with P = 3, Q = 1
    (its roots are the squares of the golden ratio and its conjugate). Its companion V
@@ -658,8 +691,11 @@ WW_HD m2 wieferich_residue(u64 p, u64 q, m2 one)
    square of the operand the bit picks goes to y, so after bit i slot x holds V_k exactly
    when bit i is 1, and the square's operand is y iff bits i and i+1 of n are equal. The
    ladder starts at k = 0, (V_0, V_1) = (2, 3), which leading zero bits leave unchanged,
-   so any top in [leading bit of n, 63] (e.g. common to a wave) gives the same result. */
-WW_HD m2 wallsunsun_residue_w(u64 p, u64 q, m2 one, double pinv, int top)
+   so any top in [leading bit of n, 63] (e.g. common to a wave) gives the same result.
+   With `table` (allowed when wallsunsun_table_ok(p)), the first five steps are read from
+   kLucasV31 instead: k = the bits of n above top - 5, and V_k, V_k+1 enter Montgomery
+   form as products with R^2, two products in place of ten. */
+WW_HD m2 wallsunsun_residue_w(u64 p, u64 q, m2 one, double pinv, int top, bool table)
 {
     u32 pm5 = mod5(p);
     u64 n = ((pm5 == 2 || pm5 == 3) ? p + 1 : p - 1) >> 1;
@@ -675,11 +711,24 @@ WW_HD m2 wallsunsun_residue_w(u64 p, u64 q, m2 one, double pinv, int top)
     WW_LOOP_LOCAL(np); /* opaque, so that adding it is not turned back into subtracting p */
     m2 x = three;                               /* k = 0: bit top+1 is 0, x holds V_1 */
     m2 y = two;                                 /* and y holds V_0 */
-    /* The bits of `same` are read one 32-bit word at a time, high word first when top
+    int start = top;                            /* next bit of n */
+    if (table) {
+        /* the bits top..top-4 done: k < 32, and slot x holds V_k iff k is odd */
+        start = top - kWssTableBits;
+        const u32 k = (u32)(n >> (start + 1));
+        const m2 vk = m2_mul_digit(kLucasV31[k], r2, p, q);
+        const m2 vk1 = m2_mul_digit(kLucasV31[k + 1], r2, p, q);
+        const bool kodd = k & 1;
+        x.d0 = kodd ? vk.d0 : vk1.d0;
+        x.d1 = kodd ? vk.d1 : vk1.d1;
+        y.d0 = kodd ? vk1.d0 : vk.d0;
+        y.d1 = kodd ? vk1.d1 : vk.d1;
+    }
+    /* The bits of `same` are read one 32-bit word at a time, high word first when start
        >= 32: the word is doubled each step and the carry out is the bit (one
        v_add_co_u32 on the device gives both). */
-    for (int half = top >= 32 ? 1 : 0; half >= 0; --half) {
-        const int lead = half ? top - 32 : (top >= 32 ? 31 : top); /* first bit, in the word */
+    for (int half = start >= 32 ? 1 : 0; half >= 0 && start >= 0; --half) {
+        const int lead = half ? start - 32 : (start >= 32 ? 31 : start); /* first bit, in the word */
         u32 w = (half ? (u32)(same >> 32) : (u32)same) << (31 - lead);
         for (int j = lead; j >= 0; --j) {
             WW_LOOP_LOCAL(p);
@@ -714,7 +763,8 @@ WW_HD int wallsunsun_top(u64 p)
 
 WW_HD m2 wallsunsun_residue(u64 p, u64 q, m2 one)
 {
-    return wallsunsun_residue_w(p, q, one, 1.0 / (double)p, wallsunsun_top(p));
+    return wallsunsun_residue_w(p, q, one, 1.0 / (double)p, wallsunsun_top(p),
+                                wallsunsun_table_ok(p));
 }
 
 } // namespace ww

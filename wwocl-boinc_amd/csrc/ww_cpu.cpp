@@ -100,7 +100,7 @@ int ww_cpu_residues(const uint64_t *p, uint64_t n, int32_t top_w, int32_t top_f,
         const m2 one = m2_one(pi);
         const double pinv = 1.0 / (double)pi;
         const m2 s = wieferich_residue_w(pi, q, one, pinv, tw);
-        const m2 f = wallsunsun_residue_w(pi, q, one, pinv, tf);
+        const m2 f = wallsunsun_residue_w(pi, q, one, pinv, tf, wallsunsun_table_ok(pi));
         out[4 * i] = s.d0;
         out[4 * i + 1] = s.d1;
         out[4 * i + 2] = f.d0;
