@@ -197,11 +197,15 @@ int ww_cpu_is_prp(uint64_t n);
 /* The device arithmetic built for the host, for verification (no counterpart in the
    reference). ww_cpu_residues writes out[4*i..4*i+3] as ww_residues does, running the
    ladders from the given leading-bit bounds (any value from p's own leading bit to 63
-   gives the same result, which is what lets a wave share one bound; -1 = p's own).
+   gives the same result, which is what lets a wave share one bound; -1 = p's own) and
+   with wss_skip leading Wall-Sun-Sun steps read from the table of V_k(3, 1): 5, or 6 when
+   the six leading bits of the ladder's exponent below top_f give k <= 45 (a wave takes 6
+   when all its lanes allow it), -1 = the largest this p allows.
    ww_cpu_spsp2 is the windowed strong PRP test of n with bounds top (>= the leading bit
    of n - 1, <= 63) and tmax (>= the trailing zero count of n - 1, <= top), -1 = n's own:
    1 or 0, or WW_E_ARG. */
-int ww_cpu_residues(const uint64_t *p, uint64_t n, int32_t top_w, int32_t top_f, uint64_t *out);
+int ww_cpu_residues(const uint64_t *p, uint64_t n, int32_t top_w, int32_t top_f,
+                    int32_t wss_skip, uint64_t *out);
 int ww_cpu_spsp2(uint64_t n, int32_t top, int32_t tmax);
 
 /* One deep-sieve prime: ell < 2^16, its base-2 pseudoprime class rho mod m with

@@ -36,7 +36,7 @@ int main(void)
     uint64_t out[3], r[4];
     uint64_t p = 1093;
     if (ww_cpu_range(127, 100000, out) != WW_OK) return 1;
-    if (ww_cpu_residues(&p, 1, -1, -1, r) != WW_OK) return 2;
+    if (ww_cpu_residues(&p, 1, -1, -1, -1, r) != WW_OK) return 2;
     if (ww_cpu_spsp2(4294967297ull, -1, -1) != 1) return 3;
     if (ww_list_bound(1ull << 40, (1ull << 40) + 1000000) == 0) return 4;
     printf("%llu %llu %llu %llu\n", (unsigned long long)out[0], (unsigned long long)out[1],

@@ -2,7 +2,8 @@
 ww_cpu_spsp2: the Wieferich and Wall-Sun-Sun ladders and the windowed strong PRP test
 give the oracle's values from every leading-bit bound a wave may share (any value from
 the lane's own up to 63), including bounds that cross the 32-bit word boundary the
-Wall-Sun-Sun ladder reads its bits by. No device needed."""
+Wall-Sun-Sun ladder reads its bits by, and with either length of the Wall-Sun-Sun
+ladder's table start. No device needed."""
 import random
 
 import pytest
@@ -47,6 +48,31 @@ def test_ladders_from_loosened_bounds():
         bounds.add((rng.randint(lw, 63), rng.randint(lf, 63)))
         for tw, tf in bounds:
             assert wwlib.cpu_residues([p], tw, tf) == want, (p, tw, tf)
+
+
+def test_wss_table_start_lengths():
+    """Five table steps are always allowed and six when the six leading bits of n below
+    the bound give k <= 45 (V_k+1 below 2^64); both match the oracle, also for p below the
+    table's entries (digits above p), and a refused six is an error."""
+    rng = random.Random(24)
+    ps = sample_primes(rng, 300) + [127, 131, 137, 1093, 3511, (1 << 64) - 59]
+    sixes = fives_only = 0
+    for p in ps:
+        want = tuple(oracle.wieferich(p)) + tuple(oracle.wallsunsun(p))
+        n = (p + 1 if p % 5 in (2, 3) else p - 1) // 2
+        lf = lead(n)
+        for tf in sorted({lf, min(lf + 1, 63), rng.randint(lf, 63), 63}):
+            assert wwlib.cpu_residues([p], -1, tf, 5) == [want], (p, tf)
+            if (n >> (tf - 5)) <= 45:
+                sixes += 1
+                assert wwlib.cpu_residues([p], -1, tf, 6) == [want], (p, tf)
+            else:
+                fives_only += 1
+                with pytest.raises(wwlib.WWError):
+                    wwlib.cpu_residues([p], -1, tf, 6)
+    assert sixes > 200 and fives_only > 100
+    with pytest.raises(wwlib.WWError):
+        wwlib.cpu_residues([1093], -1, -1, 4)
 
 
 def test_windowed_prp_from_loosened_bounds():

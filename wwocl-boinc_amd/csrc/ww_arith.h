@@ -536,9 +536,10 @@ WW_HD m2 m2_sqr_add(m2 x, m2 k, u64 p, u64 q, u64 np)
     return r;
 }
 
-/* v*y / 2^64 mod p^2 for one ordinary digit v < p: the product with a zero high digit.
-   As in m2_sqr, the cross term v*y1 + u0 <= (p-1)^2 + 2^64 - 1 has a high word below p.
-   With y = R^2 mod p^2 (R = 2^64) this is v's Montgomery form, v*R mod p^2. */
+/* v*y / 2^64 mod p^2 for any 64-bit v (the product with a zero high digit; v may exceed
+   p). v*y0 and the cross term v*y1 + u0 are both at most (2^64 - 1) p < p*2^64, so their
+   high words are below p as in m2_sqr. With y = R^2 mod p^2 (R = 2^64) this is v's
+   Montgomery form, v*R mod p^2. */
 WW_HD m2 m2_mul_digit(u64 v, m2 y, u64 p, u64 q)
 {
     u128 t = mul_wide(v, y.d0);
@@ -666,19 +667,34 @@ WW_HD m2 wieferich_residue(u64 p, u64 q, m2 one)
     return wieferich_residue_w(p, q, one, 1.0 / (double)p, 63 - clz64(p >> 1));
 }
 
-/* V_k(3, 1) = L_2k for k = 0..32: V_0 = 2, V_1 = 3, V_k+1 = 3 V_k - V_k-1. The last,
-   L_64 = 23725150497407, is below 2^45. */
-constexpr int kWssTableBits = 5;
-constexpr u64 kLucasV31[(1 << kWssTableBits) + 1] = {
+/* V_k(3, 1) = L_2k for k = 0..46: V_0 = 2, V_1 = 3, V_k+1 = 3 V_k - V_k-1. V_46 = L_92 is
+   the last below 2^64. */
+constexpr u32 kLucasV31Max = 46;
+constexpr u64 kLucasV31[kLucasV31Max + 1] = {
     2ull, 3ull, 7ull, 18ull, 47ull, 123ull, 322ull, 843ull, 2207ull, 5778ull, 15127ull,
     39603ull, 103682ull, 271443ull, 710647ull, 1860498ull, 4870847ull, 12752043ull,
     33385282ull, 87403803ull, 228826127ull, 599074578ull, 1568397607ull, 4106118243ull,
     10749957122ull, 28143753123ull, 73681302247ull, 192900153618ull, 505019158607ull,
-    1322157322203ull, 3461452808002ull, 9062201101803ull, 23725150497407ull};
+    1322157322203ull, 3461452808002ull, 9062201101803ull, 23725150497407ull,
+    62113250390418ull, 162614600673847ull, 425730551631123ull, 1114577054219522ull,
+    2918000611027443ull, 7639424778862807ull, 20000273725560978ull, 52361396397820127ull,
+    137083915467899403ull, 358890350005878082ull, 939587134549734843ull,
+    2459871053643326447ull, 6440026026380244498ull, 16860207025497407047ull};
 
-/* Whether the Wall-Sun-Sun ladder for p may start from kLucasV31: every entry is then an
-   ordinary digit below p. */
-WW_HD bool wallsunsun_table_ok(u64 p) { return p > kLucasV31[1 << kWssTableBits]; }
+/* The Wall-Sun-Sun ladder's n = (p - (p/5)) / 2. */
+WW_HD u64 wallsunsun_n(u64 p)
+{
+    u32 pm5 = mod5(p);
+    return ((pm5 == 2 || pm5 == 3) ? p + 1 : p - 1) >> 1;
+}
+
+/* How many leading ladder steps (bits top .. top-skip+1 of n) kLucasV31 can stand for:
+   6 when those six bits give k <= 45 (so V_k+1 is in the table), else 5 (k <= 31).
+   Every p >= 127 prime to 5 has n >= 64, so top >= 6 and the ladder keeps a bit. */
+WW_HD int wallsunsun_skip(u64 n, int top)
+{
+    return top >= 6 && (n >> (top - 5)) < kLucasV31Max ? 6 : 5;
+}
 
 /* F_{p-(p/5)} mod p^2 as digits (c20, c21) (wallsunsun.cl:242-300, wwcpu.cpp:311-372).
    m = p - (p/5) is even, and F_{2n} = U_n(3, 1), the Lucas sequence with P = 3, Q = 1
@@ -689,19 +705,16 @@ WW_HD bool wallsunsun_table_ok(u64 p) { return p > kLucasV31[1 << kWssTableBits]
    per bit with no sign-dependent constants, and 5 U_n = 2 V_{n+1} - 3 V_n at the end.
    The pair lives in two fixed slots (x, y): the product is symmetric and goes to x, the
    square of the operand the bit picks goes to y, so after bit i slot x holds V_k exactly
-   when bit i is 1, and the square's operand is y iff bits i and i+1 of n are equal. The
-   ladder starts at k = 0, (V_0, V_1) = (2, 3), which leading zero bits leave unchanged,
-   so any top in [leading bit of n, 63] (e.g. common to a wave) gives the same result.
-   With `table` (allowed when wallsunsun_table_ok(p)), the first five steps are read from
-   kLucasV31 instead: k = the bits of n above top - 5, and V_k, V_k+1 enter Montgomery
-   form as products with R^2, two products in place of ten. */
-WW_HD m2 wallsunsun_residue_w(u64 p, u64 q, m2 one, double pinv, int top, bool table)
+   when bit i is 1, and the square's operand is y iff bits i and i+1 of n are equal.
+   Leading zero bits leave the ladder's start (V_0, V_1) = (2, 3) unchanged, so any top in
+   [leading bit of n, 63] (e.g. common to a wave) gives the same result. The first `skip` steps (5 or 6; skip = wallsunsun_skip(n, top) or 5, or the smaller of
+   several lanes' values) are read from kLucasV31 instead: k = the bits of n above
+   top - skip, and V_k, V_k+1 (below 2^64) enter Montgomery form as products with R^2,
+   two products in place of 2 skip. */
+WW_HD m2 wallsunsun_residue_w(u64 p, u64 q, m2 one, double pinv, int top, int skip)
 {
-    u32 pm5 = mod5(p);
-    u64 n = ((pm5 == 2 || pm5 == 3) ? p + 1 : p - 1) >> 1;
+    u64 n = wallsunsun_n(p);
     u64 same = ~(n ^ (n >> 1)); /* bit i: bits i and i+1 of n are equal */
-    m2 two = m2_add(one, one, p);
-    m2 three = m2_add(two, one, p);
     /* R^2 mod p^2 is the Montgomery form of 2^64: one times 2^31, 2^31 and 2^2 */
     m2 r2 = m2_mulpow2(m2_mulpow2(m2_mulpow2(one, 31, p, pinv), 31, p, pinv), 2, p, pinv);
     m2 r2x2 = m2_add(r2, r2, p);
@@ -709,25 +722,21 @@ WW_HD m2 wallsunsun_residue_w(u64 p, u64 q, m2 one, double pinv, int top, bool t
     m2 minus3 = m2_neg(m2_add(r2x2, r2, p), p); /* -3 R^2 */
     u64 np = 0 - p;
     WW_LOOP_LOCAL(np); /* opaque, so that adding it is not turned back into subtracting p */
-    m2 x = three;                               /* k = 0: bit top+1 is 0, x holds V_1 */
-    m2 y = two;                                 /* and y holds V_0 */
-    int start = top;                            /* next bit of n */
-    if (table) {
-        /* the bits top..top-4 done: k < 32, and slot x holds V_k iff k is odd */
-        start = top - kWssTableBits;
-        const u32 k = (u32)(n >> (start + 1));
-        const m2 vk = m2_mul_digit(kLucasV31[k], r2, p, q);
-        const m2 vk1 = m2_mul_digit(kLucasV31[k + 1], r2, p, q);
-        const bool kodd = k & 1;
-        x.d0 = kodd ? vk.d0 : vk1.d0;
-        x.d1 = kodd ? vk.d1 : vk1.d1;
-        y.d0 = kodd ? vk1.d0 : vk.d0;
-        y.d1 = kodd ? vk1.d1 : vk.d1;
-    }
+    /* bits top .. start+1 done, k = n >> (start+1): slot x holds V_k iff k is odd */
+    const int start = top - skip; /* next bit of n */
+    const u32 k = (u32)(n >> (start + 1));
+    const m2 v0 = m2_mul_digit(kLucasV31[k], r2, p, q);
+    const m2 v1 = m2_mul_digit(kLucasV31[k + 1], r2, p, q);
+    const bool kodd = k & 1;
+    m2 x, y;
+    x.d0 = kodd ? v0.d0 : v1.d0;
+    x.d1 = kodd ? v0.d1 : v1.d1;
+    y.d0 = kodd ? v1.d0 : v0.d0;
+    y.d1 = kodd ? v1.d1 : v0.d1;
     /* The bits of `same` are read one 32-bit word at a time, high word first when start
        >= 32: the word is doubled each step and the carry out is the bit (one
        v_add_co_u32 on the device gives both). */
-    for (int half = start >= 32 ? 1 : 0; half >= 0 && start >= 0; --half) {
+    for (int half = start >= 32 ? 1 : 0; half >= 0; --half) {
         const int lead = half ? start - 32 : (start >= 32 ? 31 : start); /* first bit, in the word */
         u32 w = (half ? (u32)(same >> 32) : (u32)same) << (31 - lead);
         for (int j = lead; j >= 0; --j) {
@@ -756,15 +765,14 @@ WW_HD m2 wallsunsun_residue_w(u64 p, u64 q, m2 one, double pinv, int top, bool t
 /* Bit index of the leading bit of the Wall-Sun-Sun ladder's n for this p. */
 WW_HD int wallsunsun_top(u64 p)
 {
-    u32 pm5 = mod5(p);
-    u64 n = ((pm5 == 2 || pm5 == 3) ? p + 1 : p - 1) >> 1;
-    return 63 - clz64(n);
+    return 63 - clz64(wallsunsun_n(p));
 }
 
 WW_HD m2 wallsunsun_residue(u64 p, u64 q, m2 one)
 {
-    return wallsunsun_residue_w(p, q, one, 1.0 / (double)p, wallsunsun_top(p),
-                                wallsunsun_table_ok(p));
+    const int top = wallsunsun_top(p);
+    return wallsunsun_residue_w(p, q, one, 1.0 / (double)p, top,
+                                wallsunsun_skip(wallsunsun_n(p), top));
 }
 
 } // namespace ww

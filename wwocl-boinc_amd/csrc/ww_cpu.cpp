@@ -83,7 +83,8 @@ int ww_cpu_is_prp(uint64_t n)
     return no_factor_to_113(n) && strong_prp(2, n);
 }
 
-int ww_cpu_residues(const uint64_t *p, uint64_t n, int32_t top_w, int32_t top_f, uint64_t *out)
+int ww_cpu_residues(const uint64_t *p, uint64_t n, int32_t top_w, int32_t top_f,
+                    int32_t wss_skip, uint64_t *out)
 {
     if (n && (!p || !out)) return ww_set_error(WW_E_ARG, "null buffer");
     for (uint64_t i = 0; i < n; ++i) {
@@ -96,11 +97,16 @@ int ww_cpu_residues(const uint64_t *p, uint64_t n, int32_t top_w, int32_t top_f,
         if (tw < lw || tw > 63 || tf < lf || tf > 63)
             return ww_set_error(WW_E_ARG, "ladder bounds %d, %d outside [%d, 63], [%d, 63]", tw, tf,
                                 lw, lf);
+        const int most = wallsunsun_skip(wallsunsun_n(pi), tf);
+        const int skip = wss_skip < 0 ? most : wss_skip;
+        if (skip != 5 && skip != most)
+            return ww_set_error(WW_E_ARG, "wss_skip %d: this p and bound allow 5%s", skip,
+                                most == 6 ? " or 6" : "");
         const u64 q = inv64(pi);
         const m2 one = m2_one(pi);
         const double pinv = 1.0 / (double)pi;
         const m2 s = wieferich_residue_w(pi, q, one, pinv, tw);
-        const m2 f = wallsunsun_residue_w(pi, q, one, pinv, tf, wallsunsun_table_ok(pi));
+        const m2 f = wallsunsun_residue_w(pi, q, one, pinv, tf, skip);
         out[4 * i] = s.d0;
         out[4 * i + 1] = s.d1;
         out[4 * i + 2] = f.d0;

@@ -185,15 +185,15 @@ __device__ __forceinline__ u64 test_prime_wave(u64 p, bool act, ww_result *res, 
     const u64 pb = act ? p : 131; /* any prime > 5 for the bounds */
     const int top_w = wave_max(63 - clz64(pb >> 1));
     const int top_f = wave_max(wallsunsun_top(pb));
-    /* the Wall-Sun-Sun table start when every active lane allows it (a uniform branch) */
-    const bool table = __ballot(act && !wallsunsun_table_ok(p)) == 0;
+    /* six table steps when every active lane allows them, else five (wave-uniform) */
+    const int skip = __ballot(act && wallsunsun_skip(wallsunsun_n(p), top_f) < 6) ? 5 : 6;
     u64 v = 0;
     if (act) {
         const u64 q = inv64(p);
         const m2 one = m2_one(p);
         const double pinv = 1.0 / (double)p;
         const m2 s = wieferich_residue_w(p, q, one, pinv, top_w);
-        const m2 f = wallsunsun_residue_w(p, q, one, pinv, top_f, table);
+        const m2 f = wallsunsun_residue_w(p, q, one, pinv, top_f, skip);
         classify_w(p, s.d0, s.d1, res, cap, count, tag);
         classify_wss(p, f.d1, res, cap, count, tag);
         v = (s.d1 & 0xffffffffu) + (f.d1 & 0xffffffffu);

@@ -165,7 +165,7 @@ _SIGNATURES = [
     ("ww_cpu_is_prime", ctypes.c_int, [U64]),
     ("ww_cpu_is_prp", ctypes.c_int, [U64]),
     ("ww_cpu_residues", ctypes.c_int, [ctypes.POINTER(U64), U64, ctypes.c_int32,
-                                       ctypes.c_int32, ctypes.POINTER(U64)]),
+                                       ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(U64)]),
     ("ww_cpu_spsp2", ctypes.c_int, [U64, ctypes.c_int32, ctypes.c_int32]),
     ("ww_search_fused", ctypes.c_int, [ctypes.c_void_p, U64, U64]),
     ("ww_cpu_range", ctypes.c_int, [U64, U64, ctypes.POINTER(U64)]),
@@ -257,13 +257,14 @@ def presieve_table(above: int, bound: int):
     return [(buf[i].ell, buf[i].m, buf[i].bar_ell, buf[i].bar_m) for i in range(n)]
 
 
-def cpu_residues(ps, top_w: int = -1, top_f: int = -1):
+def cpu_residues(ps, top_w: int = -1, top_f: int = -1, wss_skip: int = -1):
     """Host build of the device ladders: [(s0, s1, c20, c21)] per p, from the given
-    leading-bit bounds (-1 = each p's own)."""
+    leading-bit bounds (-1 = each p's own) and Wall-Sun-Sun table steps (5 or 6, -1 = the
+    most each p allows)."""
     n = len(ps)
     pin = (U64 * max(n, 1))(*ps)
     out = (U64 * (4 * max(n, 1)))()
-    _check(load().ww_cpu_residues(pin, n, top_w, top_f, out))
+    _check(load().ww_cpu_residues(pin, n, top_w, top_f, wss_skip, out))
     return [tuple(out[4 * i:4 * i + 4]) for i in range(n)]
 
 
