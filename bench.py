@@ -31,14 +31,15 @@ CHUNK = 1 << 32
 BITS = 64  # every p in the range has bit length 64
 # HBM bytes per list entry moved by k_ww_tests: FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 per
 # launch (factors from a 1 GiB copy kernel) over the entries of the launch, measured with
-# rocprofv3 --pmc on MI355X for this workload. Algorithmic: the 8-byte list read.
-TRAFFIC_BYTES_PER_ENTRY = 8.00
+# rocprofv3 --pmc on MI355X for this workload: 806.0 MB fetched + 17.5 MB written per 2^32
+# chunk (the 8-byte list read, register spills and the result atomics).
+TRAFFIC_BYTES_PER_ENTRY = 8.37
 # k_batch keeps candidates and PRPs on chip (LDS queues). Its algorithmic HBM traffic is the
 # wide pre-sieve's 64 words (512 B) per 8192-integer segment, read once; the rest is the
 # deep-sieve table, span lookups and the cursor/slot atomics (each atomic reads as a 32-byte
 # write). Measured the same way at the defaults (deep bound 2048, pre-sieve 2^20):
-# 578.1 MB fetched + 19.1 MB written per step.
-BATCH_TRAFFIC_BYTES_PER_PRP = 3.036
+# 576.7 MB fetched + 19.0 MB written per step.
+BATCH_TRAFFIC_BYTES_PER_PRP = 3.029
 PRESIEVE_BYTES_PER_SEGMENT = 512
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
 
