@@ -640,31 +640,39 @@ WW_HD m2 m2_div5(m2 x, u64 p)
     return r;
 }
 
-/* 2^((p-1)/2) mod p^2 as digits (s0, s1) (wieferich.cl:219-245, wwcpu.cpp:253-284).
-   The exponent's bits from `top` (any bound >= its leading bit, e.g. common to a wave)
-   down to 0 are taken in windows of 5: five Montgomery squares, then one exact
-   multiplication by 2^v, instead of a square and a conditional doubling per bit. */
-WW_HD m2 wieferich_residue_w(u64 p, u64 q, m2 one, double pinv, int top)
+/* R^2 mod p^2 (R = 2^64), the Montgomery form of 2^64: one times 2^31, 2^31 and 2^2. */
+WW_HD m2 m2_r2(m2 one, u64 p, double pinv)
+{
+    return m2_mulpow2(m2_mulpow2(m2_mulpow2(one, 31, p, pinv), 31, p, pinv), 2, p, pinv);
+}
+
+/* 2^((p-1)/2) mod p^2 as digits (s0, s1) (wieferich.cl:219-245, wwcpu.cpp:253-284), with
+   r2 = R^2 mod p^2. The exponent's six bits from `top` (any bound >= its leading bit and
+   >= 5, e.g. common to a wave) give K < 64 and the start 2^K as one digit-times-R^2
+   product. The remaining top - 5 bits go in windows of Montgomery squares followed by one
+   exact multiplication by 2^v (v < 32) instead of a square and a conditional doubling per
+   bit: first (top - 5) mod 5 bits if that is not 0, then windows of 5. */
+WW_HD m2 wieferich_residue_w(u64 p, u64 q, m2 r2, double pinv, int top)
 {
     const int W = 5;
     u64 e = p >> 1;
-    int len = top + 1;
-    int first = len - ((len - 1) / W) * W; /* 1..W bits */
-    int pos = top - first + 1;
+    int pos = top - 5; /* bits below the start */
     u64 np = 0 - p;
     WW_LOOP_LOCAL(np); /* opaque, so that adding it is not turned back into subtracting p */
-    m2 a = m2_mulpow2(one, (u32)(e >> pos) & ((1u << first) - 1u), p, pinv);
+    m2 a = m2_mul_digit(1ull << (u32)(e >> pos), r2, p, q);
     while (pos > 0) {
-        pos -= W;
-        for (int k = 0; k < W; ++k) a = m2_sqr(a, p, q, np);
-        a = m2_mulpow2(a, (u32)(e >> pos) & 31u, p, pinv);
+        const int w = pos % W ? pos % W : W; /* uniform: the short window comes first */
+        pos -= w;
+        for (int k = 0; k < w; ++k) a = m2_sqr(a, p, q, np);
+        a = m2_mulpow2(a, (u32)(e >> pos) & ((1u << w) - 1u), p, pinv);
     }
     return m2_get(a, p, q);
 }
 
 WW_HD m2 wieferich_residue(u64 p, u64 q, m2 one)
 {
-    return wieferich_residue_w(p, q, one, 1.0 / (double)p, 63 - clz64(p >> 1));
+    const double pinv = 1.0 / (double)p;
+    return wieferich_residue_w(p, q, m2_r2(one, p, pinv), pinv, 63 - clz64(p >> 1));
 }
 
 /* V_k(3, 1) = L_2k for k = 0..46: V_0 = 2, V_1 = 3, V_k+1 = 3 V_k - V_k-1. V_46 = L_92 is
@@ -711,12 +719,10 @@ WW_HD int wallsunsun_skip(u64 n, int top)
    several lanes' values) are read from kLucasV31 instead: k = the bits of n above
    top - skip, and V_k, V_k+1 (below 2^64) enter Montgomery form as products with R^2,
    two products in place of 2 skip. */
-WW_HD m2 wallsunsun_residue_w(u64 p, u64 q, m2 one, double pinv, int top, int skip)
+WW_HD m2 wallsunsun_residue_w(u64 p, u64 q, m2 r2, int top, int skip)
 {
     u64 n = wallsunsun_n(p);
     u64 same = ~(n ^ (n >> 1)); /* bit i: bits i and i+1 of n are equal */
-    /* R^2 mod p^2 is the Montgomery form of 2^64: one times 2^31, 2^31 and 2^2 */
-    m2 r2 = m2_mulpow2(m2_mulpow2(m2_mulpow2(one, 31, p, pinv), 31, p, pinv), 2, p, pinv);
     m2 r2x2 = m2_add(r2, r2, p);
     m2 minus2 = m2_neg(r2x2, p);                /* -2 R^2 */
     m2 minus3 = m2_neg(m2_add(r2x2, r2, p), p); /* -3 R^2 */
@@ -771,7 +777,7 @@ WW_HD int wallsunsun_top(u64 p)
 WW_HD m2 wallsunsun_residue(u64 p, u64 q, m2 one)
 {
     const int top = wallsunsun_top(p);
-    return wallsunsun_residue_w(p, q, one, 1.0 / (double)p, top,
+    return wallsunsun_residue_w(p, q, m2_r2(one, p, 1.0 / (double)p), top,
                                 wallsunsun_skip(wallsunsun_n(p), top));
 }
 

@@ -103,10 +103,10 @@ int ww_cpu_residues(const uint64_t *p, uint64_t n, int32_t top_w, int32_t top_f,
             return ww_set_error(WW_E_ARG, "wss_skip %d: this p and bound allow 5%s", skip,
                                 most == 6 ? " or 6" : "");
         const u64 q = inv64(pi);
-        const m2 one = m2_one(pi);
         const double pinv = 1.0 / (double)pi;
-        const m2 s = wieferich_residue_w(pi, q, one, pinv, tw);
-        const m2 f = wallsunsun_residue_w(pi, q, one, pinv, tf, skip);
+        const m2 r2 = m2_r2(m2_one(pi), pi, pinv);
+        const m2 s = wieferich_residue_w(pi, q, r2, pinv, tw);
+        const m2 f = wallsunsun_residue_w(pi, q, r2, tf, skip);
         out[4 * i] = s.d0;
         out[4 * i + 1] = s.d1;
         out[4 * i + 2] = f.d0;

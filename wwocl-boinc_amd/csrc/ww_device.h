@@ -190,10 +190,10 @@ __device__ __forceinline__ u64 test_prime_wave(u64 p, bool act, ww_result *res, 
     u64 v = 0;
     if (act) {
         const u64 q = inv64(p);
-        const m2 one = m2_one(p);
         const double pinv = 1.0 / (double)p;
-        const m2 s = wieferich_residue_w(p, q, one, pinv, top_w);
-        const m2 f = wallsunsun_residue_w(p, q, one, pinv, top_f, skip);
+        const m2 r2 = m2_r2(m2_one(p), p, pinv);
+        const m2 s = wieferich_residue_w(p, q, r2, pinv, top_w);
+        const m2 f = wallsunsun_residue_w(p, q, r2, top_f, skip);
         classify_w(p, s.d0, s.d1, res, cap, count, tag);
         classify_wss(p, f.d1, res, cap, count, tag);
         v = (s.d1 & 0xffffffffu) + (f.d1 & 0xffffffffu);
