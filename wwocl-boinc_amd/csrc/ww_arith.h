@@ -86,6 +86,21 @@ WW_HD u64 mul_hi(u64 a, u64 b)
 
 /* a^2 -> 128: the cross product a0*a1 is formed once and doubled by feeding it
    to two accumulating MADs, so the result needs no 65-bit intermediate. */
+/* High word of a*b when its low word lo is already known, as in every Montgomery
+   reduction here: u = t_lo * p^-1 makes the low word of u*p equal t_lo. With
+   a*b = a1 b1 2^64 + (a1 b0 + a0 b1) 2^32 + a0 b0, the middle column plus the high half h
+   of a0 b0 has its low 32 bits fixed to hi32(lo), so its carry into the high word is
+   k = ceil((a1 b0 + a0 b1 - hi32(lo)) / 2^32) = (a1 b0 + a0 b1 + ~hi32(lo)) >> 32 for every
+   h in [0, 2^32). Three 32-bit products instead of four, and no partial sum overflows:
+   a1 b0 + ~hi32(lo) <= 2^64 - 2^32, a0 b1 + 2^32 - 1 < 2^64, a1 b1 + 2 (2^32 - 1) < 2^64. */
+WW_HD u64 mul_hi_known(u64 a, u64 b, u64 lo)
+{
+    u32 a0 = (u32)a, a1 = (u32)(a >> 32), b0 = (u32)b, b1 = (u32)(b >> 32);
+    u64 s = mad32(a1, b0, (u64)(~(u32)(lo >> 32)));
+    u64 t = mad32(a0, b1, (u32)s);
+    return mad32(a1, b1, s >> 32) + (t >> 32);
+}
+
 WW_HD u128 sqr_wide(u64 a)
 {
     u32 a0 = (u32)a, a1 = (u32)(a >> 32);
@@ -289,7 +304,7 @@ WW_HD u64 mont_sqr(u64 a, u64 n, u64 q)
 {
     u128 t = sqr_wide(a);
     u64 m = t.lo * q;
-    u64 mp = mul_hi(m, n);
+    u64 mp = mul_hi_known(m, n, t.lo);
     bool b;
     u64 r = sub_b(t.hi, mp, &b);
     return b ? r + n : r;
@@ -299,7 +314,7 @@ WW_HD u64 mont_mul(u64 a, u64 b, u64 n, u64 q)
 {
     u128 t = mul_wide(a, b);
     u64 m = t.lo * q;
-    u64 mp = mul_hi(m, n);
+    u64 mp = mul_hi_known(m, n, t.lo);
     u64 r = t.hi - mp;
     return t.hi < mp ? r + n : r;
 }
@@ -465,7 +480,7 @@ WW_HD m2 m2_neg(m2 x, u64 p)
 WW_HD m2 m2_redc(u128 t, u64 c_lo, u64 c_hi, u64 c_top, u64 p, u64 q)
 {
     u64 u0 = t.lo * q;
-    u64 v1 = mul_hi(p, u0);
+    u64 v1 = mul_hi_known(p, u0, t.lo);
     /* tp = c + u0 */
     u64 tlo = c_lo + u0;
     u64 cy = tlo < u0 ? 1 : 0;
@@ -473,7 +488,7 @@ WW_HD m2 m2_redc(u128 t, u64 c_lo, u64 c_hi, u64 c_top, u64 p, u64 q)
     u64 top = c_top | ((thi < cy) ? 1 : 0);
     u64 t1p = (top | (thi >= p ? 1 : 0)) ? thi - p : thi; /* tp >> 64 < 2p */
     u64 up0 = tlo * q;
-    u64 v1p = mul_hi(p, up0);
+    u64 v1p = mul_hi_known(p, up0, tlo);
     bool b = t.hi < v1;
     m2 r;
     r.d0 = t.hi - v1 + (b ? p : 0);
@@ -489,10 +504,10 @@ WW_HD m2 m2_sqr(m2 x, u64 p, u64 q, u64 np)
 {
     u128 t = sqr_wide(x.d0);
     u64 u0 = t.lo * q;
-    u64 v1 = mul_hi(p, u0);
+    u64 v1 = mul_hi_known(p, u0, t.lo);
     u128 c = mul_wide_add(x.d0, dbl_mod(x.d1, p, np), u0); /* high word < p */
     u64 up0 = c.lo * q;
-    u64 v1p = mul_hi(p, up0);
+    u64 v1p = mul_hi_known(p, up0, c.lo);
     bool b;
     u64 z0 = sub_b(t.hi, v1, &b);
     m2 r;
@@ -524,10 +539,10 @@ WW_HD m2 m2_sqr_add(m2 x, m2 k, u64 p, u64 q, u64 np)
 {
     u128 t = sqr_wide_add(x.d0, k.d0);
     u64 u0 = t.lo * q;
-    u64 v1 = mul_hi(p, u0);
+    u64 v1 = mul_hi_known(p, u0, t.lo);
     u128 c = mul_wide_add2(x.d0, dbl_mod(x.d1, p, np), u0, k.d1);
     u64 up0 = c.lo * q;
-    u64 v1p = mul_hi(p, up0);
+    u64 v1p = mul_hi_known(p, up0, c.lo);
     bool b;
     u64 z0 = sub_b(t.hi, v1, &b);
     m2 r;
@@ -544,10 +559,10 @@ WW_HD m2 m2_mul_digit(u64 v, m2 y, u64 p, u64 q)
 {
     u128 t = mul_wide(v, y.d0);
     u64 u0 = t.lo * q;
-    u64 v1 = mul_hi(p, u0);
+    u64 v1 = mul_hi_known(p, u0, t.lo);
     u128 c = mul_wide_add(v, y.d1, u0);
     u64 up0 = c.lo * q;
-    u64 v1p = mul_hi(p, up0);
+    u64 v1p = mul_hi_known(p, up0, c.lo);
     bool b;
     u64 z0 = sub_b(t.hi, v1, &b);
     m2 r;
@@ -563,7 +578,7 @@ WW_HD m2 m2_mul_add(m2 x, m2 y, m2 k, u64 p, u64 q, u64 np)
 {
     u128 t = mul_wide_add(x.d0, y.d0, k.d0);
     u64 u0 = t.lo * q;
-    u64 v1 = mul_hi(p, u0);
+    u64 v1 = mul_hi_known(p, u0, t.lo);
     u128 a = mul_wide_add(x.d0, y.d1, u0);
     u128 e = mul_wide_add(x.d1, y.d0, k.d1);
     bool top;
@@ -571,7 +586,7 @@ WW_HD m2 m2_mul_add(m2 x, m2 y, m2 k, u64 p, u64 q, u64 np)
     bool big = top | (sum.hi >= p);
     u64 t1p = sum.hi + (big ? np : 0); /* minus p, modulo 2^64 */
     u64 up0 = sum.lo * q;
-    u64 v1p = mul_hi(p, up0);
+    u64 v1p = mul_hi_known(p, up0, sum.lo);
     bool b;
     u64 z0 = sub_b(t.hi, v1, &b);
     m2 r;
@@ -595,11 +610,11 @@ WW_HD m2 m2_mulpow2(m2 x, u32 v, u64 p, double pinv)
 WW_HD m2 m2_get(m2 x, u64 p, u64 q)
 {
     u64 u0 = q * x.d0;
-    u64 v1 = mul_hi(p, u0);
+    u64 v1 = mul_hi_known(p, u0, x.d0);
     u64 tp = x.d1 + u0;
     u64 up0 = q * tp;
     u64 t1p = tp < x.d1 ? 1 : 0;
-    u64 v1p = mul_hi(p, up0);
+    u64 v1p = mul_hi_known(p, up0, tp);
     bool b = v1 != 0;
     m2 r;
     r.d0 = b ? p - v1 : 0;
