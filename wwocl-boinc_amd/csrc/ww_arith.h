@@ -84,8 +84,6 @@ WW_HD u64 mul_hi(u64 a, u64 b)
     return mad32(a1, b1, (t1 >> 32) + (t2 >> 32));
 }
 
-/* a^2 -> 128: the cross product a0*a1 is formed once and doubled by feeding it
-   to two accumulating MADs, so the result needs no 65-bit intermediate. */
 /* High word of a*b when its low word lo is already known, as in every Montgomery
    reduction here: u = t_lo * p^-1 makes the low word of u*p equal t_lo. With
    a*b = a1 b1 2^64 + (a1 b0 + a0 b1) 2^32 + a0 b0, the middle column plus the high half h
@@ -101,6 +99,8 @@ WW_HD u64 mul_hi_known(u64 a, u64 b, u64 lo)
     return mad32(a1, b1, s >> 32) + (t >> 32);
 }
 
+/* a^2 -> 128: the cross product a0*a1 is formed by two accumulating MADs, which
+   doubles it without a 65-bit intermediate. */
 WW_HD u128 sqr_wide(u64 a)
 {
     u32 a0 = (u32)a, a1 = (u32)(a >> 32);
