@@ -364,12 +364,13 @@ WW_HD u64 mulpow2_mod(u64 y, u32 v, u64 n, double ninv)
 /* Base-2 strong probable-prime test of odd n > 1 (getsegprimes.cl:118-164,
    prime.h:66-112), q = n^-1 mod 2^64, one = 2^64 mod n.
    With n - 1 = d 2^t the test asks for x_0 = 2^d = 1 or x_s = x_0^(2^s) = -1 for some
-   s < t. The bits of n - 1 from `top` down to `tmax` are taken in windows of 5 (five
-   squarings, then one multiplication by 2^v), which yields 2^((n-1) >> tmax); the bits
-   below tmax follow one by one, and x_0 .. x_{t-1} appear there (or x_0 at the end of
-   the windows when t = tmax). Any top >= the leading bit of n - 1 and any tmax in
-   [t, top] give the same answer, so a wave can pass common (uniform) values and run one
-   loop for all lanes. */
+   s < t. The bits of n - 1 from `top` down to `tmax` yield 2^((n-1) >> tmax): the first
+   five (or all, if fewer) as one multiplication of 1 by 2^v, then the rest in windows of
+   squarings followed by one multiplication by 2^v, a short window of (len - 5) mod 5 bits
+   first if that is not 0, then windows of 5. The bits below tmax follow one by one, and
+   x_0 .. x_{t-1} appear there (or x_0 at the end of the windows when t = tmax). Any
+   top >= the leading bit of n - 1 and any tmax in [t, top] give the same answer, so a
+   wave can pass common (uniform) values and run one loop for all lanes. */
 WW_HD bool spsp2_window(u64 n, u64 q, u64 one, int top, int tmax)
 {
     const int W = 5;
@@ -378,13 +379,14 @@ WW_HD bool spsp2_window(u64 n, u64 q, u64 one, int top, int tmax)
     double ninv = 1.0 / (double)n;
     u64 mone = n - one;
     int len = top - tmax + 1;
-    int first = len - ((len - 1) / W) * W; /* 1..W bits */
+    int first = len < W ? len : W;
     int pos = top - first + 1;
     u64 a = mulpow2_mod(one, (u32)(e >> pos) & ((1u << first) - 1u), n, ninv);
     while (pos > tmax) {
-        pos -= W;
-        for (int k = 0; k < W; ++k) a = mont_sqr(a, n, q);
-        a = mulpow2_mod(a, (u32)(e >> pos) & 31u, n, ninv);
+        const int w = (pos - tmax) % W ? (pos - tmax) % W : W; /* uniform: short one first */
+        pos -= w;
+        for (int k = 0; k < w; ++k) a = mont_sqr(a, n, q);
+        a = mulpow2_mod(a, (u32)(e >> pos) & ((1u << w) - 1u), n, ninv);
     }
     bool ok = (t == tmax) & ((a == one) | (a == mone));
     for (int i = tmax - 1; i >= 1; --i) {
