@@ -38,8 +38,8 @@ TRAFFIC_BYTES_PER_ENTRY = 8.37
 # wide pre-sieve's 64 words (512 B) per 8192-integer segment, read once; the rest is the
 # deep-sieve table, span lookups and the cursor/slot atomics (each atomic reads as a 32-byte
 # write). Measured the same way at the defaults (deep bound 2048, pre-sieve 2^20):
-# 576.7 MB fetched + 19.0 MB written per step.
-BATCH_TRAFFIC_BYTES_PER_PRP = 3.029
+# 575.8 MB fetched + 19.2 MB written per step.
+BATCH_TRAFFIC_BYTES_PER_PRP = 3.025
 PRESIEVE_BYTES_PER_SEGMENT = 512
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
 
