@@ -29,14 +29,6 @@ constexpr int kQueueCap = 128; /* < 64 left over + <= 64 appended per PRP round 
 constexpr int kBatchCandCap = kWaveCandCap + 64;
 static_assert(kBatchCandCap - 256 >= 64, "rescue bitmap must not reach the leftovers");
 
-/* Wave-uniform values as scalars (the compiler cannot see that a lane-0 broadcast is
-   uniform, and would keep them in vector registers). */
-__device__ __forceinline__ u32 uniform32(u32 v) { return __builtin_amdgcn_readfirstlane(v); }
-__device__ __forceinline__ u64 uniform64(u64 v)
-{
-    return ((u64)uniform32((u32)(v >> 32)) << 32) | uniform32((u32)v);
-}
-
 __device__ __forceinline__ void wave_sync_lds()
 {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -230,43 +222,10 @@ __attribute__((amdgpu_waves_per_eu(WW_BATCH_WAVES_PER_EU))) void k_batch(const B
             wave_sync_lds();
             bits &= ~deep_sieve(bm, rm, g_new, deep, ndeep, lane);
         }
-        /* Candidates go into the buffer grouped by t, the trailing-zero count of n - 1
-           (t = 1, 2, 3, >= 4), so that most PRP rounds hold one class and their
-           bit-by-bit tail below the wave's largest t stays short. For the odd number
-           G + 2i (G = g_new, i = 64 lane + bit), n - 1 = 2 (g + i) with g = (G - 1) / 2,
-           so t = 1 + ctz(g + i) depends only on (g + i) mod 8 (64 lane = 0 mod 8): the same
-           four bit masks for every lane of the segment. */
-        const u32 g8 = (u32)((g_new - 1) >> 1) & 7u;
-        u64 cls[4];
-        cls[0] = 0x5555555555555555ull << ((1u - g8) & 1u);  /* (g + i) odd: t = 1 */
-        cls[1] = 0x1111111111111111ull << ((2u - g8) & 3u);  /* = 2 mod 4: t = 2 */
-        cls[2] = 0x0101010101010101ull << ((4u - g8) & 7u);  /* = 4 mod 8: t = 3 */
-        cls[3] = 0x0101010101010101ull << ((0u - g8) & 7u);  /* = 0 mod 8: t >= 4 */
-        /* per-class counts packed in 16-bit fields (each total <= 4096), one scan */
-        u64 cnt = 0;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) cnt |= (u64)__popcll(bits & cls[k]) << (16 * k);
-        u64 incl = cnt;
-        for (int d = 1; d < 64; d <<= 1) {
-            u64 v = __shfl_up(incl, d, 64);
-            if (lane >= (u32)d) incl += v;
-        }
-        const u64 tot = uniform64(__shfl(incl, 63, 64));
+        /* compacted behind the leftovers, grouped by the trailing zeros of n - 1 */
         kb = rem;
         j0 = 0;
-        u32 base = rem;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            u32 pos = base + (u32)(((incl - cnt) >> (16 * k)) & 0xffffu);
-            u64 b = bits & cls[k];
-            while (b) {
-                u32 bit = __builtin_ctzll(b);
-                wc[pos++] = (uint16_t)((1u << 12) | (lane * 64 + bit));
-                b &= b - 1;
-            }
-            base += (u32)((tot >> (16 * k)) & 0xffffu);
-        }
-        ncand = base;
+        ncand = rem + compact_by_class(bits, g_new, lane, wc, rem, 1u << 12);
         wave_sync_lds();
     }
     acc_flush(acc, sums, lane);

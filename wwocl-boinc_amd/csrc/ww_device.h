@@ -89,6 +89,54 @@ __device__ __forceinline__ int wave_max(int x)
     return __builtin_amdgcn_readfirstlane(x);
 }
 
+/* Wave-uniform values as scalars (the compiler cannot see that a lane-0 broadcast is
+   uniform, and would keep them in vector registers). */
+__device__ __forceinline__ u32 uniform32(u32 v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ u64 uniform64(u64 v)
+{
+    return ((u64)uniform32((u32)(v >> 32)) << 32) | uniform32((u32)v);
+}
+
+/* Wave compaction of a segment's candidates: bit b of lane L's word stands for the odd
+   number G + 2 i, i = 64 L + b, and wc[base ...] receives tag | i for every set bit. The
+   entries are grouped by t, the trailing-zero count of n - 1 (t = 1, 2, 3, >= 4), so that
+   most strong PRP rounds hold one group and their bit-by-bit tail below the wave's largest
+   t stays short. n - 1 = 2 (g + i) with g = (G - 1) / 2, so t = 1 + ctz(g + i) depends only
+   on (g + i) mod 8 (64 L = 0 mod 8): four bit masks shared by every lane. The lanes scan
+   their four counts at once, in 16-bit fields (a segment holds at most 4096). Returns the
+   number of entries written, wave-uniform. Called by all 64 lanes. */
+__device__ __forceinline__ u32 compact_by_class(u64 bits, u64 G, u32 lane, uint16_t *wc,
+                                                u32 base, u32 tag)
+{
+    const u32 g8 = (u32)((G - 1) >> 1) & 7u;
+    u64 cls[4];
+    cls[0] = 0x5555555555555555ull << ((1u - g8) & 1u); /* g + i odd: t = 1 */
+    cls[1] = 0x1111111111111111ull << ((2u - g8) & 3u); /* = 2 mod 4: t = 2 */
+    cls[2] = 0x0101010101010101ull << ((4u - g8) & 7u); /* = 4 mod 8: t = 3 */
+    cls[3] = 0x0101010101010101ull << ((0u - g8) & 7u); /* = 0 mod 8: t >= 4 */
+    u64 cnt = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) cnt |= (u64)__popcll(bits & cls[k]) << (16 * k);
+    u64 incl = cnt;
+    for (int d = 1; d < 64; d <<= 1) {
+        u64 v = __shfl_up(incl, d, 64);
+        if (lane >= (u32)d) incl += v;
+    }
+    const u64 tot = uniform64(__shfl(incl, 63, 64));
+    u32 at = base;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        u32 pos = at + (u32)(((incl - cnt) >> (16 * k)) & 0xffffu);
+        u64 b = bits & cls[k];
+        while (b) {
+            wc[pos++] = (uint16_t)(tag | (lane * 64 + (u32)__builtin_ctzll(b)));
+            b &= b - 1;
+        }
+        at += (u32)((tot >> (16 * k)) & 0xffffu);
+    }
+    return at - base;
+}
+
 /* Base-2 strong PRP of one candidate per lane (act false: the lane only takes part in
    the wave's common loop bounds, the largest leading bit and trailing-zero count of
    n - 1, so that every lane runs the same loops). Called by all 64 lanes. */

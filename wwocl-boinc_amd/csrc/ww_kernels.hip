@@ -51,21 +51,10 @@ __global__ __launch_bounds__(kSieveThreads) void k_sieve_prp(u64 base, u64 hi, u
         bits &= ~deep_sieve(bm, rm, wave_base, deep, ndeep, lane);
     }
 
-    /* ---- wave-local compaction of candidate positions into LDS ---- */
-    u32 cnt = __popcll(bits);
-    u32 incl = cnt;
-    for (int d = 1; d < 64; d <<= 1) {
-        u32 v = __shfl_up(incl, d, 64);
-        if (lane >= (u32)d) incl += v;
-    }
-    const u32 ncand = __shfl(incl, 63, 64);
-    u32 pos = incl - cnt;
+    /* ---- wave-local compaction of candidate positions into LDS, grouped by the
+       trailing zeros of n - 1 ---- */
     uint16_t *wc = cand + wave * kWaveCandCap;
-    while (bits) {
-        u32 b = __ffsll((unsigned long long)bits) - 1;
-        wc[pos++] = (uint16_t)(lane * 64 + b);
-        bits &= bits - 1;
-    }
+    const u32 ncand = compact_by_class(bits, wave_base, lane, wc, 0, 0);
     __syncthreads();
 
     /* ---- strong PRP over the compacted candidates, survivors compacted in place ---- */
