@@ -139,6 +139,7 @@ __attribute__((amdgpu_waves_per_eu(WW_BATCH_WAVES_PER_EU))) void k_batch(const B
     u64 g_old = 0, g_new = 0;
     u64 seg_next = 0, seg_end = 0; /* segments claimed but not yet sieved */
     u64 seen = 0;                  /* cursor after this wave's last claim */
+    bool desc = true;              /* group order of the newest segment's candidates */
     for (;;) {
         if (qn >= 64 || (done && qn > 0)) {
             tests_round(qn >= 64 ? 64 : qn, qn, wq, ws, lane, acc, sums, res, res_cap,
@@ -222,10 +223,12 @@ __attribute__((amdgpu_waves_per_eu(WW_BATCH_WAVES_PER_EU))) void k_batch(const B
             wave_sync_lds();
             bits &= ~deep_sieve(bm, rm, g_new, deep, ndeep, lane);
         }
-        /* compacted behind the leftovers, grouped by the trailing zeros of n - 1 */
+        /* compacted behind the leftovers, grouped by the trailing zeros of n - 1, in the
+           opposite order to the previous segment's, whose last group the leftovers are */
         kb = rem;
         j0 = 0;
-        ncand = rem + compact_by_class(bits, g_new, lane, wc, rem, 1u << 12);
+        desc = !desc;
+        ncand = rem + compact_by_class(bits, g_new, lane, wc, rem, 1u << 12, desc);
         wave_sync_lds();
     }
     acc_flush(acc, sums, lane);

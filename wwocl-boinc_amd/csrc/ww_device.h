@@ -103,10 +103,13 @@ __device__ __forceinline__ u64 uniform64(u64 v)
    most strong PRP rounds hold one group and their bit-by-bit tail below the wave's largest
    t stays short. n - 1 = 2 (g + i) with g = (G - 1) / 2, so t = 1 + ctz(g + i) depends only
    on (g + i) mod 8 (64 L = 0 mod 8): four bit masks shared by every lane. The lanes scan
-   their four counts at once, in 16-bit fields (a segment holds at most 4096). Returns the
-   number of entries written, wave-uniform. Called by all 64 lanes. */
+   their four counts at once, in 16-bit fields (a segment holds at most 4096). The groups
+   go in ascending t, or descending when `desc` (wave-uniform): alternating it between
+   consecutive segments makes a round that mixes one segment's last entries with the next
+   one's first stay within one group. Returns the number of entries written, wave-uniform.
+   Called by all 64 lanes. */
 __device__ __forceinline__ u32 compact_by_class(u64 bits, u64 G, u32 lane, uint16_t *wc,
-                                                u32 base, u32 tag)
+                                                u32 base, u32 tag, bool desc = false)
 {
     const u32 g8 = (u32)((G - 1) >> 1) & 7u;
     u64 cls[4];
@@ -125,7 +128,8 @@ __device__ __forceinline__ u32 compact_by_class(u64 bits, u64 G, u32 lane, uint1
     const u64 tot = uniform64(__shfl(incl, 63, 64));
     u32 at = base;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
+    for (int kk = 0; kk < 4; ++kk) {
+        const int k = desc ? 3 - kk : kk;
         u32 pos = at + (u32)(((incl - cnt) >> (16 * k)) & 0xffffu);
         u64 b = bits & cls[k];
         while (b) {
