@@ -241,3 +241,25 @@ def test_random_ranges_all_forms_vs_oracle(gctx):
         assert (plain.prp_count, plain.checksum, plain.results) == want, (lo, hi)
         assert (fused.prp_count, fused.checksum, fused.results) == want, (lo, hi)
         assert bat == want, (lo, hi)
+
+
+@pytest.mark.parametrize("top", [62, 40, 20])
+def test_wss_table_start_switch_vs_oracle(gctx, top):
+    """Around the p where the six leading bits of the Wall-Sun-Sun ladder's exponent
+    n = (p - (p/5)) / 2 go from k = 45 (six table steps allowed) to k = 46 (five only):
+    waves there hold lanes of both kinds and take five steps for all of them. Residues,
+    the two-kernel form and a persistent batch against the oracle."""
+    p_switch = 46 << (top - 4)  # n = 46 * 2^(top-5), p = 2n +- 1
+    lo, hi = p_switch - 20000, p_switch + 20000
+    ps = oracle.prp_list(lo, hi)
+    ks = {(((p + 1 if p % 5 in (2, 3) else p - 1) // 2) >> (top - 5)) for p in ps}
+    assert {45, 46} <= ks, ks
+    got = gctx.residues(ps)
+    for p, g in zip(ps, got):
+        assert g == tuple(oracle.wieferich(p)) + tuple(oracle.wallsunsun(p)), p
+    want = oracle.range_results(lo, hi)
+    plain = gctx.run_range(lo, hi)
+    assert (plain.prp_count, plain.checksum, plain.results) == want
+    with wwlib.Batch(gctx, [(lo, hi)]) as b:
+        b.launch()
+        assert b.read() == [want]
