@@ -74,6 +74,26 @@ WW_HD u128 mul_wide(u64 a, u64 b)
     return r;
 }
 
+/* Low 64 bits of a*b. On the device as three v_mad_u64_u32: a0*b0, then the two cross
+   products accumulated into its high word. Only the low words of the cross MADs matter,
+   and they are emitted as written (the backend would otherwise narrow them to two
+   v_mul_lo_u32 and a v_add3_u32, four VOP3 operations where three MADs do). */
+WW_HD u64 mul_lo64(u64 a, u64 b)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    u32 a0 = (u32)a, a1 = (u32)(a >> 32), b0 = (u32)b, b1 = (u32)(b >> 32);
+    u64 t = (u64)a0 * b0;
+    u64 s, r, c1, c2;
+    asm("v_mad_u64_u32 %0, %1, %2, %3, %4" : "=v"(s), "=s"(c1) : "v"(a0), "v"(b1), "v"(t >> 32));
+    asm("v_mad_u64_u32 %0, %1, %2, %3, %4" : "=v"(r), "=s"(c2) : "v"(a1), "v"(b0), "v"(s));
+    (void)c1;
+    (void)c2;
+    return ((u64)(u32)r << 32) | (u32)t;
+#else
+    return a * b;
+#endif
+}
+
 /* High 64 bits of a*b. */
 WW_HD u64 mul_hi(u64 a, u64 b)
 {
@@ -303,7 +323,7 @@ WW_HD u64 mont_one(u64 n)
 WW_HD u64 mont_sqr(u64 a, u64 n, u64 q)
 {
     u128 t = sqr_wide(a);
-    u64 m = t.lo * q;
+    u64 m = mul_lo64(t.lo, q);
     u64 mp = mul_hi_known(m, n, t.lo);
     bool b;
     u64 r = sub_b(t.hi, mp, &b);
@@ -313,7 +333,7 @@ WW_HD u64 mont_sqr(u64 a, u64 n, u64 q)
 WW_HD u64 mont_mul(u64 a, u64 b, u64 n, u64 q)
 {
     u128 t = mul_wide(a, b);
-    u64 m = t.lo * q;
+    u64 m = mul_lo64(t.lo, q);
     u64 mp = mul_hi_known(m, n, t.lo);
     u64 r = t.hi - mp;
     return t.hi < mp ? r + n : r;
@@ -481,7 +501,7 @@ WW_HD m2 m2_neg(m2 x, u64 p)
    (x*y)/2^64 mod p^2 (wwcpu.cpp:143-166, 213-236). */
 WW_HD m2 m2_redc(u128 t, u64 c_lo, u64 c_hi, u64 c_top, u64 p, u64 q)
 {
-    u64 u0 = t.lo * q;
+    u64 u0 = mul_lo64(t.lo, q);
     u64 v1 = mul_hi_known(p, u0, t.lo);
     /* tp = c + u0 */
     u64 tlo = c_lo + u0;
@@ -489,7 +509,7 @@ WW_HD m2 m2_redc(u128 t, u64 c_lo, u64 c_hi, u64 c_top, u64 p, u64 q)
     u64 thi = c_hi + cy;
     u64 top = c_top | ((thi < cy) ? 1 : 0);
     u64 t1p = (top | (thi >= p ? 1 : 0)) ? thi - p : thi; /* tp >> 64 < 2p */
-    u64 up0 = tlo * q;
+    u64 up0 = mul_lo64(tlo, q);
     u64 v1p = mul_hi_known(p, up0, tlo);
     bool b = t.hi < v1;
     m2 r;
@@ -505,10 +525,10 @@ WW_HD m2 m2_redc(u128 t, u64 c_lo, u64 c_hi, u64 c_top, u64 p, u64 q)
 WW_HD m2 m2_sqr(m2 x, u64 p, u64 q, u64 np)
 {
     u128 t = sqr_wide(x.d0);
-    u64 u0 = t.lo * q;
+    u64 u0 = mul_lo64(t.lo, q);
     u64 v1 = mul_hi_known(p, u0, t.lo);
     u128 c = mul_wide_add(x.d0, dbl_mod(x.d1, p, np), u0); /* high word < p */
-    u64 up0 = c.lo * q;
+    u64 up0 = mul_lo64(c.lo, q);
     u64 v1p = mul_hi_known(p, up0, c.lo);
     bool b;
     u64 z0 = sub_b(t.hi, v1, &b);
@@ -540,10 +560,10 @@ WW_HD m2 m2_mul(m2 x, m2 y, u64 p, u64 q)
 WW_HD m2 m2_sqr_add(m2 x, m2 k, u64 p, u64 q, u64 np)
 {
     u128 t = sqr_wide_add(x.d0, k.d0);
-    u64 u0 = t.lo * q;
+    u64 u0 = mul_lo64(t.lo, q);
     u64 v1 = mul_hi_known(p, u0, t.lo);
     u128 c = mul_wide_add2(x.d0, dbl_mod(x.d1, p, np), u0, k.d1);
-    u64 up0 = c.lo * q;
+    u64 up0 = mul_lo64(c.lo, q);
     u64 v1p = mul_hi_known(p, up0, c.lo);
     bool b;
     u64 z0 = sub_b(t.hi, v1, &b);
@@ -560,10 +580,10 @@ WW_HD m2 m2_sqr_add(m2 x, m2 k, u64 p, u64 q, u64 np)
 WW_HD m2 m2_mul_digit(u64 v, m2 y, u64 p, u64 q)
 {
     u128 t = mul_wide(v, y.d0);
-    u64 u0 = t.lo * q;
+    u64 u0 = mul_lo64(t.lo, q);
     u64 v1 = mul_hi_known(p, u0, t.lo);
     u128 c = mul_wide_add(v, y.d1, u0);
-    u64 up0 = c.lo * q;
+    u64 up0 = mul_lo64(c.lo, q);
     u64 v1p = mul_hi_known(p, up0, c.lo);
     bool b;
     u64 z0 = sub_b(t.hi, v1, &b);
@@ -579,7 +599,7 @@ WW_HD m2 m2_mul_digit(u64 v, m2 y, u64 p, u64 q)
 WW_HD m2 m2_mul_add(m2 x, m2 y, m2 k, u64 p, u64 q, u64 np)
 {
     u128 t = mul_wide_add(x.d0, y.d0, k.d0);
-    u64 u0 = t.lo * q;
+    u64 u0 = mul_lo64(t.lo, q);
     u64 v1 = mul_hi_known(p, u0, t.lo);
     u128 a = mul_wide_add(x.d0, y.d1, u0);
     u128 e = mul_wide_add(x.d1, y.d0, k.d1);
@@ -587,7 +607,7 @@ WW_HD m2 m2_mul_add(m2 x, m2 y, m2 k, u64 p, u64 q, u64 np)
     u128 sum = add128_c(a, e, &top);
     bool big = top | (sum.hi >= p);
     u64 t1p = sum.hi + (big ? np : 0); /* minus p, modulo 2^64 */
-    u64 up0 = sum.lo * q;
+    u64 up0 = mul_lo64(sum.lo, q);
     u64 v1p = mul_hi_known(p, up0, sum.lo);
     bool b;
     u64 z0 = sub_b(t.hi, v1, &b);
@@ -611,10 +631,10 @@ WW_HD m2 m2_mulpow2(m2 x, u32 v, u64 p, double pinv)
 /* Montgomery form back to ordinary digits (REDC, wwcpu.cpp:189-206). */
 WW_HD m2 m2_get(m2 x, u64 p, u64 q)
 {
-    u64 u0 = q * x.d0;
+    u64 u0 = mul_lo64(q, x.d0);
     u64 v1 = mul_hi_known(p, u0, x.d0);
     u64 tp = x.d1 + u0;
-    u64 up0 = q * tp;
+    u64 up0 = mul_lo64(q, tp);
     u64 t1p = tp < x.d1 ? 1 : 0;
     u64 v1p = mul_hi_known(p, up0, tp);
     bool b = v1 != 0;
