@@ -24,9 +24,12 @@
  * recomputed in registers), and the Wieferich and Wall-Sun-Sun kernels are one
  * launch, so there is no g_one round trip.
  *
- * Thread safety: calls on different contexts are independent. Calls on one context
- * are serialised by the caller (the reference drives one in-order queue from one
- * host thread, main.cpp:1683-1795).
+ * Thread safety: calls on different contexts are independent. Every entry point that
+ * takes a context holds that context's lock for its duration, so calls on one context
+ * from several host threads are serialised (they still share one in-order stream, as the
+ * reference's single queue does, main.cpp:1683-1795). A batch is used under its
+ * context's lock; ww_batch_set_grid and ww_batch_destroy take no context and must not
+ * race with a launch of the same batch. Servers lock themselves.
  */
 #ifndef WW_API_H
 #define WW_API_H
@@ -103,10 +106,13 @@ int ww_open(int device, uint64_t list_capacity, uint32_t result_capacity, ww_ctx
    batches and servers first. */
 int ww_close(ww_ctx *ctx);
 
-/* Upper bound on the list length for one range [low, high): an explicit
-   Brun-Titchmarsh bound 2w/ln(w) on primes in a window of width w, plus the base-2
-   strong pseudoprimes it could hold. Replaces the primesieve-based sizing
-   (main.cpp:1087-1091). */
+/* List capacity for one range [low, high): the Montgomery-Vaughan (Brun-Titchmarsh)
+   bound 2w/ln(w) on the primes in a window of width w, which is a proven bound, times
+   1.02 plus 4096 for the base-2 strong pseudoprimes the window may hold. That slack is a
+   heuristic, not a bound (there are about 3.1e7 such pseudoprimes below 2^64, and no
+   window is known to hold more than a few): a list that overflows anyway is caught by the
+   counters (primecount > capacity, main.cpp:802-805, as the drop-in host checks after
+   every chunk). Replaces the primesieve-based sizing (main.cpp:1087-1091). */
 uint64_t ww_list_bound(uint64_t low, uint64_t high);
 
 /* Bound B (0 or 127..65535) of the deep sieve applied by the candidate generators after
@@ -129,7 +135,7 @@ int ww_get_deep_bound(ww_ctx *ctx, uint32_t *bound);
 int ww_set_presieve_bound(ww_ctx *ctx, uint32_t bound);
 int ww_get_presieve_bound(ww_ctx *ctx, uint32_t *bound);
 
-/* clearresult (clearresult.cl:36-50): zero checksum, resultcount, totalprimecount
+/* clearresult (clearresult.cl:12-26): zero checksum, resultcount, totalprimecount
    and primecount_max. Stream-ordered. */
 int ww_clear_results(ww_ctx *ctx);
 
@@ -292,6 +298,8 @@ int ww_batch_time(ww_ctx *ctx, ww_batch *b, double *ms);
    alone (their sum is ww_batch_time). Either pointer may be null. */
 int ww_batch_kernel_times(ww_ctx *ctx, ww_batch *b, double *presieve_ms, double *search_ms);
 
+/* Waits for the batch's launches on its context's stream (not for the whole device),
+   then frees it. Destroy batches before their context. */
 int ww_batch_destroy(ww_batch *b);
 
 /* One configuration of the persistent form and its measured speed. */

@@ -12,6 +12,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "../../include/ww_api.h"
@@ -26,6 +27,10 @@ using ww::DevState;
 using ww::SlotSum;
 
 struct ww_ctx {
+    /* held by every entry point that takes the context (recursive: the autotuner and the
+       range loop call other entry points), so calls on one context from several host
+       threads are serialised */
+    std::recursive_mutex mu;
     int device = 0;
     hipStream_t stream = nullptr;
     DevState *d_state = nullptr;
@@ -84,6 +89,16 @@ int fail(int code, const char *fmt, ...)
             return fail(WW_E_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_),  \
                         __FILE__, __LINE__);                                             \
     } while (0)
+
+/* Holds the context's lock for the rest of the calling entry point. */
+struct CtxLock {
+    std::unique_lock<std::recursive_mutex> l;
+    explicit CtxLock(ww_ctx *c)
+    {
+        if (c) l = std::unique_lock<std::recursive_mutex>(c->mu);
+    }
+};
+#define WW_LOCK(c) CtxLock ctx_lock_(c)
 
 int use(ww_ctx *c)
 {
@@ -200,6 +215,7 @@ int ww_ctx_internals(ww_ctx *c, int *device, const ww::DeepPrime **deep, uint32_
                      int *cus, int *tuned_blocks, uint32_t *tuned_claim,
                      const ww::WidePrime **wide, uint32_t *nwide)
 {
+    WW_LOCK(c);
     if (!c) return fail(WW_E_ARG, "null context");
     uint32_t wfirst, wcount;
     wide_span(c, &wfirst, &wcount);
@@ -253,9 +269,11 @@ uint64_t ww_list_bound(uint64_t low, uint64_t high)
 {
     if (high <= low) return 0;
     double w = (double)(high - low);
-    /* Montgomery-Vaughan: pi(x+w) - pi(x) <= 2w/ln w for w > 1; small windows are
-       bounded by the count of odd numbers instead. Base-2 strong pseudoprimes below
-       2^64 number about 3.1e7 in all; a window gets at most its share plus slack. */
+    /* Montgomery-Vaughan: pi(x+w) - pi(x) <= 2w/ln w for w > 1 (proven); small windows
+       are bounded by the count of odd numbers instead. The base-2 strong pseudoprimes on
+       top (about 3.1e7 below 2^64 in all) get the slack x1.02 + 4096: a heuristic, not a
+       bound. An overflow past it still shows in the counters (primecount above the
+       capacity), which the drop-in host checks after every chunk. */
     double primes = w < 64 ? w : 2.0 * w / std::log(w);
     double odd = w / 2 + 1;
     double b = std::fmin(primes, odd) * 1.02 + 4096;
@@ -328,6 +346,8 @@ int ww_open(int device, uint64_t list_capacity, uint32_t result_capacity, ww_ctx
 int ww_close(ww_ctx *c)
 {
     if (!c) return WW_OK;
+    {
+    WW_LOCK(c);
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (hipEvent_t e : c->ev_sieve) (void)hipEventDestroy(e);
@@ -344,12 +364,14 @@ int ww_close(ww_ctx *c)
     if (c->d_list) (void)hipFree(c->d_list);
     if (c->d_state) (void)hipFree(c->d_state);
     if (c->stream) (void)hipStreamDestroy(c->stream);
+    }
     delete c;
     return WW_OK;
 }
 
 int ww_set_deep_bound(ww_ctx *c, uint32_t bound)
 {
+    WW_LOCK(c);
     if (!c) return fail(WW_E_ARG, "null context");
     if (bound > ww::kMaxDeepBound) return fail(WW_E_ARG, "deep bound above %u", ww::kMaxDeepBound);
     uint32_t n = 0;
@@ -361,6 +383,7 @@ int ww_set_deep_bound(ww_ctx *c, uint32_t bound)
 
 int ww_get_deep_bound(ww_ctx *c, uint32_t *bound)
 {
+    WW_LOCK(c);
     if (!c || !bound) return fail(WW_E_ARG, "null argument");
     *bound = c->deep_bound;
     return WW_OK;
@@ -368,6 +391,7 @@ int ww_get_deep_bound(ww_ctx *c, uint32_t *bound)
 
 int ww_set_presieve_bound(ww_ctx *c, uint32_t bound)
 {
+    WW_LOCK(c);
     int rc = use(c);
     if (rc) return rc;
     if (bound > WW_MAX_PRESIEVE_BOUND)
@@ -400,6 +424,7 @@ int ww_set_presieve_bound(ww_ctx *c, uint32_t bound)
 
 int ww_get_presieve_bound(ww_ctx *c, uint32_t *bound)
 {
+    WW_LOCK(c);
     if (!c || !bound) return fail(WW_E_ARG, "null argument");
     *bound = c->presieve_bound;
     return WW_OK;
@@ -407,6 +432,7 @@ int ww_get_presieve_bound(ww_ctx *c, uint32_t *bound)
 
 int ww_clear_results(ww_ctx *c)
 {
+    WW_LOCK(c);
     int rc = use(c);
     if (rc) return rc;
     /* checksum, totalprimecount, then primecount_max + resultcount (the list length
@@ -418,6 +444,7 @@ int ww_clear_results(ww_ctx *c)
 
 int ww_clear_prp_count(ww_ctx *c)
 {
+    WW_LOCK(c);
     int rc = use(c);
     if (rc) return rc;
     HIP_TRY(hipMemsetAsync(&c->d_state->primecount, 0, 4, c->stream));
@@ -426,6 +453,7 @@ int ww_clear_prp_count(ww_ctx *c)
 
 int ww_getsegprimes(ww_ctx *c, uint64_t low, uint64_t high)
 {
+    WW_LOCK(c);
     int rc = use(c);
     if (rc) return rc;
     if (low < WW_MIN_START) return fail(WW_E_ARG, "low %llu below %llu", (unsigned long long)low, WW_MIN_START);
@@ -446,6 +474,7 @@ int ww_getsegprimes(ww_ctx *c, uint64_t low, uint64_t high)
 
 int ww_wieferich_wallsunsun(ww_ctx *c)
 {
+    WW_LOCK(c);
     int rc = use(c);
     if (rc) return rc;
     hipEvent_t a, b;
@@ -460,6 +489,7 @@ int ww_wieferich_wallsunsun(ww_ctx *c)
 
 int ww_search_range(ww_ctx *c, uint64_t low, uint64_t high, uint64_t chunk)
 {
+    WW_LOCK(c);
     int rc = use(c);
     if (rc) return rc;
     if (low < WW_MIN_START) return fail(WW_E_ARG, "low below %llu", WW_MIN_START);
@@ -482,6 +512,7 @@ int ww_search_range(ww_ctx *c, uint64_t low, uint64_t high, uint64_t chunk)
 
 int ww_search_fused(ww_ctx *c, uint64_t low, uint64_t high)
 {
+    WW_LOCK(c);
     int rc = use(c);
     if (rc) return rc;
     if (low < WW_MIN_START) return fail(WW_E_ARG, "low %llu below %llu", (unsigned long long)low, WW_MIN_START);
@@ -520,6 +551,7 @@ int ww_search_fused(ww_ctx *c, uint64_t low, uint64_t high)
 
 int ww_finish(ww_ctx *c)
 {
+    WW_LOCK(c);
     int rc = use(c);
     if (rc) return rc;
     HIP_TRY(hipStreamSynchronize(c->stream));
@@ -528,6 +560,7 @@ int ww_finish(ww_ctx *c)
 
 int ww_read_counters(ww_ctx *c, ww_counters *out)
 {
+    WW_LOCK(c);
     int rc = use(c);
     if (rc) return rc;
     if (!out) return fail(WW_E_ARG, "null out");
@@ -545,6 +578,7 @@ int ww_read_counters(ww_ctx *c, ww_counters *out)
 
 int ww_read_results(ww_ctx *c, ww_result *out, uint32_t cap, uint32_t *n)
 {
+    WW_LOCK(c);
     int rc = use(c);
     if (rc) return rc;
     ww_counters k;
@@ -563,6 +597,7 @@ int ww_read_results(ww_ctx *c, ww_result *out, uint32_t cap, uint32_t *n)
 
 int ww_read_prps(ww_ctx *c, uint64_t *out, uint64_t cap, uint64_t *n)
 {
+    WW_LOCK(c);
     int rc = use(c);
     if (rc) return rc;
     ww_counters k;
@@ -582,6 +617,7 @@ int ww_read_prps(ww_ctx *c, uint64_t *out, uint64_t cap, uint64_t *n)
 
 int ww_kernel_times(ww_ctx *c, double *ms_sieve, double *ms_tests, uint64_t *launches)
 {
+    WW_LOCK(c);
     int rc = use(c);
     if (rc) return rc;
     if ((rc = collect(c->ev_sieve, &c->ms_sieve))) return rc;
@@ -596,6 +632,7 @@ int ww_kernel_times(ww_ctx *c, double *ms_sieve, double *ms_tests, uint64_t *lau
 
 int ww_residues(ww_ctx *c, const uint64_t *p, uint64_t n, uint64_t *out)
 {
+    WW_LOCK(c);
     int rc = use(c);
     if (rc) return rc;
     if (n == 0) return WW_OK;
@@ -619,6 +656,7 @@ int ww_residues(ww_ctx *c, const uint64_t *p, uint64_t n, uint64_t *out)
 
 int ww_prp_flags(ww_ctx *c, const uint64_t *n, uint64_t count, uint8_t *out)
 {
+    WW_LOCK(c);
     int rc = use(c);
     if (rc) return rc;
     if (count == 0) return WW_OK;
@@ -649,6 +687,7 @@ int ww_prp_flags(ww_ctx *c, const uint64_t *n, uint64_t count, uint8_t *out)
 
 struct ww_batch {
     int device = 0;
+    hipStream_t stream = nullptr; /* the creating context's stream: every launch goes there */
     uint32_t n = 0;
     uint32_t res_cap = 0;
     int blocks_per_cu = 0;
@@ -741,6 +780,7 @@ extern "C" {
 int ww_batch_create(ww_ctx *c, const ww_range *ranges, uint32_t n, uint32_t result_capacity,
                     ww_batch **out)
 {
+    WW_LOCK(c);
     int rc = use(c);
     if (rc) return rc;
     if (!out || !ranges || n == 0 || result_capacity == 0) return fail(WW_E_ARG, "bad batch arguments");
@@ -761,6 +801,7 @@ int ww_batch_create(ww_ctx *c, const ww_range *ranges, uint32_t n, uint32_t resu
     }
     ww_batch *b = new ww_batch;
     b->device = c->device;
+    b->stream = c->stream;
     b->blocks_per_cu = c->tuned_blocks;
     b->claim = c->tuned_claim;
     b->n = n;
@@ -800,6 +841,7 @@ int ww_batch_set_grid(ww_batch *b, int blocks_per_cu, uint32_t claim)
 
 int ww_batch_launch(ww_ctx *c, ww_batch *b)
 {
+    WW_LOCK(c);
     int rc = use(c);
     if (rc) return rc;
     if (!b) return fail(WW_E_ARG, "null batch");
@@ -814,6 +856,7 @@ int ww_batch_launch(ww_ctx *c, ww_batch *b)
 
 int ww_batch_launch_graph(ww_ctx *c, ww_batch *b)
 {
+    WW_LOCK(c);
     int rc = use(c);
     if (rc) return rc;
     if (!b) return fail(WW_E_ARG, "null batch");
@@ -855,6 +898,7 @@ int ww_batch_launch_graph(ww_ctx *c, ww_batch *b)
 int ww_batch_read(ww_ctx *c, ww_batch *b, ww_range_sum *sums, ww_result *recs, uint32_t cap,
                   uint32_t *nrec, uint32_t *total)
 {
+    WW_LOCK(c);
     int rc = use(c);
     if (rc) return rc;
     if (!b) return fail(WW_E_ARG, "null batch");
@@ -879,6 +923,7 @@ int ww_batch_read(ww_ctx *c, ww_batch *b, ww_range_sum *sums, ww_result *recs, u
 
 int ww_batch_time(ww_ctx *c, ww_batch *b, double *ms)
 {
+    WW_LOCK(c);
     int rc = use(c);
     if (rc) return rc;
     if (!b || !ms) return fail(WW_E_ARG, "null argument");
@@ -891,6 +936,7 @@ int ww_batch_time(ww_ctx *c, ww_batch *b, double *ms)
 
 int ww_batch_kernel_times(ww_ctx *c, ww_batch *b, double *presieve_ms, double *search_ms)
 {
+    WW_LOCK(c);
     int rc = use(c);
     if (rc) return rc;
     if (!b) return fail(WW_E_ARG, "null batch");
@@ -907,7 +953,9 @@ int ww_batch_destroy(ww_batch *b)
 {
     if (!b) return WW_OK;
     (void)hipSetDevice(b->device);
-    (void)hipDeviceSynchronize();
+    /* launches of this batch are queued on its context's stream only: waiting for that
+       stream leaves other contexts and servers on the device running */
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
     drop_graphs(b);
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->evm) (void)hipEventDestroy(b->evm);
@@ -927,6 +975,7 @@ int ww_autotune(ww_ctx *c, uint64_t lo, uint64_t hi, const uint32_t *bounds, uin
                 const uint32_t *presieve, uint32_t np, ww_tuning *best, ww_tuning *trials,
                 uint32_t cap, uint32_t *ntrials)
 {
+    WW_LOCK(c);
     int rc = use(c);
     if (rc) return rc;
     static const uint32_t kBounds[] = {0, 1024, 2048, 4096, 8192, 16384, 65535};

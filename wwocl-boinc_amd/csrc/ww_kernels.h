@@ -13,7 +13,7 @@ namespace ww {
 
 /* Counters in device memory. Field meaning follows the reference's buffers:
    checksum / totalprimecount / resultcount (main.cpp:1446-1475, cleared by
-   clearresult.cl:36-50), primecount[0..1] (d_segprimecount, clearn.cl:12-21). */
+   clearresult.cl:12-26), primecount[0..1] (d_segprimecount, clearn.cl:12-21). */
 struct DevState {
     uint64_t checksum;
     uint64_t totalprimecount;

@@ -43,7 +43,7 @@ __global__ __launch_bounds__(kPreThreads) void k_presieve(const BatchSpan *spans
                                                           const WidePrime *tab, u32 ntab,
                                                           u64 *out)
 {
-    __shared__ u32 bm[kChunkWords];
+    __shared__ __align__(8) u32 bm[kChunkWords]; /* read back as u64 words below */
     __shared__ u32 rescue[kRescueCap];
     __shared__ u32 rcount;
 
