@@ -86,8 +86,13 @@ def test_cpu_checks_match_oracle():
 
 
 @pytest.mark.parametrize("lo,hi", [(127, 300_000), ((1 << 32) - 100_000, (1 << 32) + 100_000),
-                                   (1 << 63, (1 << 63) + 200_000)])
+                                   (1 << 63, (1 << 63) + 200_000), (495_700, 495_900),
+                                   ((1 << 64) - 200_000, (1 << 64) - 1)])
 def test_cpu_range_matches_oracle(lo, hi):
+    """ww_cpu_range is the startup self-test's CPU side, written independently of the
+    device arithmetic (128-bit products reduced by division, a Fibonacci-pair ladder):
+    pinned here against the oracle, including 495791 (the one list member on which the
+    reference's single-digit Wall-Sun-Sun phase runs) and the top of the domain."""
     prp, cks, _, flags = oracle.range_summary(lo, hi)
     assert wwlib.cpu_range(lo, hi) == (prp, cks, flags)
 

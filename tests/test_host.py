@@ -135,7 +135,8 @@ def test_host_small_range_matches_oracle(tmp_path):
     rc, out, err = run(tmp_path, 1, "3e6")
     assert rc == 0, out + err
     assert "init_data.xml not found" not in out
-    assert "GPU Wieferich/WallSunSun kernel tests passed" in out
+    assert "GPU Wieferich/WallSunSun kernel tests passed" in err  # stdout only with -debug
+    assert "Kernel profile:" in err
     text = results(tmp_path).splitlines()
     prp, cks, _ = oracle.range_results(127, 3_000_000)
     assert text[:-1] == expected_lines(127, 3_000_000)
@@ -181,3 +182,47 @@ def test_host_resume_gives_the_same_answer(tmp_path):
     assert results(tmp_path).splitlines()[-1] == "6522CBC4CA2E4CFF"
     st = (tmp_path / "stateB-WW.txt").read_text().split()
     assert st[:3] == [str(lo), str(hi), str(hi)]
+
+
+@pytest.mark.gpu
+def test_host_testprimes_checks_every_chunk(tmp_path):
+    """-testprimes (main.cpp:1731-1735): the two-kernel form, and every chunk's PRP list
+    checked against the CPU's primes of the chunk before the tests run."""
+    lo, hi = 10**9, 10**9 + 2_000_000
+    rc, out, err = run(tmp_path, lo, hi, "-testprimes")
+    assert rc == 0, out + err
+    assert "Comparing all GPU generated primes with CPU generated primes." in out
+    prp, cks, _ = oracle.range_results(lo, hi)
+    assert f"GPU primes match CPU ({prp} PRPs)" in out
+    assert results(tmp_path).splitlines() == expected_lines(lo, hi) + [f"{cks:016X}"]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("flags,target", [(["-debug", "-ws", "5"], 5.0), (["-compute"], 100.0),
+                                          (["-compute", "-ws", "5"], 100.0)],
+                         ids=["debug-ws5", "compute", "compute-ignores-ws"])
+def test_host_kernel_time_targets(tmp_path, flags, target):
+    """profileGPU (main.cpp:959-1107, 1424-1444): the chunk is sized from a timed probe to
+    10 ms, -ws N ms, or 100 ms with -compute (which ignores -ws); -debug prints the kernel
+    time split (main.cpp:1632-1661). The answer does not depend on any of it."""
+    lo, hi = 10**12, 10**12 + 10_000_000
+    rc, out, err = run(tmp_path, lo, hi, *flags)
+    assert rc == 0, out + err
+    line = next(l for l in err.splitlines() if l.startswith("Kernel profile:"))
+    probe_ms = float(line.split()[2])
+    est, actual = (int(x) for x in line.split(":")[-1].split("/"))
+    assert probe_ms > 0 and actual >= 1 << 20
+    expect = est * target / probe_ms  # the printed probe time is rounded to 1 us
+    if expect >= 1 << 34:
+        assert actual == 1 << 34
+    else:
+        assert abs(actual - max(1 << 20, expect)) <= 0.01 * expect + 1, (actual, expect)
+    if "-ws" in flags and "-compute" not in flags:
+        assert "Kernel work size of 5 specified on command line." in err
+    if "-compute" in flags:
+        assert "Compute mode enabled." in out and "Kernel work size" not in err
+    if "-debug" in flags:
+        assert "Percent of time spent in each kernel:" in out and "Prime Generator:" in out
+        assert "GPU persistent kernel test passed." in out
+    _, cks, _ = oracle.range_results(lo, hi)
+    assert results(tmp_path).splitlines() == expected_lines(lo, hi) + [f"{cks:016X}"]

@@ -22,7 +22,8 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
              "-Wall", "-Wno-unused-command-line-argument"]
 
 LIB_SOURCES = ["ww_kernels.hip", "ww_batch.hip", "ww_presieve.hip", "ww_api.cpp",
-               "ww_server.cpp", "ww_cpu.cpp", "ww_peak.hip"]
+               "ww_server.cpp", "ww_cpu.cpp", "ww_selftest.cpp", "ww_peak.hip"]
+HOST_SOURCES = ["wwocl_main.cpp", "wwocl_checks.cpp"]
 
 
 def _run(cmd):
@@ -44,13 +45,13 @@ def build_lib():
 
 
 def build_host():
-    src = os.path.join(HOST, "wwocl_main.cpp")
-    if not os.path.exists(src):
+    srcs = [os.path.join(HOST, f) for f in HOST_SOURCES]
+    if not all(os.path.exists(s) for s in srcs):
         return None
     os.makedirs(os.path.join(HERE, "bin"), exist_ok=True)
     out = os.path.join(HERE, "bin", "wwocl")
     _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"), "-o", out,
-          src, "-L", HERE, "-lww", "-Wl,-rpath,$ORIGIN/..", "-pthread"])
+          *srcs, "-L", HERE, "-lww", "-Wl,-rpath,$ORIGIN/..", "-pthread"])
     return out
 
 

@@ -17,8 +17,10 @@
  * "Tests done" lines) and takes the device the way boinc_get_opencl_ids does
  * (main.cpp:1288-1308): --device N on the command line, else <gpu_device_num> or
  * <gpu_opencl_dev_index> from init_data.xml, else device 0 with the reference's
- * "init_data.xml not found" message. The device work-size probe (main.cpp:959-1107) is
- * replaced by a fixed chunk size: the checksum does not depend on chunking.
+ * "init_data.xml not found" message. The work-size probe (profileGPU, main.cpp:959-1107)
+ * times the chunk body at the current position and scales the chunk to the reference's
+ * kernel-time targets: 10 ms, -ws N ms (1..100), or 100 ms with -compute. The checksum
+ * does not depend on chunking. Validation lives in wwocl_checks.cpp.
  */
 #include <algorithm>
 #include <cinttypes>
@@ -32,6 +34,7 @@
 #include <vector>
 
 #include "ww_api.h"
+#include "wwocl_checks.h"
 
 namespace {
 
@@ -53,18 +56,8 @@ void say(const char *fmt, ...)
     fflush(stdout);
 }
 
-[[noreturn]] void die(const char *fmt, ...)
-{
-    va_list a, b;
-    va_start(a, fmt);
-    va_copy(b, a);
-    vfprintf(stdout, fmt, a);
-    vfprintf(stderr, fmt, b);
-    va_end(b);
-    va_end(a);
-    fflush(stdout);
-    exit(EXIT_FAILURE);
-}
+/* Print on both streams and exit with EXIT_FAILURE (the reference's error pattern). */
+#define die(...) wwhost::fatal(true, __VA_ARGS__)
 
 void check(int rc, const char *what)
 {
@@ -167,37 +160,23 @@ void report_solution(const std::string &text)
     fclose(f);
 }
 
-/* The results-file line of one record, validated on the CPU: primes are formatted
-   (main.cpp:869-905), base-2 PRPs that are not prime are dropped (main.cpp:907-911),
-   anything else is a computation error (main.cpp:912-916). Returns false when dropped. */
-bool format_record(const ww_result &r, std::string &line, uint32_t &dropped)
-{
-    char buf[256];
-    if (ww_cpu_is_prime(r.p)) {
-        if (r.result == 1093) {
-            if (r.rem == 1 && r.quot == 0)
-                snprintf(buf, sizeof buf, "%" PRIu64 " is a Wieferich prime\n", r.p);
-            else
-                snprintf(buf, sizeof buf, "%" PRIu64 " is a Wieferich special instance (%+d %+d p)\n",
-                         r.p, r.rem, r.quot);
-        } else if (r.result == 1) {
-            snprintf(buf, sizeof buf, "%" PRIu64 " is a WallSunSun prime\n", r.p);
-        } else if (r.result == 2) {
-            snprintf(buf, sizeof buf, "%" PRIu64 " is a WallSunSun special instance (+0 %+d p)\n",
-                     r.p, r.quot);
-        } else {
-            die("Error: hostResult returned %d.\n", r.result);
-        }
-        line = buf;
-        return true;
+/* The library's context as the checks' view of the device. */
+class LibView : public wwhost::DeviceView {
+public:
+    explicit LibView(ww_ctx *c) : c_(c) {}
+    int read_counters(ww_counters *out) override { return ww_read_counters(c_, out); }
+    int read_results(ww_result *out, uint32_t cap, uint32_t *n) override
+    {
+        return ww_read_results(c_, out, cap, n);
     }
-    if (ww_cpu_is_prp(r.p)) {
-        ++dropped;
-        return false;
+    int read_prps(uint64_t *out, uint64_t cap, uint64_t *n) override
+    {
+        return ww_read_prps(c_, out, cap, n);
     }
-    fprintf(stderr, "Computation error, GPU found a composite result: %" PRIu64 "\n", r.p);
-    exit(EXIT_FAILURE);
-}
+
+private:
+    ww_ctx *c_;
+};
 
 struct Run {
     ww_ctx *ctx = nullptr;
@@ -206,36 +185,89 @@ struct Run {
     bool fused = true; /* chunks through the persistent kernel (no PRP list) */
 };
 
-/* checkPRPS (main.cpp:451-523): every list entry is prime or a base-2 PRP with no factor
-   <= 113, and the primes are exactly those of [lo, hi). The reference compares against
-   primesieve; here the CPU enumerates the candidates of the range with the host checks. */
-uint32_t check_prps(Run &run, uint64_t lo, uint64_t hi)
+/* One chunk [start, stop) of the search through the chunk body in use. */
+void run_chunk(Run &run, uint64_t start, uint64_t stop)
 {
-    uint64_t n = 0;
-    check(ww_read_prps(run.ctx, nullptr, 0, &n), "ww_read_prps");
-    std::vector<uint64_t> gpu(n);
-    check(ww_read_prps(run.ctx, gpu.data(), n, &n), "ww_read_prps");
-    std::vector<uint64_t> primes;
-    for (uint64_t p : gpu) {
-        if (ww_cpu_is_prime(p)) primes.push_back(p);
-        else if (!ww_cpu_is_prp(p)) die("Computation error, GPU found a composite: %" PRIu64 "\n", p);
+    if (run.fused) {
+        check(ww_search_fused(run.ctx, start, stop), "persistent kernel");
+    } else {
+        check(ww_clear_prp_count(run.ctx), "clear PRP count");
+        check(ww_getsegprimes(run.ctx, start, stop), "getsegprimes");
+        check(ww_wieferich_wallsunsun(run.ctx), "wieferich/wallsunsun");
     }
-    std::sort(primes.begin(), primes.end());
-    std::vector<uint64_t> cpu;
-    for (uint64_t v = lo | 1; v < hi && v >= lo; v += 2) {
-        if (ww_cpu_is_prp(v) && ww_cpu_is_prime(v)) cpu.push_back(v);
-        if (v + 2 < v) break;
+}
+
+/* Largest chunk the chunk body takes: 2^34 integers for the persistent kernel; for the
+   two-kernel form the reference's limit (main.cpp:1077-1079), whose list the context is
+   sized for. */
+uint64_t max_chunk(const Run &run) { return run.fused ? (1ull << 34) : 4294900000ull; }
+
+/* profileGPU (main.cpp:959-1107): the first estimate is the reference's (compute units x
+   2.2e6, or x 2.2e7 with -compute), timed at the current position (at least 1e8) once to
+   warm up and three times more; the chunk is that estimate scaled to the kernel-time
+   target. Leaves the counters cleared. */
+uint64_t profile_chunk(Run &run, int cus, uint64_t current, double target_ms, int compute)
+{
+    uint64_t est = (uint64_t)cus * (compute ? 22000000ull : 2200000ull);
+    est = std::min(est, max_chunk(run));
+    uint64_t start = std::max<uint64_t>(current, 100000000ull);
+    if (start > UINT64_MAX - est) start = UINT64_MAX - est;
+    double ms = 0;
+    for (int rep = 0; rep < 4; ++rep) {
+        check(ww_clear_results(run.ctx), "clear results");
+        check(ww_kernel_times(run.ctx, nullptr, nullptr, nullptr), "kernel times");
+        run_chunk(run, start, start + est);
+        double a = 0, b = 0;
+        check(ww_kernel_times(run.ctx, &a, &b, nullptr), "kernel times");
+        if (rep) ms += a + b; /* the first is ignored, as the reference does */
     }
-    if (primes.size() != cpu.size())
-        die("ERROR: gpu prime count: %zu does not match cpu: %zu\n", primes.size(), cpu.size());
-    for (size_t i = 0; i < cpu.size(); ++i)
-        if (primes[i] != cpu[i])
-            die("ERROR: gpu/cpu prime mismatch %" PRIu64 " vs %" PRIu64 "\n", primes[i], cpu[i]);
-    return (uint32_t)n;
+    check(ww_clear_results(run.ctx), "clear results");
+    check(ww_clear_prp_count(run.ctx), "clear PRP count");
+    const double avg = ms / 3.0;
+    double chunk = avg > 0 ? (double)est * (target_ms / avg) : (double)est;
+    uint64_t c = chunk >= (double)max_chunk(run) ? max_chunk(run) : (uint64_t)chunk;
+    if (c < (1ull << 20)) c = 1ull << 20;
+    fprintf(stderr, "Kernel profile: %0.3f ms. Estimated / Actual worksize: %" PRIu64 " / %" PRIu64 "\n",
+            avg, est, c);
+    if (run.debug)
+        printf("Kernel profile: %0.3f ms. Estimated / Actual worksize: %" PRIu64 " / %" PRIu64 "\n",
+               avg, est, c);
+    return c;
+}
+
+/* -debug (main.cpp:1632-1661): time one chunk at the current position in each kernel. The
+   two-kernel form splits the sieve + PRP kernel (with its pre-sieve) from the fused
+   Wieferich + Wall-Sun-Sun kernel; the persistent kernel, which the search uses unless
+   -testprimes, does all of it in one launch. Leaves the counters cleared. */
+void profile_kernels(Run &run, uint64_t current, uint64_t top, uint64_t chunk)
+{
+    printf("Profiling kernels...\n");
+    uint64_t stop = current + std::min<uint64_t>(chunk, 4294900000ull);
+    if (stop > top || stop < current) stop = top;
+    double prime_ms = 0, ww_ms = 0, fused_ms = 0;
+    check(ww_kernel_times(run.ctx, nullptr, nullptr, nullptr), "kernel times");
+    check(ww_clear_results(run.ctx), "clear results");
+    check(ww_clear_prp_count(run.ctx), "clear PRP count");
+    check(ww_getsegprimes(run.ctx, current, stop), "getsegprimes");
+    check(ww_wieferich_wallsunsun(run.ctx), "wieferich/wallsunsun");
+    check(ww_kernel_times(run.ctx, &prime_ms, &ww_ms, nullptr), "kernel times");
+    check(ww_clear_results(run.ctx), "clear results");
+    check(ww_search_fused(run.ctx, current, stop), "persistent kernel");
+    check(ww_kernel_times(run.ctx, nullptr, &fused_ms, nullptr), "kernel times");
+    check(ww_clear_results(run.ctx), "clear results");
+    check(ww_clear_prp_count(run.ctx), "clear PRP count");
+    const double total = prime_ms + ww_ms;
+    const int prime_pct = total > 0 ? (int)(prime_ms / total * 100.0) : 0;
+    const int ww_pct = total > 0 ? (int)(ww_ms / total * 100.0) : 0;
+    printf("Percent of time spent in each kernel:\n");
+    printf("\tPrime Generator:\t%d %%\t(%0.1f ms)\n\tWieferich+WallSunSun Test:\t%d %%\t(%0.1f ms)\n",
+           prime_pct, prime_ms, ww_pct, ww_ms);
+    printf("\tPersistent kernel (sieve, PRP and both tests in one launch):\t(%0.1f ms)\n", fused_ms);
 }
 
 /* testKernels (main.cpp:590-738): the device path and the CPU path over
-   [start, start + min(2e6, chunk)) must agree on PRP count, checksum and result count. */
+   [start, start + min(2e6, chunk)) must agree on PRP count, checksum and result count.
+   The CPU side (ww_cpu_range) is a formulation of its own, not the device arithmetic. */
 void test_kernels(Run &run, uint64_t start, uint64_t top, uint64_t chunk)
 {
     uint64_t stop = start + std::min<uint64_t>(2000000, chunk);
@@ -248,67 +280,45 @@ void test_kernels(Run &run, uint64_t start, uint64_t top, uint64_t chunk)
     check(ww_wieferich_wallsunsun(run.ctx), "wieferich/wallsunsun");
     uint64_t cpu[3];
     check(ww_cpu_range(start, stop, cpu), "cpu range");
-    uint32_t gpu_prps = check_prps(run, start, stop);
-    if (gpu_prps != cpu[0])
-        die("ERROR: gpu / cpu prp count mismatch %u / %u\n", gpu_prps, (uint32_t)cpu[0]);
-    say("GPU prime generator kernel test passed. (%u PRPs)\n", gpu_prps);
+    LibView view(run.ctx);
+    const uint32_t gpu_prps = wwhost::check_prps(view, start, stop);
     ww_counters k;
     check(ww_read_counters(run.ctx, &k), "read counters");
-    if (k.checksum != cpu[1])
-        die("ERROR: gpu / cpu checksum mismatch %" PRIu64 " / %" PRIu64 "\n", k.checksum, cpu[1]);
-    if (k.resultcount != cpu[2])
-        die("ERROR: gpu / cpu result count mismatch %u / %u\n", k.resultcount, (uint32_t)cpu[2]);
-    say("GPU Wieferich/WallSunSun kernel tests passed. (%" PRIu64 " checksum, %u results)\n",
-        k.checksum, k.resultcount);
+    wwhost::compare_selftest(gpu_prps, k, cpu);
+    fprintf(stderr, "GPU prime generator kernel test passed. (%u PRPs)\n", gpu_prps);
+    if (run.debug) printf("GPU prime generator kernel test passed. (%u PRPs)\n", gpu_prps);
+    fprintf(stderr, "GPU Wieferich/WallSunSun kernel tests passed. (%" PRIu64 " checksum, %u results)\n",
+            k.checksum, k.resultcount);
+    if (run.debug)
+        printf("GPU Wieferich/WallSunSun kernel tests passed. (%" PRIu64 " checksum, %u results)\n",
+               k.checksum, k.resultcount);
     if (run.fused) {
         /* the chunk body the search uses must agree with the CPU path too */
         check(ww_clear_results(run.ctx), "clear results");
         check(ww_search_fused(run.ctx, start, stop), "persistent kernel");
         ww_counters f;
         check(ww_read_counters(run.ctx, &f), "read counters");
-        if (f.totalprimecount != cpu[0] || f.checksum != cpu[1] || f.resultcount != cpu[2])
-            die("ERROR: persistent kernel / cpu mismatch: %" PRIu64 " / %" PRIu64 " PRPs, checksum %"
-                PRIu64 " / %" PRIu64 ", %u / %u results\n",
-                f.totalprimecount, cpu[0], f.checksum, cpu[1], f.resultcount, (uint32_t)cpu[2]);
+        wwhost::compare_selftest_fused(f, cpu);
         if (run.debug) printf("GPU persistent kernel test passed.\n");
     }
 }
 
-/* getResults (main.cpp:768-956). */
+/* getResults (main.cpp:768-956): validation in wwocl_checks.cpp, then the results file,
+   the -debug report and the speed bookkeeping. */
 void get_results(Run &run, int ckpt_time, State &st)
 {
-    ww_counters k;
-    check(ww_read_counters(run.ctx, &k), "read counters");
+    LibView view(run.ctx);
+    const wwhost::CheckpointReport rep = wwhost::collect_results(view, run.list_cap, kNumResults);
+    const ww_counters &k = rep.counters;
     st.cksm += k.checksum;
-    if (k.primecount_max > run.list_cap) die("Error: prime array overflow!\n");
-    uint32_t dropped = 0;
-    if (k.resultcount > 0) {
-        if (k.resultcount > kNumResults)
-            die("Error: number of results (%u) overflowed array.\n", k.resultcount);
-        std::vector<ww_result> recs(k.resultcount);
-        uint32_t n = 0;
-        check(ww_read_results(run.ctx, recs.data(), k.resultcount, &n), "read results");
-        recs.resize(n);
-        /* by p, Wieferich records before Wall-Sun-Sun records of the same p: the order the
-           reference's stable bubble sort yields (main.cpp:843-854) */
-        std::stable_sort(recs.begin(), recs.end(), [](const ww_result &a, const ww_result &b) {
-            if (a.p != b.p) return a.p < b.p;
-            return (a.result == 1093) && (b.result != 1093);
-        });
-        std::string text;
-        for (const ww_result &r : recs) {
-            std::string line;
-            if (format_record(r, line, dropped)) text += line;
-        }
-        report_solution(text);
-    }
+    if (k.resultcount > 0) report_solution(rep.text);
     if (run.debug) {
         printf("PRPs tested since last checkpoint: %" PRIu64 "\n", k.totalprimecount);
-        printf("Max number of results: %u\n", kNumResults);
-        printf("Number of results: %u\n", k.resultcount);
-        printf("PRPs removed from results: %u\n", dropped);
-        printf("Largest kernel PRP count: %u\n", k.primecount_max);
-        printf("Max kernel PRP count for malloc: %" PRIu64 "\n", run.list_cap);
+        printf("Max number of results:             %u\n", kNumResults);
+        printf("Number of results:                 %u\n", k.resultcount);
+        printf("PRPs removed from results:         %u\n", rep.dropped);
+        printf("Largest kernel PRP count:          %u\n", k.primecount_max);
+        printf("Max kernel PRP count for malloc:   %" PRIu64 "\n", run.list_cap);
     }
     if (ckpt_time == 0) ckpt_time = 1;
     double p_sec = ((double)(k.totalprimecount / (uint64_t)ckpt_time)) / 1000000.0;
@@ -344,9 +354,10 @@ void usage(const char *prog)
 {
     printf("Usage: %s START END -testprimes -compute -debug -ws #\nWhere 127 <= START < END < 2^64\n", prog);
     printf("-testprimes is optional and will compare all GPU generated primes with CPU generated primes (very slow)\n\n");
-    printf("-compute is optional and uses the largest chunk size.\n\n");
-    printf("-debug is optional and will display more information.\n\n");
-    printf("-ws # is optional chunk size scale, 1..100 (default 100 = 2^32 integers per chunk).\n\n");
+    printf("-compute is optional and will fully load the gpu with large kernels (100 ms each).\n\n");
+    printf("-debug is optional and will display more information in standalone mode.\n\n");
+    printf("-ws # is optional worksize for adjusting the GPU load higher for faster computation or lower for better screen refresh rate.\n");
+    printf(" If using compute mode, -ws is ignored.  Usage: -ws 10 for default (10 ms kernels), -ws 1 for lowest GPU load, -ws 100 for highest GPU load.\n\n");
     printf("Example format: %s 10000000 20000000\n", prog);
     printf("            Or: %s 1e7 2e7\n", prog);
     exit(EXIT_FAILURE);
@@ -374,9 +385,18 @@ int main(int argc, char *argv[])
         } else if (strcmp(argv[w], "-debug") == 0) {
             debug = 1;
             printf("Debug info enabled.\n");
-        } else if (strcmp(argv[w], "-ws") == 0 && w + 1 < argc) {
+        }
+    }
+    /* command-line kernel time in ms, ignored in compute mode (main.cpp:1424-1444) */
+    if (!compute) {
+        for (int w = 3; w + 1 < argc; ++w) {
+            if (strcmp(argv[w], "-ws") != 0) continue;
             int v = 0;
-            if (sscanf(argv[w + 1], "%d", &v) == 1 && v > 0 && v <= 100) ws = v;
+            if (sscanf(argv[w + 1], "%d", &v) == 1 && v > 0 && v <= 100) {
+                fprintf(stderr, "Kernel work size of %d specified on command line.\n", v);
+                if (debug) printf("Kernel work size of %d specified on command line.\n", v);
+                ws = v;
+            }
         }
     }
 
@@ -407,10 +427,6 @@ int main(int argc, char *argv[])
         return 0;
     }
 
-    /* chunk size: 2^32 integers (a 3.2 GB list at most), scaled down by -ws */
-    uint64_t chunk = 1ull << 32;
-    if (!compute && ws) chunk = std::max<uint64_t>(1ull << 20, chunk / 100 * (uint64_t)ws);
-
     const int device = pick_device(argc, argv);
     int ndev = 0;
     if (ww_device_count(&ndev) != WW_OK || device < 0 || device >= ndev)
@@ -423,11 +439,20 @@ int main(int argc, char *argv[])
     Run run;
     run.debug = debug;
     run.fused = !test_prps;
-    run.list_cap = ww_list_bound(1ull << 32, (1ull << 32) + chunk); /* worst case: low p */
-    if (run.list_cap < ww_list_bound(WW_MIN_START, WW_MIN_START + chunk))
-        run.list_cap = ww_list_bound(WW_MIN_START, WW_MIN_START + chunk);
+    /* the PRP list: the two-kernel form (-testprimes, and the -debug split) needs one for a
+       whole chunk; the persistent kernel needs none, so then it only holds the self-test's
+       2e6-integer range (worst case: the smallest p, where primes are densest) */
+    const uint64_t list_span = (test_prps || debug) ? 4294900000ull : 2000000ull;
+    run.list_cap = std::max(ww_list_bound(1ull << 32, (1ull << 32) + list_span),
+                            ww_list_bound(WW_MIN_START, WW_MIN_START + list_span));
     check(ww_open(device, run.list_cap, kNumResults, &run.ctx), "ww_open");
-    if (debug) printf("chunk size: %" PRIu64 " integers, list capacity %" PRIu64 "\n", chunk, run.list_cap);
+
+    const double target_ms = compute ? 100.0 : (ws ? (double)ws : 10.0);
+    const uint64_t chunk = profile_chunk(run, info.compute_units, st.current, target_ms, compute);
+    if (debug) {
+        printf("chunk size: %" PRIu64 " integers, list capacity %" PRIu64 "\n", chunk, run.list_cap);
+        profile_kernels(run, st.current, top, chunk);
+    }
 
     test_kernels(run, st.current, top, chunk);
     say("Starting search...\n");
@@ -445,11 +470,12 @@ int main(int argc, char *argv[])
             last_progress = now;
         }
         if (run.fused) {
-            check(ww_search_fused(run.ctx, start, stop), "persistent kernel");
-        } else { /* -testprimes needs the PRP list of the two-kernel form */
+            run_chunk(run, start, stop);
+        } else { /* -testprimes checks the PRP list of the two-kernel form (main.cpp:1731-1735) */
+            LibView view(run.ctx);
             check(ww_clear_prp_count(run.ctx), "clear PRP count");
             check(ww_getsegprimes(run.ctx, start, stop), "getsegprimes");
-            if (test_prps) printf("GPU primes match CPU (%u PRPs)\n", check_prps(run, start, stop));
+            printf("GPU primes match CPU (%u PRPs)\n", wwhost::check_prps(view, start, stop));
             check(ww_wieferich_wallsunsun(run.ctx), "wieferich/wallsunsun");
         }
         if (++in_flight == 3) { /* bound the queue depth as main.cpp:1685-1691 */
