@@ -375,6 +375,32 @@ int ww_server_get_stats(ww_server *s, ww_server_stats *out);
 /* Stop after the work already submitted, then free everything. */
 int ww_server_destroy(ww_server *s);
 
+/* ---------------------------------------------------------------------------
+ * Allocation-free search. No counterpart in the reference: an entry point for callers
+ * that cannot allocate device memory during the call (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).
+ * ------------------------------------------------------------------------- */
+
+/* Most ranges one allocation-free call takes. */
+#define WW_NOALLOC_MAX_RANGES 4096u
+
+/* The persistent search of n ranges (1 <= n <= WW_NOALLOC_MAX_RANGES) as the batched form
+   does it (ww_batch_create + ww_batch_launch + ww_batch_read), but with every buffer in
+   device memory the caller owns: d_ranges (n ww_range, input; a range that is not
+   WW_MIN_START <= lo < hi is searched as empty), d_sums (n ww_range_sum, output: per
+   range as the plain form gives it), d_records (record_cap records, pad = range index;
+   order unspecified) and d_record_count (one uint32_t: records produced, which may exceed
+   record_cap). It calls no hipMalloc, hipFree or allocating library routine: the span
+   table, segment prefix and kernel state (about 160 KB) live in one fixed device array
+   per GPU, whose address is fetched when a context is opened on that GPU. Every call on
+   a GPU shares that array, so calls that overlap on one GPU (from contexts on different
+   streams) race; calls on one context are ordered on its stream. It runs without the wide
+   pre-sieve (same results, slower launch). Stream-ordered: ww_finish, then read the
+   outputs. */
+int ww_search_ranges_noalloc(ww_ctx *ctx, const ww_range *d_ranges, uint32_t n,
+                             ww_range_sum *d_sums, ww_result *d_records, uint32_t record_cap,
+                             uint32_t *d_record_count);
+
 /* Measurement support, no counterpart in the reference: the issue rate of
    independent 32x32->64 multiplies on every CU of `device` (kind 0: v_mad_u64_u32,
    kind 1: v_mul_hi_u32), in operations per second. Prices the roofline of the

@@ -299,6 +299,12 @@ int ww_open(int device, uint64_t list_capacity, uint32_t result_capacity, ww_ctx
         return rc;
     };
     if (hipSetDevice(device) != hipSuccess) return bail(fail(WW_E_HIP, "hipSetDevice"));
+    {
+        /* the allocation-free entry point's scratch address on this device */
+        const hipError_t ne = ww::noalloc_init(device);
+        if (ne != hipSuccess)
+            return bail(fail(WW_E_HIP, "scratch address (hipGetSymbolAddress): %s", hipGetErrorString(ne)));
+    }
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) != hipSuccess)
         return bail(fail(WW_E_HIP, "hipGetDeviceProperties"));
@@ -546,6 +552,26 @@ int ww_search_fused(ww_ctx *c, uint64_t low, uint64_t high)
     HIP_TRY(hipEventRecord(b, c->stream));
     c->launches++;
     if (c->ev_tests.size() >= kMaxPendingEvents) return collect(c->ev_tests, &c->ms_tests);
+    return WW_OK;
+}
+
+int ww_search_ranges_noalloc(ww_ctx *c, const ww_range *d_ranges, uint32_t n,
+                             ww_range_sum *d_sums, ww_result *d_records, uint32_t record_cap,
+                             uint32_t *d_record_count)
+{
+    WW_LOCK(c);
+    int rc = use(c);
+    if (rc) return rc;
+    if (n == 0 || n > WW_NOALLOC_MAX_RANGES)
+        return fail(WW_E_CAPACITY, "ww_search_ranges_noalloc: %u ranges (1..%u allowed)", n,
+                    WW_NOALLOC_MAX_RANGES);
+    if (!d_ranges || !d_sums || !d_record_count || (record_cap && !d_records))
+        return fail(WW_E_ARG, "ww_search_ranges_noalloc: null buffer");
+    int nb = c->tuned_blocks > 0 ? c->tuned_blocks : ww::batch_blocks_per_cu();
+    if (nb <= 0) nb = 4;
+    HIP_TRY(ww::launch_noalloc(c->device, d_ranges, n, reinterpret_cast<SlotSum *>(d_sums),
+                               d_records, record_cap, d_record_count, c->cus * nb, c->tuned_claim,
+                               c->d_deep, c->ndeep, c->stream));
     return WW_OK;
 }
 

@@ -85,6 +85,15 @@ hipError_t launch_batch(const BatchSpan *spans, const uint64_t *prefix, uint32_t
 hipError_t launch_fold(const BatchState *bst, const SlotSum *sum, const ww_result *bres,
                        uint32_t bcap, DevState *ds, ww_result *res, uint32_t cap, hipStream_t s);
 
+/* The allocation-free search (ww_noalloc.hip): noalloc_init fetches this device's
+   address of the fixed scratch array (once per device, with the device current);
+   launch_noalloc builds the spans and prefix there from the caller's ranges, clears the
+   caller's sums, runs the persistent kernel and writes the record count. */
+hipError_t noalloc_init(int device);
+hipError_t launch_noalloc(int device, const ww_range *ranges, uint32_t n, SlotSum *sums,
+                          ww_result *res, uint32_t res_cap, uint32_t *res_count, int grid,
+                          uint32_t claim, const DeepPrime *deep, uint32_t ndeep, hipStream_t s);
+
 } // namespace ww
 
 #endif

@@ -24,6 +24,7 @@ WW_DEFAULT_RESULTS = 15000
 WW_DEFAULT_DEEP_BOUND = 2048
 WW_DEFAULT_PRESIEVE_BOUND = 1048576
 WW_MAX_PRESIEVE_BOUND = 4194304
+WW_NOALLOC_MAX_RANGES = 4096
 
 U64 = ctypes.c_uint64
 U32 = ctypes.c_uint32
@@ -168,6 +169,9 @@ _SIGNATURES = [
                                        ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(U64)]),
     ("ww_cpu_spsp2", ctypes.c_int, [U64, ctypes.c_int32, ctypes.c_int32]),
     ("ww_search_fused", ctypes.c_int, [ctypes.c_void_p, U64, U64]),
+    ("ww_search_ranges_noalloc", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, U32,
+                                                ctypes.c_void_p, ctypes.c_void_p, U32,
+                                                ctypes.c_void_p]),
     ("ww_cpu_range", ctypes.c_int, [U64, U64, ctypes.POINTER(U64)]),
     ("ww_deep_table", U32, [U32, ctypes.POINTER(DeepPrime), U32]),
     ("ww_presieve_table", U32, [U32, U32, ctypes.POINTER(WidePrime), U32]),
