@@ -338,8 +338,9 @@ typedef struct ww_server ww_server;
 typedef struct {
     uint32_t slots;             /* requests resident at once (0 = 4096) */
     uint32_t tick_segments;     /* 8192-integer segments per launch (0 = 2^17) */
-    uint64_t hbm_budget;        /* device bytes admissions may reserve (0 = 64 GiB; capped at 90% of free HBM) */
-    uint64_t bytes_per_request; /* reservation of one admitted request (0 = 1 MiB) */
+    uint64_t hbm_budget;        /* device bytes for the admitted requests' resident sieve state,
+                                   allocated as one arena (0 = 64 GiB; capped at 90% of free HBM) */
+    uint64_t bytes_per_request; /* reserved per request on top of its sieve state (0 = none) */
     uint32_t claim;             /* segments per claim (0 = 1: shortest tick tails) */
     int32_t blocks_per_cu;      /* workgroups per CU (-1 = context default, 0 = as many as fit) */
 } ww_server_config;
@@ -354,13 +355,21 @@ typedef struct {
 typedef struct {
     double t_start;
     uint64_t submitted, completed, tested, ticks, segments, pending, active;
+    uint64_t reserved_bytes;      /* HBM budget held by the active requests now */
+    uint64_t peak_reserved_bytes; /* the most held at once */
+    uint64_t peak_active;         /* the most requests active at once */
 } ww_server_stats;
 
 /* Start a server on the context's device (uses its deep-sieve table and tuned geometry;
-   cfg may be null). Allocates device and pinned memory. */
+   cfg may be null). Allocates device and pinned memory, including the HBM-budget arena.
+   An admitted request holds, until it completes, its resident sieve state: the wide
+   pre-sieve's words for all its segments (512 bytes per 8192 integers, written once at
+   admission), plus bytes_per_request. Requests are admitted first come, first served,
+   when a slot and a free run of the arena of their size are available. */
 int ww_server_create(ww_ctx *ctx, const ww_server_config *cfg, ww_server **out);
 
-/* Queue a request for [lo, hi) (WW_MIN_START <= lo < hi); non-blocking. */
+/* Queue a request for [lo, hi) (WW_MIN_START <= lo < hi); non-blocking. WW_E_CAPACITY
+   for a request whose reservation exceeds the whole HBM budget. */
 int ww_server_submit(ww_server *s, uint64_t lo, uint64_t hi, uint64_t *id);
 
 /* Completed requests, oldest first: up to cap into out, count in *n; waits up to

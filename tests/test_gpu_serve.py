@@ -65,3 +65,49 @@ def test_serving_sample_vs_reference_code(gctx):
         c = by_range[(r["lo"], r["hi"])]
         assert (c["prp_count"], c["checksum"], len(c["records"])) == (
             r["prp_count"], r["checksum"], r["cpu_result_count"]), r["index"]
+
+
+def test_serve_admission_bound_by_the_hbm_budget(gctx):
+    """Each admitted request holds its resident sieve state (the pre-sieve words of all its
+    segments, 512 bytes per 8192 integers) in the server's HBM arena until it completes.
+    With 64 slots but a budget for three such requests at a time, the budget, not the
+    slots, holds requests back; every result still equals the oracle's."""
+    w = 1 << 22                   # 512 segments: 256 KiB of words each
+    need = (w // 8192) * 512
+    reqs = [(0.0, lo, lo + w) for lo in range(3 * 10**14 + 1, 3 * 10**14 + 1 + 12 * w, w)]
+    with wwlib.Server(gctx, slots=64, tick_segments=256, hbm_budget=3 * need) as srv:
+        done, ids, _ = wwlib.serve(srv, reqs, realtime=False)
+        st = srv.stats()
+        # a request needing more than the whole budget is refused at once
+        with pytest.raises(wwlib.WWError) as e:
+            srv.submit(10**12, 10**12 + 4 * w)
+        assert e.value.code == wwlib.WW_E_CAPACITY
+    assert st.completed == len(reqs) and st.active == 0 and st.reserved_bytes == 0
+    assert st.peak_active == 3 and st.peak_reserved_bytes == 3 * need
+    # the budget made requests wait: some were admitted only after others had finished
+    spans = sorted((c["t_admit"], c["t_done"]) for c in done.values())
+    assert sum(1 for a, _ in spans if a > min(d for _, d in spans)) >= len(reqs) - 3
+    for rid, (lo, hi) in ids.items():
+        c = done[rid]
+        assert (c["prp_count"], c["checksum"], c["records"]) == oracle.range_results(lo, hi)
+
+
+def test_serve_budget_fragments_and_coalesces(gctx):
+    """Requests of mixed widths against a tight budget: blocks are freed and merged as
+    requests complete, so wide requests behind narrow ones still get admitted."""
+    ws = [1 << 20, 1 << 23, 1 << 21, 1 << 23, 1 << 20, 1 << 22, 1 << 23, 1 << 19]
+    lo = 5 * 10**13 + 1
+    reqs = []
+    for w in ws:
+        reqs.append((0.0, lo, lo + w))
+        lo += w
+    budget = ((1 << 23) // 8192) * 512 * 2   # two of the widest at once
+    with wwlib.Server(gctx, slots=16, tick_segments=512, hbm_budget=budget) as srv:
+        done, ids, _ = wwlib.serve(srv, reqs, realtime=False)
+        st = srv.stats()
+    assert st.completed == len(reqs) and st.reserved_bytes == 0
+    assert st.peak_reserved_bytes <= budget
+    for rid, (a, b) in ids.items():
+        plain = gctx.run_range(a, b)
+        assert (done[rid]["prp_count"], done[rid]["checksum"], done[rid]["records"]) == (
+            plain.prp_count, plain.checksum, plain.results)

@@ -823,6 +823,7 @@ int ww_batch_create(ww_ctx *c, const ww_range *ranges, uint32_t n, uint32_t resu
         spans[i].seg0 = 0;
         spans[i].slot = i;
         spans[i].pad = 0;
+        spans[i].pre_seg = prefix[i]; /* the batch's pre-sieve words follow its layout */
         prefix[i + 1] = prefix[i] + ww::segments_of(r.lo, r.hi);
     }
     ww_batch *b = new ww_batch;

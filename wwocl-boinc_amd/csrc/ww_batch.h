@@ -10,10 +10,14 @@ namespace ww {
 
 /* Segments [seg0, seg0 + count) of the range [lo, hi) of slot `slot`; segment k covers
    the integers [(lo | 1) + 8192 k, (lo | 1) + 8192 (k + 1)) within the range. The count
-   is implied by the prefix array. */
+   is implied by the prefix array. The span's first segment reads its pre-sieve words at
+   block pre_seg of the pre-sieve buffer (64 words per block): the span's position in the
+   launch (its prefix) when the buffer follows the launch's layout, or where the range's
+   resident words start (the serving form). */
 struct BatchSpan {
     uint64_t lo, hi, seg0;
     uint32_t slot, pad;
+    uint64_t pre_seg;
 };
 
 /* Per range slot: checksum and number of tested primes (list entries). */

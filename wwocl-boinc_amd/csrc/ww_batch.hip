@@ -211,7 +211,7 @@ __attribute__((amdgpu_waves_per_eu(WW_BATCH_WAVES_PER_EU))) void k_batch(const B
 
         /* sieve, then compact candidate positions into LDS behind the leftovers */
         u64 bits = sieve_word(g_new, lane * (u32)kIntsPerWord, sp.hi);
-        if (pre) bits &= ~pre[s * kPresieveWordsPerSeg + lane];
+        if (pre) bits &= ~pre[(sp.pre_seg + (s - prefix[a])) * kPresieveWordsPerSeg + lane];
         if (ndeep) {
             /* the end of the candidate buffer is free until the compaction below: it holds
                the deep sieve's pseudoprime classes meanwhile (the leftovers sit at the

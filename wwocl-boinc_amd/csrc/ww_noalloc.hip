@@ -64,6 +64,7 @@ __global__ __launch_bounds__(kSpanThreads) void k_noalloc_spans(const ww_range *
             sp.seg0 = 0;
             sp.slot = i;
             sp.pad = 0;
+            sp.pre_seg = 0; /* no pre-sieve in this launch */
             spans[i] = sp;
         }
         mine += segs[k];

@@ -117,6 +117,7 @@ __global__ void k_set_span(BatchSpan *span, u64 *prefix, u64 lo, u64 hi)
     span->seg0 = 0;
     span->slot = 0;
     span->pad = 0;
+    span->pre_seg = 0;
     prefix[0] = 0;
     prefix[1] = segments_of(lo, hi);
 }

@@ -2,16 +2,21 @@
  * ww_server.cpp -- the serving form: continuous batching of range requests.
  *
  * An engine thread owns a HIP stream and a fixed pool of device slots (per-slot checksum
- * and count accumulate on the device across launches). Requests wait in an admission
- * queue until a slot and their share of the HBM budget are free. Every "tick" the engine
+ * and count accumulate on the device across launches). Each admitted request keeps its
+ * sieve state resident in HBM until it completes: the wide pre-sieve's words for all its
+ * segments (512 bytes per 8192 integers), written once at admission into a block of the
+ * server's arena, which is the HBM budget. Requests wait in an admission queue (FIFO)
+ * until a slot and a free arena block of their size are available, so the budget bounds
+ * how many requests, and how wide ones, are resident at once. Every "tick" the engine
  * takes the next segments of the active requests in admission order, up to the tick
- * size, and runs one persistent k_batch launch over them; up to kRing ticks are in
- * flight (span tables, record buffers and sum snapshots are triple-buffered), so the host
- * builds tick t+1 while tick t runs and requests arriving meanwhile join the next tick.
- * A request is complete when the tick holding its last segment has finished; its
- * checksum and count come from that tick's snapshot of the slot sums, its records from
- * the ticks that produced them. The sieve tables stay resident on the device; the fused
- * kernel keeps candidates and PRPs in LDS, so requests need no PRP buffers in HBM.
+ * size, and runs one persistent k_batch launch over them, each span reading its words
+ * from its request's block; up to kRing ticks are in flight (span tables, record buffers
+ * and sum snapshots are triple-buffered), so the host builds tick t+1 while tick t runs
+ * and requests arriving meanwhile join the next tick. A request is complete when the
+ * tick holding its last segment has finished (ticks retire in order, so no launch still
+ * reads its block, which is then freed); its checksum and count come from that tick's
+ * snapshot of the slot sums, its records from the ticks that produced them. Candidates
+ * and PRPs stay in LDS: requests need no PRP buffers in HBM.
  *
  * No counterpart in the reference (one range per process, main.cpp:1110-1803); per
  * request the results equal the plain form's on that range.
@@ -25,6 +30,7 @@
 #include <cstdio>
 #include <cstring>
 #include <deque>
+#include <map>
 #include <mutex>
 #include <thread>
 #include <unordered_map>
@@ -42,6 +48,7 @@ namespace {
 constexpr int kRing = 3;                    /* ticks in flight */
 constexpr uint32_t kTickRecords = 1u << 16; /* result records per tick (capacity) */
 constexpr uint32_t kTickCopy = 1024;        /* records copied back with every tick */
+constexpr uint64_t kBlockBytes = ww::kPresieveWordsPerSeg * sizeof(uint64_t); /* 512: one segment's words */
 
 double now_s()
 {
@@ -54,6 +61,8 @@ struct Req {
     uint64_t checksum = 0, prp_count = 0;
     double t_submit = 0, t_admit = 0, t_done = 0;
     uint32_t slot = 0;
+    uint64_t blocks = 0;  /* arena blocks reserved (its sieve state + bytes_per_request) */
+    uint64_t pre_off = 0; /* first block of its resident pre-sieve words */
     std::vector<ww_result> recs;
 };
 
@@ -63,7 +72,6 @@ struct Tick {
     BatchState *d_state = nullptr, *h_state = nullptr;
     ww_result *d_res = nullptr, *h_res = nullptr;
     SlotSum *h_sums = nullptr;
-    uint64_t *d_pre = nullptr; /* wide pre-sieve of the tick's segments (or none) */
     hipEvent_t done = nullptr;
     bool busy = false;
     uint64_t segments = 0;
@@ -87,7 +95,15 @@ struct ww_server {
     ww::WidePrime *d_wide = nullptr;
     uint32_t nwide = 0;
     int cus = 0;
-    uint32_t max_active = 0;
+    /* the HBM budget as an arena of 512-byte blocks: resident pre-sieve words of the
+       admitted requests (allocated only when the pre-sieve is on; without it the blocks are
+       only counted) */
+    uint64_t *d_arena = nullptr;
+    uint64_t arena_blocks = 0;
+    std::map<uint64_t, uint64_t> free_blocks; /* first block -> length, coalesced */
+    uint64_t extra_blocks = 0;                /* bytes_per_request in blocks */
+    BatchSpan *d_adm_span = nullptr;          /* one-span table of an admission's pre-sieve */
+    uint64_t *d_adm_prefix = nullptr;
     Tick ticks[kRing];
     int ring_next = 0;
 
@@ -108,6 +124,49 @@ struct ww_server {
 };
 
 namespace {
+
+/* Blocks a request of nseg segments reserves. */
+uint64_t blocks_for(const ww_server *s, uint64_t nseg)
+{
+    return (s->nwide ? nseg : 0) + s->extra_blocks;
+}
+
+/* First fit in the arena; false when no free run is long enough. Caller holds mu. */
+bool arena_take(ww_server *s, uint64_t n, uint64_t *off)
+{
+    if (n == 0) {
+        *off = 0;
+        return true;
+    }
+    for (auto it = s->free_blocks.begin(); it != s->free_blocks.end(); ++it) {
+        if (it->second < n) continue;
+        *off = it->first;
+        const uint64_t rest = it->second - n, at = it->first + n;
+        s->free_blocks.erase(it);
+        if (rest) s->free_blocks.emplace(at, rest);
+        return true;
+    }
+    return false;
+}
+
+/* Return a run to the arena, merged with its free neighbours. Caller holds mu. */
+void arena_give(ww_server *s, uint64_t off, uint64_t n)
+{
+    if (n == 0) return;
+    auto next = s->free_blocks.lower_bound(off);
+    if (next != s->free_blocks.end() && off + n == next->first) {
+        n += next->second;
+        next = s->free_blocks.erase(next);
+    }
+    if (next != s->free_blocks.begin()) {
+        auto prev = std::prev(next);
+        if (prev->first + prev->second == off) {
+            prev->second += n;
+            return;
+        }
+    }
+    s->free_blocks.emplace(off, n);
+}
 
 bool hip_ok(ww_server *s, hipError_t e, const char *what)
 {
@@ -155,6 +214,9 @@ bool retire(ww_server *s, Tick &t)
         s->done.push_back(q.id);
         s->active.erase(std::find(s->active.begin(), s->active.end(), q.id));
         s->free_slots.push_back(slot);
+        /* every tick with its segments has finished (ticks retire in order) */
+        arena_give(s, q.pre_off, q.blocks);
+        s->stats.reserved_bytes -= q.blocks * kBlockBytes;
     }
     s->stats.ticks++;
     s->stats.segments += t.segments;
@@ -170,8 +232,9 @@ void engine_main(ww_server *s)
     if (!hip_ok(s, hipSetDevice(s->device), "hipSetDevice")) return;
     const uint64_t T = s->cfg.tick_segments;
     for (;;) {
-        /* admission: FIFO into free slots within the HBM budget */
+        /* admission: FIFO into free slots and free arena blocks (the HBM budget) */
         std::vector<uint32_t> cleared;
+        std::vector<uint64_t> admitted;
         {
             std::unique_lock<std::mutex> g(s->mu);
             bool inflight = false;
@@ -183,20 +246,46 @@ void engine_main(ww_server *s)
                 s->cv.wait_for(g, std::chrono::milliseconds(2));
                 continue;
             }
-            while (!s->pending.empty() && !s->free_slots.empty() &&
-                   s->active.size() < s->max_active) {
+            while (!s->pending.empty() && !s->free_slots.empty()) {
                 Req &q = s->reqs[s->pending.front()];
+                const uint64_t need = blocks_for(s, q.nseg);
+                if (!arena_take(s, need, &q.pre_off)) break; /* the head waits for HBM */
+                q.blocks = need;
                 s->pending.pop_front();
                 q.slot = s->free_slots.back();
                 s->free_slots.pop_back();
                 q.t_admit = now_s();
                 s->active.push_back(q.id);
                 cleared.push_back(q.slot);
+                if (s->nwide) admitted.push_back(q.id);
+                s->stats.reserved_bytes += need * kBlockBytes;
+                s->stats.peak_reserved_bytes =
+                    std::max(s->stats.peak_reserved_bytes, s->stats.reserved_bytes);
+                s->stats.peak_active = std::max<uint64_t>(s->stats.peak_active, s->active.size());
             }
         }
         for (uint32_t slot : cleared)
             if (!hip_ok(s, hipMemsetAsync(&s->d_sums[slot], 0, sizeof(SlotSum), s->stream), "memset"))
                 return;
+        /* each admitted request's sieve state, once, into its arena block: the one-span
+           table is rewritten per request, in stream order behind the previous pre-sieve */
+        for (uint64_t id : admitted) {
+            uint64_t lo, hi, off, nseg;
+            {
+                std::lock_guard<std::mutex> g(s->mu);
+                const Req &q = s->reqs[id];
+                lo = q.lo;
+                hi = q.hi;
+                off = q.pre_off;
+                nseg = q.nseg;
+            }
+            const int pgrid = (int)std::min<uint64_t>((nseg + 255) / 256, (uint64_t)(s->cus > 0 ? s->cus : 1));
+            if (!hip_ok(s, ww::launch_set_span(lo, hi, s->d_adm_span, s->d_adm_prefix, s->stream), "span") ||
+                !hip_ok(s, ww::launch_presieve(s->d_adm_span, s->d_adm_prefix, 1, s->d_wide, s->nwide,
+                                               s->d_arena + off * ww::kPresieveWordsPerSeg,
+                                               pgrid > 0 ? pgrid : 1, s->stream), "k_presieve"))
+                return;
+        }
 
         Tick &t = s->ticks[s->ring_next];
         if (t.busy && !retire(s, t)) return;
@@ -218,6 +307,7 @@ void engine_main(ww_server *s)
                 sp.seg0 = q.next;
                 sp.slot = q.slot;
                 sp.pad = 0;
+                sp.pre_seg = q.pre_off + q.next; /* its words in the request's block */
                 q.next += take;
                 total += take;
                 t.h_prefix[++nspans] = total;
@@ -241,15 +331,9 @@ void engine_main(ww_server *s)
                                            hipMemcpyHostToDevice, s->stream), "spans") &&
                   hip_ok(s, hipMemcpyAsync(t.d_prefix, t.h_prefix, (nspans + 1) * sizeof(uint64_t),
                                            hipMemcpyHostToDevice, s->stream), "prefix") &&
-                  (!t.d_pre ||
-                   hip_ok(s, ww::launch_presieve(t.d_spans, t.d_prefix, nspans, s->d_wide, s->nwide,
-                                                 t.d_pre,
-                                                 (int)std::min<uint64_t>((total + 255) / 256,
-                                                                         (uint64_t)(s->cus > 0 ? s->cus : 1)),
-                                                 s->stream), "k_presieve")) &&
                   hip_ok(s, ww::launch_batch(t.d_spans, t.d_prefix, nspans, t.d_state, s->d_sums, 0,
                                              t.d_res, kTickRecords, s->grid, s->cfg.claim, s->d_deep,
-                                             s->ndeep, t.d_pre, s->stream), "k_batch") &&
+                                             s->ndeep, s->d_arena, s->stream), "k_batch") &&
                   hip_ok(s, hipMemcpyAsync(t.h_state, t.d_state, sizeof(BatchState),
                                            hipMemcpyDeviceToHost, s->stream), "state") &&
                   hip_ok(s, hipMemcpyAsync(t.h_res, t.d_res, kTickCopy * sizeof(ww_result),
@@ -275,7 +359,6 @@ void free_tick(Tick &t)
     if (t.d_prefix) (void)hipFree(t.d_prefix);
     if (t.d_state) (void)hipFree(t.d_state);
     if (t.d_res) (void)hipFree(t.d_res);
-    if (t.d_pre) (void)hipFree(t.d_pre);
     if (t.h_spans) (void)hipHostFree(t.h_spans);
     if (t.h_prefix) (void)hipHostFree(t.h_prefix);
     if (t.h_state) (void)hipHostFree(t.h_state);
@@ -299,6 +382,9 @@ void destroy(ww_server *s)
     if (s->d_sums) (void)hipFree(s->d_sums);
     if (s->d_deep) (void)hipFree(s->d_deep);
     if (s->d_wide) (void)hipFree(s->d_wide);
+    if (s->d_arena) (void)hipFree(s->d_arena);
+    if (s->d_adm_span) (void)hipFree(s->d_adm_span);
+    if (s->d_adm_prefix) (void)hipFree(s->d_adm_prefix);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
 }
@@ -322,7 +408,7 @@ int ww_server_create(ww_ctx *ctx, const ww_server_config *cfg, ww_server **out)
     s->cfg.slots = 4096;
     s->cfg.tick_segments = 1u << 17;
     s->cfg.hbm_budget = 64ull << 30;
-    s->cfg.bytes_per_request = 1u << 20;
+    s->cfg.bytes_per_request = 0;
     s->cfg.claim = 0;
     s->cfg.blocks_per_cu = -1;
     if (cfg) {
@@ -357,8 +443,9 @@ int ww_server_create(ww_ctx *ctx, const ww_server_config *cfg, ww_server **out)
             s->cfg.hbm_budget > free_b * 9 / 10)
             s->cfg.hbm_budget = free_b * 9 / 10; /* admission never exceeds free HBM */
     }
-    uint64_t by_budget = s->cfg.hbm_budget / s->cfg.bytes_per_request;
-    s->max_active = (uint32_t)std::min<uint64_t>(s->cfg.slots, by_budget ? by_budget : 1);
+    s->arena_blocks = s->cfg.hbm_budget / kBlockBytes;
+    s->extra_blocks = (s->cfg.bytes_per_request + kBlockBytes - 1) / kBlockBytes;
+    if (s->arena_blocks) s->free_blocks.emplace(0, s->arena_blocks);
     for (uint32_t i = s->cfg.slots; i > 0; --i) s->free_slots.push_back(i - 1);
 
     hipError_t e = hipSetDevice(s->device);
@@ -368,6 +455,9 @@ int ww_server_create(ww_ctx *ctx, const ww_server_config *cfg, ww_server **out)
     if (e == hipSuccess && s->nwide) e = hipMalloc(&s->d_wide, s->nwide * sizeof(ww::WidePrime));
     if (e == hipSuccess && s->nwide)
         e = hipMemcpy(s->d_wide, ctx_wide, s->nwide * sizeof(ww::WidePrime), hipMemcpyDeviceToDevice);
+    if (e == hipSuccess && s->nwide) e = hipMalloc(&s->d_arena, s->arena_blocks * kBlockBytes);
+    if (e == hipSuccess && s->nwide) e = hipMalloc(&s->d_adm_span, sizeof(BatchSpan));
+    if (e == hipSuccess && s->nwide) e = hipMalloc(&s->d_adm_prefix, 2 * sizeof(uint64_t));
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMalloc(&s->d_sums, s->cfg.slots * sizeof(SlotSum));
     if (e == hipSuccess) e = hipMemsetAsync(s->d_sums, 0, s->cfg.slots * sizeof(SlotSum), s->stream);
@@ -376,9 +466,6 @@ int ww_server_create(ww_ctx *ctx, const ww_server_config *cfg, ww_server **out)
         if (e == hipSuccess) e = hipMalloc(&t.d_prefix, (s->cfg.slots + 1) * sizeof(uint64_t));
         if (e == hipSuccess) e = hipMalloc(&t.d_state, sizeof(BatchState));
         if (e == hipSuccess) e = hipMalloc(&t.d_res, kTickRecords * sizeof(ww_result));
-        if (e == hipSuccess && s->nwide)
-            e = hipMalloc(&t.d_pre, (uint64_t)s->cfg.tick_segments * ww::kPresieveWordsPerSeg *
-                                        sizeof(uint64_t));
         if (e == hipSuccess) e = hipHostMalloc(&t.h_spans, s->cfg.slots * sizeof(BatchSpan), 0);
         if (e == hipSuccess) e = hipHostMalloc(&t.h_prefix, (s->cfg.slots + 1) * sizeof(uint64_t), 0);
         if (e == hipSuccess) e = hipHostMalloc(&t.h_state, sizeof(BatchState), 0);
@@ -408,6 +495,13 @@ int ww_server_submit(ww_server *s, uint64_t lo, uint64_t hi, uint64_t *id)
     q.lo = lo;
     q.hi = hi;
     q.nseg = ww::segments_of(lo, hi);
+    if (blocks_for(s, q.nseg) > s->arena_blocks)
+        return ww_set_error(WW_E_CAPACITY,
+                            "request [%llu, %llu) needs %llu bytes of resident sieve state, above "
+                            "the HBM budget of %llu bytes",
+                            (unsigned long long)lo, (unsigned long long)hi,
+                            (unsigned long long)(blocks_for(s, q.nseg) * kBlockBytes),
+                            (unsigned long long)(s->arena_blocks * kBlockBytes));
     q.t_submit = now_s();
     s->stats.submitted++;
     if (q.nseg == 0) { /* no odd number in range: complete at once */

@@ -89,7 +89,8 @@ class RequestInfo(ctypes.Structure):
 class ServerStats(ctypes.Structure):
     _fields_ = [("t_start", ctypes.c_double), ("submitted", U64), ("completed", U64),
                 ("tested", U64), ("ticks", U64), ("segments", U64), ("pending", U64),
-                ("active", U64)]
+                ("active", U64), ("reserved_bytes", U64), ("peak_reserved_bytes", U64),
+                ("peak_active", U64)]
 
 
 class DeviceInfo(ctypes.Structure):
