@@ -135,6 +135,13 @@ int ww_get_deep_bound(ww_ctx *ctx, uint32_t *bound);
 int ww_set_presieve_bound(ww_ctx *ctx, uint32_t bound);
 int ww_get_presieve_bound(ww_ctx *ctx, uint32_t *bound);
 
+/* Launch shape of the persistent kernel for launches issued afterwards: waves per
+   workgroup (2, 4 or 8) and the wide pre-sieve's segment, the LDS bitmap one workgroup
+   marks at a time, as a bit count (15..20: 2^15 to 2^20 odd numbers). No counterpart in
+   the reference; results do not depend on either. Defaults 4 and 20. */
+int ww_set_launch_shape(ww_ctx *ctx, uint32_t waves_per_group, uint32_t segment_bits);
+int ww_get_launch_shape(ww_ctx *ctx, uint32_t *waves_per_group, uint32_t *segment_bits);
+
 /* clearresult (clearresult.cl:12-26): zero checksum, resultcount, totalprimecount
    and primecount_max. Stream-ordered. */
 int ww_clear_results(ww_ctx *ctx);
@@ -304,26 +311,55 @@ int ww_batch_destroy(ww_batch *b);
 
 /* One configuration of the persistent form and its measured speed. */
 typedef struct {
-    uint32_t deep_bound;     /* ww_set_deep_bound */
-    int32_t blocks_per_cu;   /* ww_batch_set_grid; 0 = as many as fit */
-    uint32_t claim;          /* largest segments per claim */
-    uint32_t presieve_bound; /* ww_set_presieve_bound */
-    double ms;               /* kernel time of the sample (pre-sieve + search) */
-    double prps_per_s;       /* tested primes per second on the sample */
+    uint32_t deep_bound;      /* ww_set_deep_bound */
+    int32_t blocks_per_cu;    /* ww_batch_set_grid; 0 = as many as fit */
+    uint32_t claim;           /* largest segments per claim */
+    uint32_t presieve_bound;  /* ww_set_presieve_bound */
+    double ms;                /* kernel time of the sample (pre-sieve + search) */
+    double prps_per_s;        /* tested primes per second on the sample */
+    uint32_t waves_per_group; /* ww_set_launch_shape */
+    uint32_t segment_bits;    /* ww_set_launch_shape */
 } ww_tuning;
 
+/* The autotuner's search space; a null list (or count 0) takes the built-in one:
+   deep bounds {0, 1024, 2048, 4096, 8192, 16384, 65535}, workgroups per CU {0, 4, 5}
+   (scaled to the shape: as many as fit, and 4 or 5 waves per SIMD), claims {1, 4, 16, 64},
+   pre-sieve bounds {0, 2^18, 2^19, 2^20, 2^21}, waves per workgroup {2, 4, 8}, segment
+   bits {15, ..., 20}. */
+typedef struct {
+    const uint32_t *deep_bounds;
+    uint32_t n_deep_bounds;
+    const int32_t *blocks_per_cu;
+    uint32_t n_blocks_per_cu;
+    const uint32_t *claims;
+    uint32_t n_claims;
+    const uint32_t *presieve_bounds;
+    uint32_t n_presieve_bounds;
+    const uint32_t *waves_per_group;
+    uint32_t n_waves_per_group;
+    const uint32_t *segment_bits;
+    uint32_t n_segment_bits;
+} ww_tune_space;
+
 /* Autotuner replacing the reference's work-size profiling (profileGPU,
-   main.cpp:959-1107): times the persistent form on the sample range [lo, hi) for every
-   combination of the given deep-sieve bounds, workgroups per CU and claim sizes at the
-   context's pre-sieve bound, then each given pre-sieve bound at the best of those (null
-   lists = built-in defaults). Keeps the fastest as this context's deep and pre-sieve
-   bounds and as the default geometry of batches created afterwards, and returns it in
-   *best. `trials` (may be null) receives up to cap measured configurations; *ntrials
-   their count. Results do not depend on any of these settings. */
-int ww_autotune(ww_ctx *ctx, uint64_t lo, uint64_t hi, const uint32_t *bounds, uint32_t nb,
-                const int32_t *blocks, uint32_t nbl, const uint32_t *claims, uint32_t nc,
-                const uint32_t *presieve, uint32_t np, ww_tuning *best, ww_tuning *trials,
-                uint32_t cap, uint32_t *ntrials);
+   main.cpp:959-1107): times the persistent form (pre-sieve + search kernel) on the sample
+   range [lo, hi) in stages, each at the best of the ones before: deep-sieve bound x
+   workgroups per CU x claim size; the work-group shape (waves per workgroup, with its
+   workgroups per CU); the pre-sieve bound; the pre-sieve's segment size. space may be
+   null. Keeps the fastest as this context's bounds, launch shape and the default geometry
+   of batches created afterwards, and returns it in *best. `trials` (may be null) receives
+   up to cap measured configurations; *ntrials their count. Results do not depend on any
+   of these settings. */
+int ww_autotune(ww_ctx *ctx, uint64_t lo, uint64_t hi, const ww_tune_space *space,
+                ww_tuning *best, ww_tuning *trials, uint32_t cap, uint32_t *ntrials);
+
+/* The context's tuned configuration (bounds, geometry, launch shape) as a small JSON file
+   naming the device (marketing name, target, compute units), and back: ww_tuning_load
+   applies a file written on the same kind of device and refuses (WW_E_ARG) one from
+   another, or a malformed one. A tuning cache: the drop-in loads it at start instead of
+   tuning again. No counterpart in the reference. */
+int ww_tuning_save(ww_ctx *ctx, const char *path);
+int ww_tuning_load(ww_ctx *ctx, const char *path, ww_tuning *applied);
 
 /* ---------------------------------------------------------------------------
  * Serving form: continuous batching of range requests. No counterpart in the

@@ -106,26 +106,113 @@ def test_batch_single_range_equals_plain_at_full_size(gctx):
         assert abs(a + s - total) <= 0.05 + 0.01 * total
 
 
-def test_autotune_keeps_results(gctx):
+def _restore_defaults(ctx):
+    ctx.set_deep_bound(wwlib.WW_DEFAULT_DEEP_BOUND)
+    ctx.set_presieve_bound(wwlib.WW_DEFAULT_PRESIEVE_BOUND)
+    ctx.set_launch_shape(4, 20)
+
+
+def test_autotune_stages_and_tuned_results_vs_oracle(gctx):
+    """The tuner's stages (deep bound x geometry, work-group shape, pre-sieve bound,
+    pre-sieve segment size) each measured, the fastest kept as the context's settings, and
+    the tuned configuration's results equal to the oracle's in the persistent, fused and
+    plain forms."""
     lo, hi = (1 << 62) + 12345, (1 << 62) + 12345 + (1 << 24)
-    plain = gctx.run_range(lo, hi)
+    want = oracle.range_results(lo, hi)
     try:
         pre0 = gctx.presieve_bound()
         best, trials = gctx.autotune(lo, hi, bounds=[0, 4096], blocks=[0, 4], claims=[1, 16],
-                                     presieve=[0, pre0, 1 << 19])
-        assert len(trials) == 8 + (2 if pre0 not in (0, 1 << 19) else 1)
-        assert all(t["presieve_bound"] == pre0 for t in trials[:8])
-        assert {t["presieve_bound"] for t in trials[8:]} == {0, 1 << 19} - {pre0}
+                                     presieve=[0, pre0, 1 << 19], waves=[2, 4, 8],
+                                     segment_bits=[16, 20])
+        assert len(trials[:8]) == 8 and all(t["waves_per_group"] == 4 for t in trials[:8])
+        assert {t["waves_per_group"] for t in trials} == {2, 4, 8}
+        # 8-wave groups: as many as fit, and 4 * 4 / 8 = 2 per CU
+        assert {t["blocks_per_cu"] for t in trials if t["waves_per_group"] == 8} == {0, 2}
+        assert {t["presieve_bound"] for t in trials} == {0, pre0, 1 << 19}
+        if best["presieve_bound"]:
+            assert {16, 20} <= {t["segment_bits"] for t in trials}
         assert best["ms"] == min(t["ms"] for t in trials)
         assert gctx.deep_bound() == best["deep_bound"]
         assert gctx.presieve_bound() == best["presieve_bound"]
+        assert gctx.launch_shape() == (best["waves_per_group"], best["segment_bits"])
         with wwlib.Batch(gctx, [(lo, hi)]) as b:
             b.launch()
-            (prp, cks, recs), = b.read()
-        assert (prp, cks, recs) == (plain.prp_count, plain.checksum, plain.results)
+            assert b.read() == [want]
+        gctx.clear_results()
+        gctx.search_fused(lo, hi)
+        c = gctx.counters()
+        assert (c.totalprimecount, c.checksum) == want[:2]
+        got = gctx.run_range(lo, hi)
+        assert (got.prp_count, got.checksum, got.results) == want
     finally:
-        gctx.set_deep_bound(wwlib.WW_DEFAULT_DEEP_BOUND)
-        gctx.set_presieve_bound(wwlib.WW_DEFAULT_PRESIEVE_BOUND)
+        _restore_defaults(gctx)
+
+
+@pytest.mark.parametrize("waves,bits", [(2, 15), (8, 17), (2, 20), (8, 19)])
+def test_launch_shapes_vs_oracle(gctx, waves, bits):
+    """Every work-group shape and pre-sieve segment size gives the oracle's results, on the
+    edge ranges and a batch of them."""
+    from test_gpu_parity import RANGES
+    try:
+        gctx.set_launch_shape(waves, bits)
+        assert gctx.launch_shape() == (waves, bits)
+        with wwlib.Batch(gctx, RANGES) as b:
+            b.launch(graph=True)
+            got = b.read()
+        for (lo, hi), g in zip(RANGES, got):
+            assert g == oracle.range_results(lo, hi), (lo, hi)
+        lo, hi = (1 << 63) - (1 << 23), (1 << 63) + (1 << 23)  # pre-sieved chunks of any size
+        gctx.clear_results()
+        gctx.search_fused(lo, hi)
+        c = gctx.counters()
+        want = oracle.range_results(lo, hi)
+        assert (c.totalprimecount, c.checksum) == want[:2]
+    finally:
+        _restore_defaults(gctx)
+
+
+def test_launch_shape_arguments_checked(gctx):
+    for w, bits in [(3, 20), (16, 20), (4, 14), (4, 21)]:
+        with pytest.raises(wwlib.WWError) as e:
+            gctx.set_launch_shape(w, bits)
+        assert e.value.code == wwlib.WW_E_ARG
+    assert gctx.launch_shape() == (4, 20)
+
+
+def test_tuning_cache_round_trip(gctx, tmp_path):
+    """ww_tuning_save / ww_tuning_load: a saved configuration applies on a fresh context of
+    the same device; files from another device or malformed ones are refused."""
+    path = tmp_path / "tune.json"
+    try:
+        gctx.set_deep_bound(4096)
+        gctx.set_presieve_bound(1 << 19)
+        gctx.set_launch_shape(8, 18)
+        gctx.tuning_save(path)
+        text = path.read_text()
+        assert '"waves_per_group": 8' in text and '"segment_bits": 18' in text
+        with wwlib.Context(0, list_capacity=1 << 16) as ctx:
+            t = ctx.tuning_load(path)
+            assert (t["deep_bound"], t["presieve_bound"], t["waves_per_group"],
+                    t["segment_bits"]) == (4096, 1 << 19, 8, 18)
+            assert ctx.deep_bound() == 4096 and ctx.presieve_bound() == 1 << 19
+            assert ctx.launch_shape() == (8, 18)
+            lo, hi = 10**15, 10**15 + (1 << 22)
+            with wwlib.Batch(ctx, [(lo, hi)]) as b:
+                b.launch()
+                assert b.read() == [oracle.range_results(lo, hi)]
+            other = tmp_path / "other.json"
+            other.write_text(text.replace('"compute_units": ', '"compute_units": 1'))
+            with pytest.raises(wwlib.WWError) as e:
+                ctx.tuning_load(other)
+            assert e.value.code == wwlib.WW_E_ARG and b"not on this" in wwlib.load().ww_last_error()
+            bad = tmp_path / "bad.json"
+            bad.write_text(text.replace('"segment_bits": 18', '"segment_bits": 99'))
+            with pytest.raises(wwlib.WWError):
+                ctx.tuning_load(bad)
+            with pytest.raises(wwlib.WWError):
+                ctx.tuning_load(tmp_path / "missing.json")
+    finally:
+        _restore_defaults(gctx)
 
 
 def test_batch_workload_sample_vs_reference_code(gctx):

@@ -12,9 +12,13 @@ import wwlib
 EXE = wwlib.HOST_EXE
 
 
-def run(tmp_path, *args, timeout=600):
+def run(tmp_path, *args, timeout=600, env=None):
+    # the tuning cache stays in the test's directory unless a test names one
+    e = dict(os.environ, WW_TUNE_CACHE=str(tmp_path / "no-cache.json"))
+    if env:
+        e.update(env)
     r = subprocess.run([EXE, *map(str, args)], cwd=tmp_path, capture_output=True, text=True,
-                       timeout=timeout)
+                       timeout=timeout, env=e)
     return r.returncode, r.stdout, r.stderr
 
 
@@ -226,3 +230,25 @@ def test_host_kernel_time_targets(tmp_path, flags, target):
         assert "GPU persistent kernel test passed." in out
     _, cks, _ = oracle.range_results(lo, hi)
     assert results(tmp_path).splitlines() == expected_lines(lo, hi) + [f"{cks:016X}"]
+
+
+@pytest.mark.gpu
+def test_host_tune_writes_and_later_runs_load_the_cache(tmp_path):
+    """-tune runs the library's autotuner and saves the per-device cache; the next start
+    loads it. The answer is the same either way."""
+    cache = tmp_path / "tuning.json"
+    lo, hi = 10**12, 10**12 + 4_000_000
+    _, cks, _ = oracle.range_results(lo, hi)
+    want = expected_lines(lo, hi) + [f"{cks:016X}"]
+    rc, out, err = run(tmp_path, lo, hi, "-tune", env={"WW_TUNE_CACHE": str(cache)})
+    assert rc == 0, out + err
+    assert "Tuned (" in out and f"Tuning saved to {cache}" in err
+    assert cache.exists() and '"segment_bits"' in cache.read_text()
+    assert results(tmp_path).splitlines() == want
+    for f in ("stateA-WW.txt", "stateB-WW.txt"):
+        if (tmp_path / f).exists():
+            (tmp_path / f).unlink()
+    rc, out, err = run(tmp_path, lo, hi, env={"WW_TUNE_CACHE": str(cache)})
+    assert rc == 0, out + err
+    assert f"Tuning loaded from {cache}" in err
+    assert results(tmp_path).splitlines() == want

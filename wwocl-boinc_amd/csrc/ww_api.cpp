@@ -11,8 +11,11 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
+#include <string>
 #include <vector>
 
 #include "../../include/ww_api.h"
@@ -62,6 +65,11 @@ struct ww_ctx {
     ww_result *d_fres = nullptr;
     int tuned_blocks = 0;      /* defaults for batches created from now on */
     uint32_t tuned_claim = 8;
+    /* launch shape for launches issued from now on: waves per workgroup of the persistent
+       kernel, and the pre-sieve's segment (in 8192-integer segments) */
+    int waves = ww::kDefaultBatchWaves;
+    uint32_t presieve_chunk = ww::kPresieveMaxChunkSegs;
+    ww_tuning last_tuning{}; /* the tuner's (or a loaded cache's) best, for ww_tuning_save */
     uint64_t default_chunk = 0;
     /* event pairs of launches whose durations have not been collected yet */
     std::vector<hipEvent_t> ev_sieve, ev_tests;
@@ -191,9 +199,11 @@ int chunk_presieve(ww_ctx *c, uint64_t low, uint64_t high, const uint64_t **pre)
             return fail(WW_E_HIP, "pre-sieve buffer: %s", hipGetErrorString(e));
         c->pre_plain_segs = nseg;
     }
-    const int pgrid = (int)std::min<uint64_t>((nseg + 255) / 256, (uint64_t)c->cus);
+    const int pgrid = (int)std::min<uint64_t>((nseg + c->presieve_chunk - 1) / c->presieve_chunk,
+                                              (uint64_t)c->cus);
     if ((e = ww::launch_presieve_range(low, high, c->d_span1, c->d_prefix1, c->d_wide + wfirst, wcount,
-                                       c->d_pre_plain, pgrid > 0 ? pgrid : 1, c->stream)) != hipSuccess)
+                                       c->d_pre_plain, pgrid > 0 ? pgrid : 1, c->stream,
+                                       c->presieve_chunk)) != hipSuccess)
         return fail(WW_E_HIP, "pre-sieve: %s", hipGetErrorString(e));
     *pre = c->d_pre_plain;
     return WW_OK;
@@ -213,7 +223,8 @@ int ww_set_error(int code, const char *fmt, ...)
 /* Context internals the serving engine needs (same library). */
 int ww_ctx_internals(ww_ctx *c, int *device, const ww::DeepPrime **deep, uint32_t *ndeep,
                      int *cus, int *tuned_blocks, uint32_t *tuned_claim,
-                     const ww::WidePrime **wide, uint32_t *nwide)
+                     const ww::WidePrime **wide, uint32_t *nwide, int *waves,
+                     uint32_t *presieve_chunk)
 {
     WW_LOCK(c);
     if (!c) return fail(WW_E_ARG, "null context");
@@ -227,6 +238,8 @@ int ww_ctx_internals(ww_ctx *c, int *device, const ww::DeepPrime **deep, uint32_
     *cus = c->cus;
     *tuned_blocks = c->tuned_blocks;
     *tuned_claim = c->tuned_claim;
+    *waves = c->waves;
+    *presieve_chunk = c->presieve_chunk;
     return WW_OK;
 }
 
@@ -436,6 +449,28 @@ int ww_get_presieve_bound(ww_ctx *c, uint32_t *bound)
     return WW_OK;
 }
 
+int ww_set_launch_shape(ww_ctx *c, uint32_t waves_per_group, uint32_t segment_bits)
+{
+    WW_LOCK(c);
+    if (!c) return fail(WW_E_ARG, "null context");
+    if (!ww::batch_waves_ok((int)waves_per_group))
+        return fail(WW_E_ARG, "waves per workgroup %u: 2, 4 or 8", waves_per_group);
+    if (segment_bits < 15 || segment_bits > 20)
+        return fail(WW_E_ARG, "segment bits %u outside 15..20", segment_bits);
+    c->waves = (int)waves_per_group;
+    c->presieve_chunk = 1u << (segment_bits - 12); /* 4096 odd numbers per segment */
+    return WW_OK;
+}
+
+int ww_get_launch_shape(ww_ctx *c, uint32_t *waves_per_group, uint32_t *segment_bits)
+{
+    WW_LOCK(c);
+    if (!c) return fail(WW_E_ARG, "null context");
+    if (waves_per_group) *waves_per_group = (uint32_t)c->waves;
+    if (segment_bits) *segment_bits = 12 + (uint32_t)__builtin_ctz(c->presieve_chunk);
+    return WW_OK;
+}
+
 int ww_clear_results(ww_ctx *c)
 {
     WW_LOCK(c);
@@ -542,11 +577,11 @@ int ww_search_fused(ww_ctx *c, uint64_t low, uint64_t high)
     HIP_TRY(hipEventRecord(a, c->stream));
     const uint64_t *pre = nullptr;
     if ((rc = chunk_presieve(c, low, high, &pre))) return rc;
-    int nb = c->tuned_blocks > 0 ? c->tuned_blocks : ww::batch_blocks_per_cu();
+    int nb = c->tuned_blocks > 0 ? c->tuned_blocks : ww::batch_blocks_per_cu(c->waves);
     if (nb <= 0) nb = 4;
     HIP_TRY(ww::launch_batch(c->d_span1, c->d_prefix1, 1, c->d_fstate, c->d_fsum, 1, c->d_fres,
                              c->result_cap, c->cus * nb, c->tuned_claim, c->d_deep, c->ndeep, pre,
-                             c->stream));
+                             c->stream, c->waves));
     HIP_TRY(ww::launch_fold(c->d_fstate, c->d_fsum, c->d_fres, c->result_cap, c->d_state,
                             c->d_results, c->result_cap, c->stream));
     HIP_TRY(hipEventRecord(b, c->stream));
@@ -567,11 +602,11 @@ int ww_search_ranges_noalloc(ww_ctx *c, const ww_range *d_ranges, uint32_t n,
                     WW_NOALLOC_MAX_RANGES);
     if (!d_ranges || !d_sums || !d_record_count || (record_cap && !d_records))
         return fail(WW_E_ARG, "ww_search_ranges_noalloc: null buffer");
-    int nb = c->tuned_blocks > 0 ? c->tuned_blocks : ww::batch_blocks_per_cu();
+    int nb = c->tuned_blocks > 0 ? c->tuned_blocks : ww::batch_blocks_per_cu(c->waves);
     if (nb <= 0) nb = 4;
     HIP_TRY(ww::launch_noalloc(c->device, d_ranges, n, reinterpret_cast<SlotSum *>(d_sums),
                                d_records, record_cap, d_record_count, c->cus * nb, c->tuned_claim,
-                               c->d_deep, c->ndeep, c->stream));
+                               c->d_deep, c->ndeep, c->stream, c->waves));
     return WW_OK;
 }
 
@@ -728,6 +763,8 @@ struct ww_batch {
     hipGraph_t graph = nullptr, graph_pre = nullptr;
     hipGraphExec_t exec = nullptr, exec_pre = nullptr;
     int graph_grid = 0;
+    int graph_waves = 0;
+    uint32_t graph_chunk = 0;
     uint32_t graph_claim = 0;
     uint32_t graph_ndeep = 0;
     uint64_t graph_wide = 0;
@@ -739,7 +776,7 @@ namespace {
 
 int batch_grid(ww_ctx *c, ww_batch *b)
 {
-    int nb = b->blocks_per_cu > 0 ? b->blocks_per_cu : ww::batch_blocks_per_cu();
+    int nb = b->blocks_per_cu > 0 ? b->blocks_per_cu : ww::batch_blocks_per_cu(c->waves);
     if (nb <= 0) nb = 4;
     return c->cus * nb;
 }
@@ -757,10 +794,10 @@ int batch_enqueue_presieve(ww_ctx *c, ww_batch *b)
 {
     uint32_t first, count;
     wide_span(c, &first, &count);
-    const uint64_t chunks = (b->nseg + 255) / 256;
+    const uint64_t chunks = (b->nseg + c->presieve_chunk - 1) / c->presieve_chunk;
     const int pgrid = (int)std::min<uint64_t>(chunks, (uint64_t)c->cus);
     HIP_TRY(ww::launch_presieve(b->d_spans, b->d_prefix, b->n, c->d_wide + first, count, b->d_pre,
-                                pgrid > 0 ? pgrid : 1, c->stream));
+                                pgrid > 0 ? pgrid : 1, c->stream, c->presieve_chunk));
     return WW_OK;
 }
 
@@ -768,7 +805,7 @@ int batch_enqueue_search(ww_ctx *c, ww_batch *b, int grid, bool pre)
 {
     HIP_TRY(ww::launch_batch(b->d_spans, b->d_prefix, b->n, b->d_state, b->d_sums, b->n, b->d_res,
                              b->res_cap, grid, b->claim, c->d_deep, c->ndeep,
-                             pre ? b->d_pre : nullptr, c->stream));
+                             pre ? b->d_pre : nullptr, c->stream, c->waves));
     return WW_OK;
 }
 
@@ -894,7 +931,8 @@ int ww_batch_launch_graph(ww_ctx *c, ww_batch *b)
     const bool pre = batch_presieves(c, b);
     if (b->exec && (b->graph_grid != grid || b->graph_claim != b->claim ||
                     b->graph_ndeep != c->ndeep || b->graph_wide != wide ||
-                    b->graph_wide_table != c->d_wide))
+                    b->graph_wide_table != c->d_wide || b->graph_waves != c->waves ||
+                    b->graph_chunk != c->presieve_chunk))
         drop_graphs(b);
     if (!b->exec) {
         if (pre && (rc = capture(c, &b->graph_pre, &b->exec_pre,
@@ -908,6 +946,8 @@ int ww_batch_launch_graph(ww_ctx *c, ww_batch *b)
             return rc;
         }
         b->graph_grid = grid;
+        b->graph_waves = c->waves;
+        b->graph_chunk = c->presieve_chunk;
         b->graph_claim = b->claim;
         b->graph_ndeep = c->ndeep;
         b->graph_wide = wide;
@@ -997,10 +1037,8 @@ int ww_batch_destroy(ww_batch *b)
     return WW_OK;
 }
 
-int ww_autotune(ww_ctx *c, uint64_t lo, uint64_t hi, const uint32_t *bounds, uint32_t nb,
-                const int32_t *blocks, uint32_t nbl, const uint32_t *claims, uint32_t nc,
-                const uint32_t *presieve, uint32_t np, ww_tuning *best, ww_tuning *trials,
-                uint32_t cap, uint32_t *ntrials)
+int ww_autotune(ww_ctx *c, uint64_t lo, uint64_t hi, const ww_tune_space *space,
+                ww_tuning *best, ww_tuning *trials, uint32_t cap, uint32_t *ntrials)
 {
     WW_LOCK(c);
     int rc = use(c);
@@ -1009,44 +1047,61 @@ int ww_autotune(ww_ctx *c, uint64_t lo, uint64_t hi, const uint32_t *bounds, uin
     static const int32_t kBlocks[] = {0, 4, 5};
     static const uint32_t kClaims[] = {1, 4, 16, 64};
     static const uint32_t kPresieve[] = {0, 1u << 18, 1u << 19, 1u << 20, 1u << 21};
-    if (!bounds || !nb) {
-        bounds = kBounds;
-        nb = sizeof kBounds / sizeof *kBounds;
+    static const uint32_t kWaves[] = {2, 4, 8};
+    static const uint32_t kSegBits[] = {15, 16, 17, 18, 19, 20};
+    ww_tune_space sp{};
+    if (space) sp = *space;
+#define WW_DEFAULT_LIST(f, n, d)                                                         \
+    if (!sp.f || !sp.n) {                                                                \
+        sp.f = d;                                                                        \
+        sp.n = sizeof d / sizeof *d;                                                     \
     }
-    if (!blocks || !nbl) {
-        blocks = kBlocks;
-        nbl = sizeof kBlocks / sizeof *kBlocks;
-    }
-    if (!claims || !nc) {
-        claims = kClaims;
-        nc = sizeof kClaims / sizeof *kClaims;
-    }
-    if (!presieve || !np) {
-        presieve = kPresieve;
-        np = sizeof kPresieve / sizeof *kPresieve;
-    }
-    for (uint32_t i = 0; i < np; ++i)
-        if (presieve[i] > WW_MAX_PRESIEVE_BOUND)
-            return fail(WW_E_ARG, "pre-sieve bound %u above %u", presieve[i], WW_MAX_PRESIEVE_BOUND);
+    WW_DEFAULT_LIST(deep_bounds, n_deep_bounds, kBounds)
+    WW_DEFAULT_LIST(blocks_per_cu, n_blocks_per_cu, kBlocks)
+    WW_DEFAULT_LIST(claims, n_claims, kClaims)
+    WW_DEFAULT_LIST(presieve_bounds, n_presieve_bounds, kPresieve)
+    WW_DEFAULT_LIST(waves_per_group, n_waves_per_group, kWaves)
+    WW_DEFAULT_LIST(segment_bits, n_segment_bits, kSegBits)
+#undef WW_DEFAULT_LIST
+    for (uint32_t i = 0; i < sp.n_presieve_bounds; ++i)
+        if (sp.presieve_bounds[i] > WW_MAX_PRESIEVE_BOUND)
+            return fail(WW_E_ARG, "pre-sieve bound %u above %u", sp.presieve_bounds[i], WW_MAX_PRESIEVE_BOUND);
+    for (uint32_t i = 0; i < sp.n_waves_per_group; ++i)
+        if (!ww::batch_waves_ok((int)sp.waves_per_group[i]))
+            return fail(WW_E_ARG, "waves per workgroup %u: 2, 4 or 8", sp.waves_per_group[i]);
+    for (uint32_t i = 0; i < sp.n_segment_bits; ++i)
+        if (sp.segment_bits[i] < 15 || sp.segment_bits[i] > 20)
+            return fail(WW_E_ARG, "segment bits %u outside 15..20", sp.segment_bits[i]);
     const uint32_t saved_bound = c->deep_bound, saved_pre = c->presieve_bound;
+    const int saved_waves = c->waves;
+    const uint32_t saved_chunk = c->presieve_chunk;
+    auto restore = [&] {
+        ww_set_deep_bound(c, saved_bound);
+        ww_set_presieve_bound(c, saved_pre);
+        c->waves = saved_waves;
+        c->presieve_chunk = saved_chunk;
+    };
     /* the sample batch holds pre-sieve words only if the pre-sieve is on when it is made */
     if (!saved_pre && (rc = ww_set_presieve_bound(c, WW_DEFAULT_PRESIEVE_BOUND))) return rc;
     ww_range r = {lo, hi};
     ww_batch *b = nullptr;
     if ((rc = ww_batch_create(c, &r, 1, 1u << 16, &b))) {
-        ww_set_presieve_bound(c, saved_pre);
+        restore();
         return rc;
     }
     if ((rc = ww_set_presieve_bound(c, saved_pre))) {
         ww_batch_destroy(b);
+        restore();
         return rc;
     }
+    const uint32_t seg_bits0 = 12 + (uint32_t)__builtin_ctz(c->presieve_chunk);
     ww_tuning top;
     memset(&top, 0, sizeof top);
     top.ms = 1e300;
     uint32_t n = 0;
-    /* one timed configuration (the first launch warms) */
-    auto trial = [&](uint32_t deep, int32_t bpc, uint32_t claim) -> int {
+    /* one timed configuration of the context's current bounds and shape (the first launch
+       warms up) */
+    auto trial = [&](int32_t bpc, uint32_t claim) -> int {
         int e;
         if ((e = ww_batch_set_grid(b, bpc, claim))) return e;
         double ms = 0, t = 0;
@@ -1058,43 +1113,193 @@ int ww_autotune(ww_ctx *c, uint64_t lo, uint64_t hi, const uint32_t *bounds, uin
         }
         if ((e = ww_batch_read(c, b, &sum, nullptr, 0, nullptr, nullptr))) return e;
         ww_tuning tr;
-        tr.deep_bound = deep;
+        memset(&tr, 0, sizeof tr);
+        tr.deep_bound = c->deep_bound;
         tr.blocks_per_cu = bpc;
         tr.claim = claim;
         tr.presieve_bound = c->presieve_bound;
         tr.ms = ms;
         tr.prps_per_s = ms > 0 ? sum.prp_count / (ms * 1e-3) : 0;
+        tr.waves_per_group = (uint32_t)c->waves;
+        tr.segment_bits = 12 + (uint32_t)__builtin_ctz(c->presieve_chunk);
         if (trials && n < cap) trials[n] = tr;
         ++n;
         if (ms < top.ms) top = tr;
         return WW_OK;
     };
-    /* deep bound x geometry at the context's pre-sieve bound, then the pre-sieve bound at
-       the best of those */
-    for (uint32_t i = 0; i < nb && rc == WW_OK; ++i) {
-        if ((rc = ww_set_deep_bound(c, bounds[i]))) break;
-        for (uint32_t j = 0; j < nbl && rc == WW_OK; ++j)
-            for (uint32_t k = 0; k < nc && rc == WW_OK; ++k) rc = trial(bounds[i], blocks[j], claims[k]);
+    auto apply = [&](const ww_tuning &t) -> int {
+        int e;
+        if ((e = ww_set_deep_bound(c, t.deep_bound))) return e;
+        if ((e = ww_set_presieve_bound(c, t.presieve_bound))) return e;
+        return ww_set_launch_shape(c, t.waves_per_group, t.segment_bits);
+    };
+    /* stage 1: deep bound x geometry at the context's shape and pre-sieve bound */
+    for (uint32_t i = 0; i < sp.n_deep_bounds && rc == WW_OK; ++i) {
+        if ((rc = ww_set_deep_bound(c, sp.deep_bounds[i]))) break;
+        for (uint32_t j = 0; j < sp.n_blocks_per_cu && rc == WW_OK; ++j)
+            for (uint32_t k = 0; k < sp.n_claims && rc == WW_OK; ++k)
+                rc = trial(sp.blocks_per_cu[j], sp.claims[k]);
     }
-    if (rc == WW_OK) rc = ww_set_deep_bound(c, top.deep_bound);
-    const ww_tuning stage1 = top;
-    for (uint32_t i = 0; i < np && rc == WW_OK; ++i) {
-        if (presieve[i] == stage1.presieve_bound) continue; /* measured already */
-        if ((rc = ww_set_presieve_bound(c, presieve[i]))) break;
-        rc = trial(stage1.deep_bound, stage1.blocks_per_cu, stage1.claim);
+    /* stage 2: work-group shape; workgroups per CU scaled to it (the listed counts are for
+       four-wave workgroups: b 4-wave groups hold as many waves as 4b/w groups of w waves) */
+    if (rc == WW_OK) rc = apply(top);
+    const ww_tuning s1 = top;
+    for (uint32_t i = 0; i < sp.n_waves_per_group && rc == WW_OK; ++i) {
+        const uint32_t w = sp.waves_per_group[i];
+        if ((rc = ww_set_launch_shape(c, w, seg_bits0))) break;
+        std::vector<int32_t> seen;
+        for (uint32_t j = 0; j < sp.n_blocks_per_cu && rc == WW_OK; ++j) {
+            const int32_t b4 = sp.blocks_per_cu[j];
+            if (b4 && (4 * b4) % (int32_t)w) continue; /* no whole number of groups */
+            const int32_t bw = b4 ? 4 * b4 / (int32_t)w : 0;
+            if (std::find(seen.begin(), seen.end(), bw) != seen.end()) continue;
+            seen.push_back(bw);
+            if ((int)w == s1.waves_per_group && bw == s1.blocks_per_cu) continue; /* measured */
+            rc = trial(bw, s1.claim);
+        }
+    }
+    /* stage 3: pre-sieve bound; stage 4: the pre-sieve's segment size */
+    if (rc == WW_OK) rc = apply(top);
+    const ww_tuning s2 = top;
+    for (uint32_t i = 0; i < sp.n_presieve_bounds && rc == WW_OK; ++i) {
+        if (sp.presieve_bounds[i] == s2.presieve_bound) continue;
+        if ((rc = ww_set_presieve_bound(c, sp.presieve_bounds[i]))) break;
+        rc = trial(s2.blocks_per_cu, s2.claim);
+    }
+    if (rc == WW_OK) rc = apply(top);
+    const ww_tuning s3 = top;
+    for (uint32_t i = 0; i < sp.n_segment_bits && rc == WW_OK && s3.presieve_bound; ++i) {
+        if (sp.segment_bits[i] == s3.segment_bits) continue;
+        if ((rc = ww_set_launch_shape(c, s3.waves_per_group, sp.segment_bits[i]))) break;
+        rc = trial(s3.blocks_per_cu, s3.claim);
     }
     ww_batch_destroy(b);
+    if (rc == WW_OK) rc = apply(top);
     if (rc) {
-        ww_set_deep_bound(c, saved_bound);
-        ww_set_presieve_bound(c, saved_pre);
+        restore();
         return rc;
     }
-    ww_set_deep_bound(c, top.deep_bound);
-    ww_set_presieve_bound(c, top.presieve_bound);
     c->tuned_blocks = top.blocks_per_cu;
     c->tuned_claim = top.claim;
+    c->last_tuning = top;
     if (best) *best = top;
     if (ntrials) *ntrials = n;
+    return WW_OK;
+}
+
+int ww_tuning_save(ww_ctx *c, const char *path)
+{
+    WW_LOCK(c);
+    int rc = use(c);
+    if (rc) return rc;
+    if (!path) return fail(WW_E_ARG, "null path");
+    ww_device_info_t info;
+    if ((rc = ww_device_info(c->device, &info))) return rc;
+    FILE *f = fopen(path, "w");
+    if (!f) return fail(WW_E_ARG, "cannot write %s", path);
+    uint32_t waves = 0, bits = 0;
+    ww_get_launch_shape(c, &waves, &bits);
+    const int n = fprintf(f,
+                          "{\n  \"device\": \"%s\",\n  \"arch\": \"%s\",\n  \"compute_units\": %d,\n"
+                          "  \"deep_bound\": %u,\n  \"blocks_per_cu\": %d,\n  \"claim\": %u,\n"
+                          "  \"presieve_bound\": %u,\n  \"waves_per_group\": %u,\n"
+                          "  \"segment_bits\": %u,\n  \"ms\": %.6f,\n  \"prps_per_s\": %.1f\n}\n",
+                          info.name, info.arch, info.compute_units, c->deep_bound, c->tuned_blocks,
+                          c->tuned_claim, c->presieve_bound, waves, bits, c->last_tuning.ms,
+                          c->last_tuning.prps_per_s);
+    if (fclose(f) != 0 || n < 0) return fail(WW_E_ARG, "cannot write %s", path);
+    return WW_OK;
+}
+
+} // extern "C"
+
+namespace {
+
+/* The value text after "key": in a flat JSON object (no nesting), or null. */
+const char *json_value(const std::string &text, const char *key)
+{
+    const std::string k = std::string("\"") + key + "\"";
+    size_t at = text.find(k);
+    if (at == std::string::npos) return nullptr;
+    at = text.find(':', at + k.size());
+    if (at == std::string::npos) return nullptr;
+    const char *v = text.c_str() + at + 1;
+    while (*v == ' ' || *v == '\t' || *v == '\n' || *v == '\r') ++v;
+    return v;
+}
+
+bool json_uint(const std::string &text, const char *key, long long lo, long long hi, long long *out)
+{
+    const char *v = json_value(text, key);
+    if (!v) return false;
+    char *end = nullptr;
+    const long long x = strtoll(v, &end, 10);
+    if (end == v || x < lo || x > hi) return false;
+    *out = x;
+    return true;
+}
+
+bool json_string(const std::string &text, const char *key, std::string *out)
+{
+    const char *v = json_value(text, key);
+    if (!v || *v != '"') return false;
+    const char *e = strchr(v + 1, '"');
+    if (!e) return false;
+    out->assign(v + 1, e);
+    return true;
+}
+
+} // namespace
+
+extern "C" {
+
+int ww_tuning_load(ww_ctx *c, const char *path, ww_tuning *applied)
+{
+    WW_LOCK(c);
+    int rc = use(c);
+    if (rc) return rc;
+    if (!path) return fail(WW_E_ARG, "null path");
+    FILE *f = fopen(path, "r");
+    if (!f) return fail(WW_E_ARG, "cannot read %s", path);
+    std::string text;
+    char buf[1024];
+    size_t k;
+    while ((k = fread(buf, 1, sizeof buf, f)) > 0 && text.size() < 65536) text.append(buf, k);
+    fclose(f);
+    ww_device_info_t info;
+    if ((rc = ww_device_info(c->device, &info))) return rc;
+    std::string name, arch;
+    long long cus = 0, deep = 0, bpc = 0, claim = 0, pre = 0, waves = 0, bits = 0;
+    if (!json_string(text, "device", &name) || !json_string(text, "arch", &arch) ||
+        !json_uint(text, "compute_units", 1, 1 << 20, &cus) ||
+        !json_uint(text, "deep_bound", 0, ww::kMaxDeepBound, &deep) ||
+        !json_uint(text, "blocks_per_cu", 0, 64, &bpc) || !json_uint(text, "claim", 1, 4096, &claim) ||
+        !json_uint(text, "presieve_bound", 0, WW_MAX_PRESIEVE_BOUND, &pre) ||
+        !json_uint(text, "waves_per_group", 2, 8, &waves) || !json_uint(text, "segment_bits", 15, 20, &bits))
+        return fail(WW_E_ARG, "%s is not a tuning file of this library", path);
+    if (name != info.name || arch != info.arch || cus != info.compute_units)
+        return fail(WW_E_ARG, "%s was tuned on %s (%s, %lld CUs), not on this %s (%s, %d CUs)", path,
+                    name.c_str(), arch.c_str(), cus, info.name, info.arch, info.compute_units);
+    if (deep && deep < 127) return fail(WW_E_ARG, "%s: deep bound %lld", path, deep);
+    if ((rc = ww_set_deep_bound(c, (uint32_t)deep))) return rc;
+    if ((rc = ww_set_presieve_bound(c, (uint32_t)pre))) return rc;
+    if ((rc = ww_set_launch_shape(c, (uint32_t)waves, (uint32_t)bits))) return rc;
+    c->tuned_blocks = (int)bpc;
+    c->tuned_claim = (uint32_t)claim;
+    ww_tuning t;
+    memset(&t, 0, sizeof t);
+    t.deep_bound = (uint32_t)deep;
+    t.blocks_per_cu = (int32_t)bpc;
+    t.claim = (uint32_t)claim;
+    t.presieve_bound = (uint32_t)pre;
+    t.waves_per_group = (uint32_t)waves;
+    t.segment_bits = (uint32_t)bits;
+    const char *v = json_value(text, "ms");
+    if (v) t.ms = strtod(v, nullptr);
+    v = json_value(text, "prps_per_s");
+    if (v) t.prps_per_s = strtod(v, nullptr);
+    c->last_tuning = t;
+    if (applied) *applied = t;
     return WW_OK;
 }
 

@@ -53,15 +53,24 @@ constexpr uint32_t kPresieveMinPrime = 1024;
    positions would cost more than the strong PRP tests it saves (those segments drop
    nothing extra). */
 constexpr uint64_t kPresieveMinSegs = 16;
+/* The pre-sieve's LDS segment, in 8192-integer segments: a power of two from 8 (2^15 bits,
+   one per odd number) to 256 (2^20 bits, the default). */
+constexpr uint32_t kPresieveMinChunkSegs = 8;
+constexpr uint32_t kPresieveMaxChunkSegs = 256;
+bool presieve_chunk_ok(uint32_t chunk_segs);
 
-int batch_blocks_per_cu();
+/* Work-group shapes of the persistent kernel: 2, 4 or 8 waves per workgroup. */
+constexpr int kDefaultBatchWaves = 4;
+bool batch_waves_ok(int waves);
+/* Resident workgroups per CU of the persistent kernel at that shape (0 on error). */
+int batch_blocks_per_cu(int waves = kDefaultBatchWaves);
 
 /* The wide pre-sieve over all segments of the spans: out[64 s + w] = odd numbers to drop
    in word w of segment s (multiples of tab's primes, minus their pseudoprime classes).
    No allocation, no memset: safe to capture into a graph. */
 hipError_t launch_presieve(const BatchSpan *spans, const uint64_t *prefix, uint32_t nspans,
                            const WidePrime *tab, uint32_t ntab, uint64_t *out, int grid,
-                           hipStream_t s);
+                           hipStream_t s, uint32_t chunk_segs = kPresieveMaxChunkSegs);
 
 /* Writes the one-span table of the range [lo, hi) (segments from lo | 1) into span and
    prefix (device scratch), so that a single range needs no host copy. */
@@ -71,17 +80,18 @@ hipError_t launch_set_span(uint64_t lo, uint64_t hi, BatchSpan *span, uint64_t *
 /* launch_set_span, then launch_presieve over that one span (a plain-form chunk). */
 hipError_t launch_presieve_range(uint64_t lo, uint64_t hi, BatchSpan *span, uint64_t *prefix,
                                  const WidePrime *tab, uint32_t ntab, uint64_t *out, int grid,
-                                 hipStream_t s);
+                                 hipStream_t s, uint32_t chunk_segs = kPresieveMaxChunkSegs);
 
 /* Clears the state and the first `nslots` slot sums (0: none, so sums accumulate over
    launches), then runs the persistent kernel over all spans; each wave claims up to
    `claim` consecutive segments at a time. `pre` (may be null) is the wide pre-sieve of
-   the same spans.
+   the same spans. `waves` is the work-group shape (2, 4 or 8 waves).
    No allocation, no memset: safe to capture into a graph. */
 hipError_t launch_batch(const BatchSpan *spans, const uint64_t *prefix, uint32_t nspans,
                         BatchState *st, SlotSum *sums, uint32_t nslots, ww_result *res,
                         uint32_t res_cap, int grid, uint32_t claim, const DeepPrime *deep,
-                        uint32_t ndeep, const uint64_t *pre, hipStream_t s);
+                        uint32_t ndeep, const uint64_t *pre, hipStream_t s,
+                        int waves = kDefaultBatchWaves);
 
 /* Adds one single-range persistent launch to the plain form's counters and records: its
    checksum and tested count to the totals, and its records (up to bcap) after the ones
@@ -96,7 +106,8 @@ hipError_t launch_fold(const BatchState *bst, const SlotSum *sum, const ww_resul
 hipError_t noalloc_init(int device);
 hipError_t launch_noalloc(int device, const ww_range *ranges, uint32_t n, SlotSum *sums,
                           ww_result *res, uint32_t res_cap, uint32_t *res_count, int grid,
-                          uint32_t claim, const DeepPrime *deep, uint32_t ndeep, hipStream_t s);
+                          uint32_t claim, const DeepPrime *deep, uint32_t ndeep, hipStream_t s,
+                          int waves = kDefaultBatchWaves);
 
 } // namespace ww
 

@@ -23,7 +23,6 @@ namespace ww {
 
 namespace {
 
-constexpr int kBatchWaves = 4;
 constexpr int kQueueCap = 128; /* < 64 left over + <= 64 appended per PRP round */
 /* candidates of one segment (at most kWaveCandCap) behind fewer than 64 leftovers */
 constexpr int kBatchCandCap = kWaveCandCap + 64;
@@ -96,22 +95,25 @@ __device__ __forceinline__ void tests_round(u32 n, u32 &qn, u64 *wq, u32 *ws, u3
     }
 }
 
-#ifndef WW_BATCH_WAVES_PER_EU
-#define WW_BATCH_WAVES_PER_EU 5 /* 96 VGPRs: five 4-wave workgroups per CU */
-#endif
+/* Waves per SIMD each shape is compiled for (96 VGPRs at 5, 128 at 4). Eight-wave
+   workgroups (54 KB of LDS each) fit two per CU, four waves per SIMD. */
+template <int W> struct BatchShape {
+    static constexpr int kWavesPerEU = W == 8 ? 4 : 5;
+};
 
-__global__ __launch_bounds__(64 * kBatchWaves)
-__attribute__((amdgpu_waves_per_eu(WW_BATCH_WAVES_PER_EU))) void k_batch(const BatchSpan *spans,
+template <int W>
+__global__ __launch_bounds__(64 * W)
+__attribute__((amdgpu_waves_per_eu(BatchShape<W>::kWavesPerEU))) void k_batch(const BatchSpan *spans,
                                                              const u64 *prefix, u32 nspans,
                                                              BatchState *st, SlotSum *sums,
                                                              ww_result *res, u32 res_cap,
                                                              u32 claim, const DeepPrime *deep,
                                                              u32 ndeep, const u64 *pre)
 {
-    __shared__ __align__(16) uint16_t cand[kBatchWaves][kBatchCandCap];
-    __shared__ u64 deep_bm[kBatchWaves][64];
-    __shared__ u64 qp[kBatchWaves][kQueueCap];
-    __shared__ u32 qs[kBatchWaves][kQueueCap];
+    __shared__ __align__(16) uint16_t cand[W][kBatchCandCap];
+    __shared__ u64 deep_bm[W][64];
+    __shared__ u64 qp[W][kQueueCap];
+    __shared__ u32 qs[W][kQueueCap];
 
     const u32 lane = threadIdx.x & 63;
     const u32 wave = threadIdx.x >> 6;
@@ -173,7 +175,7 @@ __attribute__((amdgpu_waves_per_eu(WW_BATCH_WAVES_PER_EU))) void k_batch(const B
                per wave of the grid, as last seen by this wave), so the launch ends with
                short claims and no long tail */
             const u64 left = total > seen ? total - seen : 0;
-            u64 k = left / ((u64)gridDim.x * kBatchWaves * 4);
+            u64 k = left / ((u64)gridDim.x * W * 4);
             k = k < 1 ? 1 : (k > claim ? claim : k);
             u64 c = 0;
             if (lane == 0) c = atomicAdd((unsigned long long *)&st->cursor, (unsigned long long)k);
@@ -281,25 +283,38 @@ hipError_t launch_fold(const BatchState *bst, const SlotSum *sum, const ww_resul
     return hipGetLastError();
 }
 
-int batch_blocks_per_cu()
+bool batch_waves_ok(int waves) { return waves == 2 || waves == 4 || waves == 8; }
+
+int batch_blocks_per_cu(int waves)
 {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_batch, 64 * kBatchWaves, 0) != hipSuccess)
-        return 0;
-    return nb;
+    hipError_t e = hipErrorInvalidValue;
+    if (waves == 2) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_batch<2>, 128, 0);
+    if (waves == 4) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_batch<4>, 256, 0);
+    if (waves == 8) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_batch<8>, 512, 0);
+    return e == hipSuccess ? nb : 0;
 }
 
 hipError_t launch_batch(const BatchSpan *spans, const u64 *prefix, u32 nspans, BatchState *st,
                         SlotSum *sums, u32 nslots, ww_result *res, u32 res_cap, int grid,
                         u32 claim, const DeepPrime *deep, u32 ndeep, const u64 *pre,
-                        hipStream_t s)
+                        hipStream_t s, int waves)
 {
+    if (!batch_waves_ok(waves)) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_batch_clear, dim3(nslots > 256 ? (nslots + 255) / 256 : 1), dim3(256), 0, s,
                        st, sums, nslots);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_batch, dim3(grid), dim3(64 * kBatchWaves), 0, s, spans, prefix, nspans, st,
-                       sums, res, res_cap, claim < 1 ? 1u : claim, deep, ndeep, pre);
+    const u32 c = claim < 1 ? 1u : claim;
+    if (waves == 2)
+        hipLaunchKernelGGL(k_batch<2>, dim3(grid), dim3(128), 0, s, spans, prefix, nspans, st, sums,
+                           res, res_cap, c, deep, ndeep, pre);
+    else if (waves == 8)
+        hipLaunchKernelGGL(k_batch<8>, dim3(grid), dim3(512), 0, s, spans, prefix, nspans, st, sums,
+                           res, res_cap, c, deep, ndeep, pre);
+    else
+        hipLaunchKernelGGL(k_batch<4>, dim3(grid), dim3(256), 0, s, spans, prefix, nspans, st, sums,
+                           res, res_cap, c, deep, ndeep, pre);
     return hipGetLastError();
 }
 

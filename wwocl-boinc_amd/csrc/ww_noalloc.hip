@@ -107,7 +107,8 @@ hipError_t noalloc_init(int device)
 
 hipError_t launch_noalloc(int device, const ww_range *ranges, uint32_t n, SlotSum *sums,
                           ww_result *res, uint32_t res_cap, uint32_t *res_count, int grid,
-                          uint32_t claim, const DeepPrime *deep, uint32_t ndeep, hipStream_t s)
+                          uint32_t claim, const DeepPrime *deep, uint32_t ndeep, hipStream_t s,
+                          int waves)
 {
     if (n == 0 || n > kMaxRanges) return hipErrorInvalidValue;
     void *base = nullptr;
@@ -124,7 +125,8 @@ hipError_t launch_noalloc(int device, const ww_range *ranges, uint32_t n, SlotSu
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     /* clears the state and the caller's n sums, then the persistent kernel (no pre-sieve) */
-    e = launch_batch(spans, prefix, n, st, sums, n, res, res_cap, grid, claim, deep, ndeep, nullptr, s);
+    e = launch_batch(spans, prefix, n, st, sums, n, res, res_cap, grid, claim, deep, ndeep, nullptr, s,
+                     waves);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_noalloc_count, dim3(1), dim3(64), 0, s, st, res_count);
     return hipGetLastError();

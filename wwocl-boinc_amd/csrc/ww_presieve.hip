@@ -9,13 +9,15 @@
  * So, as with the per-segment deep sieve, no member of the reference's PRP list is ever
  * dropped; only composites are, before their strong PRP test.
  *
- * Work unit: a chunk of 256 consecutive segments (2^21 integers), split where it crosses
- * from one range of the batch to the next. One workgroup marks a chunk in a 128 KB LDS
- * bitmap, so each prime's start positions (two 64-bit Barrett reductions) are computed
- * once per 2^20 odd numbers instead of once per 4096. Pseudoprime classes are collected
- * in an LDS list (at most a few hundred per chunk; the table starts above 1024 for that);
- * if the list overflows, the chunk drops nothing (exact, only slower). Pieces shorter than
- * 16 segments are not pre-sieved.
+ * Work unit: a chunk of `chunk_segs` consecutive segments (a power of two from 8 to 256:
+ * a sieve segment of 2^15 to 2^20 bits, one per odd number; 256 = 2^21 integers by
+ * default), split where it crosses from one range of the batch to the next. One workgroup
+ * marks a chunk in an LDS bitmap (128 KB at the largest size), so each prime's start
+ * positions (two 64-bit Barrett reductions) are computed once per chunk instead of once
+ * per 4096 odd numbers. Pseudoprime classes are collected in an LDS list (at most a few
+ * hundred per chunk; the table starts above 1024 for that); if the list overflows, the
+ * chunk drops nothing (exact, only slower). Pieces shorter than min(16, chunk_segs)
+ * segments are not pre-sieved.
  */
 #include "ww_arith.h"
 #include "ww_batch.h"
@@ -25,8 +27,8 @@ namespace ww {
 namespace {
 
 constexpr int kPreThreads = 1024;
-constexpr u32 kChunkSegs = 256;
-constexpr u32 kChunkPos = kChunkSegs * 4096; /* odd numbers per chunk */
+constexpr u32 kChunkSegs = kPresieveMaxChunkSegs;
+constexpr u32 kChunkPos = kChunkSegs * 4096; /* odd numbers per chunk, at most */
 constexpr u32 kChunkWords = kChunkPos / 32;
 constexpr u32 kRescueCap = 6144;
 
@@ -41,18 +43,20 @@ __device__ __forceinline__ u64 barrett64(u64 x, u64 m, u64 bar)
 __global__ __launch_bounds__(kPreThreads) void k_presieve(const BatchSpan *spans,
                                                           const u64 *prefix, u32 nspans,
                                                           const WidePrime *tab, u32 ntab,
-                                                          u64 *out)
+                                                          u64 *out, u32 chunk_segs)
 {
     __shared__ __align__(8) u32 bm[kChunkWords]; /* read back as u64 words below */
     __shared__ u32 rescue[kRescueCap];
     __shared__ u32 rcount;
 
     const u64 total = prefix[nspans];
-    const u64 nchunks = (total + kChunkSegs - 1) / kChunkSegs;
+    const u64 nchunks = (total + chunk_segs - 1) / chunk_segs;
+    const u64 min_piece = chunk_segs < kPresieveMinSegs ? chunk_segs : kPresieveMinSegs;
+    const u32 chunk_words = chunk_segs * (4096 / 32);
     for (u64 c = blockIdx.x; c < nchunks; c += gridDim.x) {
-        const u64 s0 = c * kChunkSegs;
-        const u64 s1 = s0 + kChunkSegs < total ? s0 + kChunkSegs : total;
-        for (u32 w = threadIdx.x; w < kChunkWords; w += kPreThreads) bm[w] = 0;
+        const u64 s0 = c * chunk_segs;
+        const u64 s1 = s0 + chunk_segs < total ? s0 + chunk_segs : total;
+        for (u32 w = threadIdx.x; w < chunk_words; w += kPreThreads) bm[w] = 0;
         if (threadIdx.x == 0) rcount = 0;
         __syncthreads();
 
@@ -71,7 +75,7 @@ __global__ __launch_bounds__(kPreThreads) void k_presieve(const BatchSpan *spans
             const u32 len = (u32)(e - s) * 4096;
             /* a short piece (a small range, or the end of one) is left to the strong PRP
                test: its words stay zero */
-            const u32 nprimes = e - s >= kPresieveMinSegs ? ntab : 0;
+            const u32 nprimes = e - s >= min_piece ? ntab : 0;
             for (u32 j = threadIdx.x; j < nprimes; j += kPreThreads) {
                 const WidePrime d = tab[j];
                 const u64 ell = d.ell;
@@ -81,7 +85,7 @@ __global__ __launch_bounds__(kPreThreads) void k_presieve(const BatchSpan *spans
                 if (dd & 1) dd += ell;
                 u32 i = (u32)(dd >> 1); /* < ell < 2^22 */
                 if (G <= ell && G + 2 * (u64)i == ell) i += (u32)ell; /* never ell itself */
-                /* positions in the chunk, below 2^20 + 2^23: 32-bit arithmetic */
+                /* positions in the chunk, below 2^20 + 2^22: 32-bit arithmetic */
                 const u32 end = off + len;
                 for (u32 at = off + i; at < end; at += (u32)ell)
                     atomicOr(&bm[at >> 5], 1u << (at & 31));
@@ -132,18 +136,26 @@ hipError_t launch_set_span(u64 lo, u64 hi, BatchSpan *span, u64 *prefix, hipStre
 
 hipError_t launch_presieve_range(u64 lo, u64 hi, BatchSpan *span, u64 *prefix,
                                  const WidePrime *tab, u32 ntab, u64 *out, int grid,
-                                 hipStream_t s)
+                                 hipStream_t s, u32 chunk_segs)
 {
     hipError_t e = launch_set_span(lo, hi, span, prefix, s);
     if (e != hipSuccess) return e;
-    return launch_presieve(span, prefix, 1, tab, ntab, out, grid, s);
+    return launch_presieve(span, prefix, 1, tab, ntab, out, grid, s, chunk_segs);
+}
+
+bool presieve_chunk_ok(u32 chunk_segs)
+{
+    return chunk_segs >= kPresieveMinChunkSegs && chunk_segs <= kPresieveMaxChunkSegs &&
+           (chunk_segs & (chunk_segs - 1)) == 0;
 }
 
 hipError_t launch_presieve(const BatchSpan *spans, const u64 *prefix, u32 nspans,
-                           const WidePrime *tab, u32 ntab, u64 *out, int grid, hipStream_t s)
+                           const WidePrime *tab, u32 ntab, u64 *out, int grid, hipStream_t s,
+                           u32 chunk_segs)
 {
+    if (!presieve_chunk_ok(chunk_segs)) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_presieve, dim3(grid), dim3(kPreThreads), 0, s, spans, prefix, nspans,
-                       tab, ntab, out);
+                       tab, ntab, out, chunk_segs);
     return hipGetLastError();
 }
 

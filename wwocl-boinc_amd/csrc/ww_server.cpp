@@ -95,6 +95,8 @@ struct ww_server {
     ww::WidePrime *d_wide = nullptr;
     uint32_t nwide = 0;
     int cus = 0;
+    int waves = ww::kDefaultBatchWaves;                     /* the context's shape at creation */
+    uint32_t presieve_chunk = ww::kPresieveMaxChunkSegs;
     /* the HBM budget as an arena of 512-byte blocks: resident pre-sieve words of the
        admitted requests (allocated only when the pre-sieve is on; without it the blocks are
        only counted) */
@@ -279,11 +281,13 @@ void engine_main(ww_server *s)
                 off = q.pre_off;
                 nseg = q.nseg;
             }
-            const int pgrid = (int)std::min<uint64_t>((nseg + 255) / 256, (uint64_t)(s->cus > 0 ? s->cus : 1));
+            const int pgrid = (int)std::min<uint64_t>((nseg + s->presieve_chunk - 1) / s->presieve_chunk,
+                                                      (uint64_t)(s->cus > 0 ? s->cus : 1));
             if (!hip_ok(s, ww::launch_set_span(lo, hi, s->d_adm_span, s->d_adm_prefix, s->stream), "span") ||
                 !hip_ok(s, ww::launch_presieve(s->d_adm_span, s->d_adm_prefix, 1, s->d_wide, s->nwide,
                                                s->d_arena + off * ww::kPresieveWordsPerSeg,
-                                               pgrid > 0 ? pgrid : 1, s->stream), "k_presieve"))
+                                               pgrid > 0 ? pgrid : 1, s->stream, s->presieve_chunk),
+                           "k_presieve"))
                 return;
         }
 
@@ -333,7 +337,7 @@ void engine_main(ww_server *s)
                                            hipMemcpyHostToDevice, s->stream), "prefix") &&
                   hip_ok(s, ww::launch_batch(t.d_spans, t.d_prefix, nspans, t.d_state, s->d_sums, 0,
                                              t.d_res, kTickRecords, s->grid, s->cfg.claim, s->d_deep,
-                                             s->ndeep, s->d_arena, s->stream), "k_batch") &&
+                                             s->ndeep, s->d_arena, s->stream, s->waves), "k_batch") &&
                   hip_ok(s, hipMemcpyAsync(t.h_state, t.d_state, sizeof(BatchState),
                                            hipMemcpyDeviceToHost, s->stream), "state") &&
                   hip_ok(s, hipMemcpyAsync(t.h_res, t.d_res, kTickCopy * sizeof(ww_result),
@@ -394,7 +398,8 @@ void destroy(ww_server *s)
 /* internal accessors of the context (ww_api.cpp) */
 int ww_ctx_internals(ww_ctx *c, int *device, const ww::DeepPrime **deep, uint32_t *ndeep,
                      int *cus, int *tuned_blocks, uint32_t *tuned_claim,
-                     const ww::WidePrime **wide, uint32_t *nwide);
+                     const ww::WidePrime **wide, uint32_t *nwide, int *waves,
+                     uint32_t *presieve_chunk);
 int ww_set_error(int code, const char *fmt, ...);
 
 extern "C" {
@@ -424,7 +429,7 @@ int ww_server_create(ww_ctx *ctx, const ww_server_config *cfg, ww_server **out)
     const ww::DeepPrime *ctx_deep = nullptr;
     const ww::WidePrime *ctx_wide = nullptr;
     int rc = ww_ctx_internals(ctx, &s->device, &ctx_deep, &s->ndeep, &cus, &tuned_blocks,
-                              &tuned_claim, &ctx_wide, &s->nwide);
+                              &tuned_claim, &ctx_wide, &s->nwide, &s->waves, &s->presieve_chunk);
     s->cus = cus;
     if (rc) {
         delete s;
@@ -435,7 +440,7 @@ int ww_server_create(ww_ctx *ctx, const ww_server_config *cfg, ww_server **out)
     (void)tuned_claim;
     if (!s->cfg.claim) s->cfg.claim = 1;
     int bpc = s->cfg.blocks_per_cu >= 0 ? s->cfg.blocks_per_cu : tuned_blocks;
-    if (bpc <= 0) bpc = ww::batch_blocks_per_cu();
+    if (bpc <= 0) bpc = ww::batch_blocks_per_cu(s->waves);
     s->grid = cus * (bpc > 0 ? bpc : 4);
     {
         size_t free_b = 0, total_b = 0;

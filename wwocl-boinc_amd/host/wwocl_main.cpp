@@ -2,7 +2,7 @@
  * wwocl_main.cpp -- the drop-in host: the reference's command line, checkpoint files and
  * results file, driving the gfx950 path through the C ABI (include/ww_api.h).
  *
- *   wwocl START END [-testprimes] [-compute] [-debug] [-ws N]
+ *   wwocl START END [-testprimes] [-compute] [-debug] [-ws N] [-tune]
  *
  * Behaviour follows main.cpp of the reference:
  *   START/END parsing with "Me X" notation          main.cpp:1193-1274
@@ -20,7 +20,10 @@
  * "init_data.xml not found" message. The work-size probe (profileGPU, main.cpp:959-1107)
  * times the chunk body at the current position and scales the chunk to the reference's
  * kernel-time targets: 10 ms, -ws N ms (1..100), or 100 ms with -compute. The checksum
- * does not depend on chunking. Validation lives in wwocl_checks.cpp.
+ * does not depend on chunking. The library's autotuner (sieve bounds, geometry, launch
+ * shape) runs with -tune and writes a per-device tuning cache, which later starts load
+ * ($WW_TUNE_CACHE, else ~/.cache/wwocl/tuning-<target>-<CUs>cu.json). Validation lives in
+ * wwocl_checks.cpp.
  */
 #include <algorithm>
 #include <cinttypes>
@@ -30,6 +33,7 @@
 #include <cstring>
 #include <ctime>
 #include <string>
+#include <sys/stat.h>
 #include <unistd.h>
 #include <vector>
 
@@ -350,6 +354,51 @@ int pick_device(int argc, char *argv[])
     return 0;
 }
 
+/* The tuning cache of this device. */
+std::string tuning_cache_path(const ww_device_info_t &info)
+{
+    const char *env = getenv("WW_TUNE_CACHE");
+    if (env && *env) return env;
+    const char *home = getenv("HOME");
+    std::string dir = std::string(home && *home ? home : ".") + "/.cache";
+    mkdir(dir.c_str(), 0755);
+    dir += "/wwocl";
+    mkdir(dir.c_str(), 0755);
+    std::string target;
+    for (const char *c = info.arch; *c && *c != ':'; ++c) target += *c;
+    return dir + "/tuning-" + target + "-" + std::to_string(info.compute_units) + "cu.json";
+}
+
+/* -tune: the library's autotuner on 2^31 integers at the current position (at least 1e8),
+   then the cache; otherwise the cache when there is one. */
+void apply_tuning(Run &run, const ww_device_info_t &info, uint64_t current, bool tune)
+{
+    const std::string path = tuning_cache_path(info);
+    if (tune) {
+        uint64_t lo = std::max<uint64_t>(current, 100000000ull);
+        if (lo > UINT64_MAX - (1ull << 31)) lo = UINT64_MAX - (1ull << 31);
+        ww_tuning best;
+        uint32_t n = 0;
+        fprintf(stderr, "Tuning the GPU kernels...\n");
+        check(ww_autotune(run.ctx, lo, lo + (1ull << 31), nullptr, &best, nullptr, 0, &n), "autotune");
+        say("Tuned (%u trials): deep bound %u, pre-sieve bound %u, %u waves per workgroup, "
+            "%d workgroups per CU, claims of %u, %u-bit sieve segments: %.3e primes/s\n",
+            n, best.deep_bound, best.presieve_bound, best.waves_per_group, best.blocks_per_cu,
+            best.claim, best.segment_bits, best.prps_per_s);
+        if (ww_tuning_save(run.ctx, path.c_str()) == WW_OK)
+            fprintf(stderr, "Tuning saved to %s\n", path.c_str());
+        else
+            fprintf(stderr, "Tuning not saved: %s\n", ww_last_error());
+        return;
+    }
+    if (access(path.c_str(), R_OK) != 0) return;
+    ww_tuning t;
+    if (ww_tuning_load(run.ctx, path.c_str(), &t) == WW_OK)
+        fprintf(stderr, "Tuning loaded from %s\n", path.c_str());
+    else
+        fprintf(stderr, "Tuning cache ignored: %s\n", ww_last_error());
+}
+
 void usage(const char *prog)
 {
     printf("Usage: %s START END -testprimes -compute -debug -ws #\nWhere 127 <= START < END < 2^64\n", prog);
@@ -358,6 +407,7 @@ void usage(const char *prog)
     printf("-debug is optional and will display more information in standalone mode.\n\n");
     printf("-ws # is optional worksize for adjusting the GPU load higher for faster computation or lower for better screen refresh rate.\n");
     printf(" If using compute mode, -ws is ignored.  Usage: -ws 10 for default (10 ms kernels), -ws 1 for lowest GPU load, -ws 100 for highest GPU load.\n\n");
+    printf("-tune is optional and tunes the kernels for this GPU, saving the result for later runs.\n\n");
     printf("Example format: %s 10000000 20000000\n", prog);
     printf("            Or: %s 1e7 2e7\n", prog);
     exit(EXIT_FAILURE);
@@ -374,7 +424,7 @@ int main(int argc, char *argv[])
     fprintf(stderr, "\n");
     if (argc < 3) usage(argv[0]);
 
-    int test_prps = 0, debug = 0, compute = 0, ws = 0;
+    int test_prps = 0, debug = 0, compute = 0, ws = 0, tune = 0;
     for (int w = 3; w < argc; ++w) {
         if (strcmp(argv[w], "-compute") == 0) {
             compute = 1;
@@ -385,6 +435,8 @@ int main(int argc, char *argv[])
         } else if (strcmp(argv[w], "-debug") == 0) {
             debug = 1;
             printf("Debug info enabled.\n");
+        } else if (strcmp(argv[w], "-tune") == 0) {
+            tune = 1;
         }
     }
     /* command-line kernel time in ms, ignored in compute mode (main.cpp:1424-1444) */
@@ -446,6 +498,7 @@ int main(int argc, char *argv[])
     run.list_cap = std::max(ww_list_bound(1ull << 32, (1ull << 32) + list_span),
                             ww_list_bound(WW_MIN_START, WW_MIN_START + list_span));
     check(ww_open(device, run.list_cap, kNumResults, &run.ctx), "ww_open");
+    apply_tuning(run, info, st.current, tune != 0);
 
     const double target_ms = compute ? 100.0 : (ws ? (double)ws : 10.0);
     const uint64_t chunk = profile_chunk(run, info.compute_units, st.current, target_ms, compute);

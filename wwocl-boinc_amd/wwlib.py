@@ -67,12 +67,23 @@ class WidePrime(ctypes.Structure):
 class Tuning(ctypes.Structure):
     _fields_ = [("deep_bound", U32), ("blocks_per_cu", ctypes.c_int32), ("claim", U32),
                 ("presieve_bound", U32), ("ms", ctypes.c_double),
-                ("prps_per_s", ctypes.c_double)]
+                ("prps_per_s", ctypes.c_double), ("waves_per_group", U32),
+                ("segment_bits", U32)]
 
     def as_dict(self):
         return {"deep_bound": self.deep_bound, "blocks_per_cu": self.blocks_per_cu,
                 "claim": self.claim, "presieve_bound": self.presieve_bound, "ms": self.ms,
-                "prps_per_s": self.prps_per_s}
+                "prps_per_s": self.prps_per_s, "waves_per_group": self.waves_per_group,
+                "segment_bits": self.segment_bits}
+
+
+class TuneSpace(ctypes.Structure):
+    _fields_ = [("deep_bounds", ctypes.POINTER(U32)), ("n_deep_bounds", U32),
+                ("blocks_per_cu", ctypes.POINTER(ctypes.c_int32)), ("n_blocks_per_cu", U32),
+                ("claims", ctypes.POINTER(U32)), ("n_claims", U32),
+                ("presieve_bounds", ctypes.POINTER(U32)), ("n_presieve_bounds", U32),
+                ("waves_per_group", ctypes.POINTER(U32)), ("n_waves_per_group", U32),
+                ("segment_bits", ctypes.POINTER(U32)), ("n_segment_bits", U32)]
 
 
 class ServerConfig(ctypes.Structure):
@@ -148,10 +159,13 @@ _SIGNATURES = [
                                              ctypes.POINTER(ctypes.c_double),
                                              ctypes.POINTER(ctypes.c_double)]),
     ("ww_batch_destroy", ctypes.c_int, [ctypes.c_void_p]),
-    ("ww_autotune", ctypes.c_int, [ctypes.c_void_p, U64, U64, ctypes.POINTER(U32), U32,
-                                   ctypes.POINTER(ctypes.c_int32), U32, ctypes.POINTER(U32), U32,
-                                   ctypes.POINTER(U32), U32, ctypes.POINTER(Tuning),
-                                   ctypes.POINTER(Tuning), U32, ctypes.POINTER(U32)]),
+    ("ww_autotune", ctypes.c_int, [ctypes.c_void_p, U64, U64, ctypes.POINTER(TuneSpace),
+                                   ctypes.POINTER(Tuning), ctypes.POINTER(Tuning), U32,
+                                   ctypes.POINTER(U32)]),
+    ("ww_tuning_save", ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p]),
+    ("ww_tuning_load", ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.POINTER(Tuning)]),
+    ("ww_set_launch_shape", ctypes.c_int, [ctypes.c_void_p, U32, U32]),
+    ("ww_get_launch_shape", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(U32), ctypes.POINTER(U32)]),
     ("ww_server_create", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ServerConfig),
                                         ctypes.POINTER(ctypes.c_void_p)]),
     ("ww_server_submit", ctypes.c_int, [ctypes.c_void_p, U64, U64, ctypes.POINTER(U64)]),
@@ -344,21 +358,48 @@ class Context:
         _check(self.lib.ww_get_presieve_bound(self.h, ctypes.byref(v)))
         return v.value
 
-    def autotune(self, lo: int, hi: int, bounds=None, blocks=None, claims=None, presieve=None):
-        """Tune the persistent form on [lo, hi); returns (best, [trials]) as dicts."""
+    def autotune(self, lo: int, hi: int, bounds=None, blocks=None, claims=None, presieve=None,
+                 waves=None, segment_bits=None):
+        """Tune the persistent form on [lo, hi); returns (best, [trials]) as dicts. A None
+        list takes the library's built-in search space for that knob."""
+        keep = []
+
         def arr(t, v):
-            return ((t * len(v))(*v), len(v)) if v else (None, 0)
-        b, nb = arr(U32, bounds)
-        bl, nbl = arr(ctypes.c_int32, blocks)
-        c, nc = arr(U32, claims)
-        ps, nps = arr(U32, presieve)
+            if not v:
+                return None, 0
+            a = (t * len(v))(*v)
+            keep.append(a)
+            return ctypes.cast(a, ctypes.POINTER(t)), len(v)
+        sp = TuneSpace()
+        sp.deep_bounds, sp.n_deep_bounds = arr(U32, bounds)
+        sp.blocks_per_cu, sp.n_blocks_per_cu = arr(ctypes.c_int32, blocks)
+        sp.claims, sp.n_claims = arr(U32, claims)
+        sp.presieve_bounds, sp.n_presieve_bounds = arr(U32, presieve)
+        sp.waves_per_group, sp.n_waves_per_group = arr(U32, waves)
+        sp.segment_bits, sp.n_segment_bits = arr(U32, segment_bits)
         best = Tuning()
-        cap = 256
+        cap = 512
         trials = (Tuning * cap)()
         n = U32()
-        _check(self.lib.ww_autotune(self.h, lo, hi, b, nb, bl, nbl, c, nc, ps, nps,
-                                    ctypes.byref(best), trials, cap, ctypes.byref(n)))
+        _check(self.lib.ww_autotune(self.h, lo, hi, ctypes.byref(sp), ctypes.byref(best), trials,
+                                    cap, ctypes.byref(n)))
         return best.as_dict(), [trials[i].as_dict() for i in range(min(n.value, cap))]
+
+    def set_launch_shape(self, waves_per_group: int, segment_bits: int):
+        _check(self.lib.ww_set_launch_shape(self.h, waves_per_group, segment_bits))
+
+    def launch_shape(self):
+        w, b = U32(), U32()
+        _check(self.lib.ww_get_launch_shape(self.h, ctypes.byref(w), ctypes.byref(b)))
+        return w.value, b.value
+
+    def tuning_save(self, path: str):
+        _check(self.lib.ww_tuning_save(self.h, str(path).encode()))
+
+    def tuning_load(self, path: str) -> dict:
+        t = Tuning()
+        _check(self.lib.ww_tuning_load(self.h, str(path).encode(), ctypes.byref(t)))
+        return t.as_dict()
 
     # reference kernel-level entry points
     def clear_results(self):
@@ -631,15 +672,22 @@ class Server:
             pass
 
 
-def serve(server: Server, requests, realtime: bool = True):
+def serve(server: Server, requests, realtime: bool = True, progress_s: float = 0.0):
     """Submit (t_arrival, lo, hi) requests at their arrival times (seconds from now) and
-    collect every completion. Returns (completions by id, {id: (lo, hi)}, wall seconds)."""
+    collect every completion. Returns (completions by id, {id: (lo, hi)}, wall seconds).
+    progress_s > 0 prints a progress line to stderr that often."""
+    import sys
     import time
     t0 = time.perf_counter()
     ids, done = {}, {}
     i = 0
+    next_note = progress_s
     while i < len(requests) or len(done) < len(ids):
         now = time.perf_counter() - t0
+        if progress_s > 0 and now >= next_note:
+            print(f"serve: {now:.0f} s, {i} of {len(requests)} submitted, {len(done)} done",
+                  file=sys.stderr, flush=True)
+            next_note += progress_s
         while i < len(requests) and (not realtime or requests[i][0] <= now):
             _, lo, hi = requests[i]
             ids[server.submit(lo, hi)] = (lo, hi)
