@@ -7,7 +7,10 @@ on a sample of their ranges.
     at most 2^22;
   * serving: Poisson arrivals at 100/s for 10 s, seed 1 (wwlib.serving_workload): the
     SHA-256 of the whole request list, its first 16 requests, and reference summaries of
-    16 sampled requests of width at most 2^24.
+    16 sampled requests of width at most 2^24;
+  * serving_streams: the 20/s and 50/s streams (10 s, seed 1): SHA-256, and reference
+    summaries of 8 sampled requests of width at most 2^24 each; plus the two narrowest
+    requests of the 100/s stream wider than 2^28.
 
     python tests/golden/make_workloads.py          (needs oracle/_ref built)
 """
@@ -63,10 +66,25 @@ def main():
                           for i in spick],
         },
     }
+    streams = {}
+    for rate in (20.0, 50.0):
+        reqs = wwlib.serving_workload(rate=rate, duration_s=10.0, seed=1)
+        sm = [i for i, (_, lo, hi) in enumerate(reqs) if hi - lo <= (1 << 24)]
+        pk = sm[:: max(1, len(sm) // 8)][:8]
+        streams[str(rate)] = {
+            "rate": rate, "duration_s": 10.0, "seed": 1, "count": len(reqs),
+            "sha256": digest(reqs),
+            "reference": [dict(index=i, **summarize(reqs[i][1], reqs[i][2])) for i in pk]}
+    wide = sorted((hi - lo, i) for i, (_, lo, hi) in enumerate(serving) if hi - lo > (1 << 28))[:2]
+    streams["100.0_wide"] = {
+        "rate": 100.0, "duration_s": 10.0, "seed": 1,
+        "reference": [dict(index=i, **summarize(serving[i][1], serving[i][2])) for _, i in wide]}
+    fixture["serving_streams"] = streams
     with open(os.path.join(HERE, "workloads.json"), "w") as f:
         json.dump(fixture, f, indent=0)
     print(f"batch: {len(batch)} ranges, {len(pick)} summaries; serving: {len(serving)} "
-          f"requests, {len(spick)} summaries")
+          f"requests, {len(spick)} summaries; streams: "
+          + ", ".join(f"{k}: {len(v['reference'])}" for k, v in streams.items()))
 
 
 if __name__ == "__main__":

@@ -125,6 +125,28 @@ def test_prp_list_vs_oracle_and_sieve(gctx, lo, hi):
         assert primes == [p for p in got if oracle.is_prime(p)]
 
 
+def test_prp_list_whole_range_127_1e8(gctx):
+    """The whole [127, 10^8) range's PRP list (5,761,822 entries) from the sieve + strong PRP
+    kernel equals the oracle's list, and after primality filtering it is exactly the
+    sieve of Eratosthenes' primes of the range (pi(10^8) - 30 = 5,761,425); the rest are
+    the range's base-2 strong pseudoprimes with no factor up to 113."""
+    import numpy as np
+    lo, hi = 127, 10**8
+    gctx.clear_results()
+    gctx.clear_prp_count()
+    gctx.getsegprimes(lo, hi)
+    gctx.finish()
+    got = np.sort(np.array(gctx.prps(), dtype=np.uint64))
+    want = np.array(oracle.prp_list(lo, hi), dtype=np.uint64)
+    assert got.shape == want.shape and (got == want).all()
+    primes = np.array(oracle.sieve_primes(lo, hi), dtype=np.uint64)
+    assert len(primes) == 5_761_425
+    assert np.isin(primes, got).all()
+    rest = np.setdiff1d(got, primes)
+    assert len(got) == len(primes) + len(rest)
+    assert all(oracle.is_prp(int(n)) and not oracle.is_prime(int(n)) for n in rest)
+
+
 def test_chunking_invariance(gctx):
     lo, hi = 10**13, 10**13 + 3 * (1 << 22) + 12345
     ref = gctx.run_range(lo, hi)

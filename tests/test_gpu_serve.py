@@ -111,3 +111,21 @@ def test_serve_budget_fragments_and_coalesces(gctx):
         plain = gctx.run_range(a, b)
         assert (done[rid]["prp_count"], done[rid]["checksum"], done[rid]["records"]) == (
             plain.prp_count, plain.checksum, plain.results)
+
+
+@pytest.mark.parametrize("stream", ["20.0", "50.0", "100.0_wide"])
+def test_serving_streams_vs_reference_code(gctx, stream):
+    """Requests of the 20 and 50 req/s streams, and the two narrowest requests of the 100
+    req/s stream wider than 2^28, against the reference's own CPU path
+    (tests/golden/workloads.json, written by tests/golden/make_workloads.py)."""
+    w = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                                    "workloads.json")))["serving_streams"][stream]["reference"]
+    if stream.endswith("wide"):
+        assert all(r["hi"] - r["lo"] > (1 << 28) for r in w)
+    with wwlib.Server(gctx) as srv:
+        done, ids, _ = wwlib.serve(srv, [(0.0, r["lo"], r["hi"]) for r in w], realtime=False)
+    by_range = {ids[rid]: c for rid, c in done.items()}
+    for r in w:
+        c = by_range[(r["lo"], r["hi"])]
+        assert (c["prp_count"], c["checksum"], len(c["records"])) == (
+            r["prp_count"], r["checksum"], r["cpu_result_count"]), r["index"]

@@ -36,3 +36,18 @@ def test_serving_workload_is_pinned():
     assert [list(r) for r in reqs[:16]] == s["first"]
     assert _digest(reqs) == s["sha256"]
     assert all(0 <= t < s["duration_s"] and lo & 1 and hi > lo for t, lo, hi in reqs)
+
+
+def test_serving_streams_are_pinned():
+    """The 20 and 50 req/s streams and the wide requests of the 100 req/s stream whose
+    reference summaries the GPU suite checks."""
+    for key in ("20.0", "50.0"):
+        s = W["serving_streams"][key]
+        reqs = wwlib.serving_workload(rate=s["rate"], duration_s=s["duration_s"], seed=s["seed"])
+        assert len(reqs) == s["count"] and _digest(reqs) == s["sha256"]
+        for r in s["reference"]:
+            assert tuple(reqs[r["index"]][1:]) == (r["lo"], r["hi"])
+    wide = W["serving_streams"]["100.0_wide"]
+    reqs = wwlib.serving_workload(rate=100.0, duration_s=10.0, seed=1)
+    for r in wide["reference"]:
+        assert tuple(reqs[r["index"]][1:]) == (r["lo"], r["hi"]) and r["hi"] - r["lo"] > 1 << 28
