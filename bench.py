@@ -65,30 +65,67 @@ def mul32_path_executed(b: int) -> int:
     return 78 * b - 117
 
 
-CPU_SLICE = 1 << 27  # 16 slices of 2^27: about 12 s of reference CPU work on 16 threads
+# The box's CPU share for one GPU: worker pools are sized to at most this many threads
+# whatever the host reports (nproc and os.cpu_count() show the whole machine there).
+CPU_SHARE = 16
+CPU_SLICE3 = 1 << 22  # the 2^63 leg: 16 slices of 2^22 at offsets k*2^29
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def cpu_baseline():
-    """The reference's own CPU path (compiled into oracle/_ref) on 16 sampled 2^27-wide
-    slices of the workload, [2^63 + k*2^29, +2^27) for k = 0..15, each slice split over
-    the threads."""
+    """The reference's own CPU path (wwcpu.cpp + prime.h compiled into oracle/_ref, the
+    restated loop of main.cpp:648-697 around them; the oracle's C port when that is not
+    built) on the three samples of the plan: the whole range [127, 1e8), the slice
+    [2^50, 2^50 + 2^24), and 16 slices [2^63 + k 2^29, +2^22) of the workload range, each
+    split into contiguous pieces over the threads. `value` is the 2^63 leg's rate (the
+    metric's domain); every leg is reported with its own rate."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
 
-    threads = min(16, os.cpu_count() or 1)
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except AttributeError:
+        allowed = os.cpu_count() or 1
+    threads = max(1, min(allowed, CPU_SHARE))
     kind = "reference" if oracle.ref_lib() is not None else "port"
+
+    def run(lo, hi):
+        if kind == "reference":
+            return oracle.ref_range(lo, hi, threads)[0]
+        return oracle.range_summary(lo, hi, threads)[0]
+
+    legs = {}
     t0 = time.perf_counter()
-    total = 0
+    prp = run(127, 10**8)
+    dt = time.perf_counter() - t0
+    legs["range_127_1e8"] = {"prps": prp, "seconds": dt, "value": prp / dt}
+    t0 = time.perf_counter()
+    prp = run(1 << 50, (1 << 50) + (1 << 24))
+    dt = time.perf_counter() - t0
+    legs["slice_2p50_w2p24"] = {"prps": prp, "seconds": dt, "value": prp / dt}
+    t0 = time.perf_counter()
+    prp = 0
     for k in range(16):
         lo = LO + k * (1 << 29)
-        if kind == "reference":
-            prp = oracle.ref_range(lo, lo + CPU_SLICE, threads)[0]
-        else:
-            prp = oracle.range_summary(lo, lo + CPU_SLICE, threads)[0]
-        total += prp
+        prp += run(lo, lo + CPU_SLICE3)
     dt = time.perf_counter() - t0
-    return {"value": total / dt, "unit": "tested primes/s", "cores": threads, "kind": kind,
-            "sample": f"16 slices [2^63 + k*2^29, +2^27), k=0..15: {total} PRPs in {dt:.2f} s"}
+    legs["slices_2p63_16x2p22"] = {"prps": prp, "seconds": dt, "value": prp / dt}
+    head = legs["slices_2p63_16x2p22"]
+    return {"value": head["value"], "unit": "tested primes/s", "cores": threads, "kind": kind,
+            "sample": f"16 slices [2^63 + k*2^29, +2^22), k=0..15: {head['prps']} PRPs in "
+                      f"{head['seconds']:.2f} s on {threads} threads",
+            "nproc": os.cpu_count(), "affinity_cpus": allowed, "cpu_share": CPU_SHARE,
+            "cpu_model": cpu_model(), "legs": legs}
 
 
 def run_plain(ctx, steps, warmup):
@@ -181,18 +218,24 @@ def run_batched_workload(ctx):
 
 def run_serving(ctx, rates, seconds):
     """Serving form: Poisson request streams (seed 1, widths 2^20..2^30, starts in
-    [2^32, 2^64 - 2^30]) submitted in real time; p50/p99 latency (submit to result) and
-    tested primes/s; 16 requests per rate re-run in the plain form must agree."""
+    [2^32, 2^64 - 2^30]) submitted in real time against the default 64 GiB HBM budget;
+    p50/p99 latency (submit to result) and tested primes/s; 16 requests per rate re-run in
+    the plain form must agree. The stated configuration runs 10 minutes per rate
+    (--serve-only --serve-seconds 600); the default bench run is a short one."""
     out = {}
     for rate in rates:
         reqs = wwlib.serving_workload(rate=rate, duration_s=seconds, seed=1)
         with wwlib.Server(ctx) as srv:
-            done, ids, wall = wwlib.serve(srv, reqs, realtime=True)
+            done, ids, wall = wwlib.serve(srv, reqs, realtime=True,
+                                          progress_s=60.0 if seconds > 60 else 0.0)
             st = srv.stats()
         tested = sum(c["prp_count"] for c in done.values())
         span = max(c["t_done"] for c in done.values()) - min(c["t_submit"] for c in done.values())
         rec = {"requests": len(done), "tested": tested, "tested_per_s": tested / span,
-               "offered_req_per_s": rate, "seconds": seconds, "ticks": st.ticks}
+               "offered_req_per_s": rate, "seconds": seconds,
+               "run": "stated (10 min)" if seconds >= 600 else f"short ({seconds:g} s)",
+               "ticks": st.ticks, "peak_active": st.peak_active,
+               "peak_reserved_bytes": st.peak_reserved_bytes}
         rec.update(wwlib.latency_summary(done))
         for rid in sorted(ids)[:: max(1, len(ids) // 16)][:16]:
             lo, hi = ids[rid]
@@ -223,6 +266,9 @@ def main():
                          "bound 2^20, full occupancy)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="also launch one plain copy kernel of BYTES (counter calibration)")
+    ap.add_argument("--serve-only", action="store_true",
+                    help="run only the serving legs (--serve-rates, --serve-seconds) and print "
+                         "their JSON line")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -230,6 +276,13 @@ def main():
         sys.exit(2)
 
     info = wwlib.device_info(0)
+    if args.serve_only:
+        with wwlib.Context(0, list_capacity=wwlib.list_bound(LO, LO + CHUNK)) as ctx:
+            serving = run_serving(ctx, [float(r) for r in args.serve_rates.split(",")],
+                                  args.serve_seconds)
+        print(json.dumps({"metric": "serving p50/p99 latency (Poisson, seed 1)",
+                          "device": info.name.decode(), "serving": serving}), flush=True)
+        return
     peak = wwlib.peak_mul32(0, 0)
     cap = wwlib.list_bound(LO, LO + CHUNK)
     copy_ms = wwlib.copy_probe(args.copy_probe) if args.copy_probe else None
@@ -246,7 +299,15 @@ def main():
                       "by_presieve_bound": {
                           str(b): max(t["prps_per_s"] for t in trials
                                       if t["presieve_bound"] == b)
-                          for b in sorted({t["presieve_bound"] for t in trials})}}
+                          for b in sorted({t["presieve_bound"] for t in trials})},
+                      "by_waves_per_group": {
+                          str(w): max(t["prps_per_s"] for t in trials
+                                      if t["waves_per_group"] == w)
+                          for w in sorted({t["waves_per_group"] for t in trials})},
+                      "by_segment_bits": {
+                          str(b): max(t["prps_per_s"] for t in trials
+                                      if t["segment_bits"] == b)
+                          for b in sorted({t["segment_bits"] for t in trials})}}
         if args.form in ("both", "persistent"):
             pers = run_persistent(ctx, args.steps, args.warmup)
         if args.form in ("both", "plain"):
