@@ -35,11 +35,13 @@ BITS = 64  # every p in the range has bit length 64
 # chunk (the 8-byte list read, register spills and the result atomics).
 TRAFFIC_BYTES_PER_ENTRY = 8.37
 # k_batch keeps candidates and PRPs on chip (LDS queues). Its algorithmic HBM traffic is the
-# wide pre-sieve's 64 words (512 B) per 8192-integer segment, read once; the rest is the
-# deep-sieve table, span lookups and the cursor/slot atomics (each atomic reads as a 32-byte
-# write). Measured the same way at the defaults (deep bound 2048, pre-sieve 2^20):
-# 575.8 MB fetched + 19.2 MB written per step.
-BATCH_TRAFFIC_BYTES_PER_PRP = 3.025
+# wide pre-sieve's 64 words (512 B) per 8192-integer segment, read once; the rest is register
+# spill reloads, the deep-sieve table, span lookups and the cursor/slot atomics (each atomic
+# reads as a 32-byte write). Measured the same way at the defaults (deep bound 2048,
+# pre-sieve 2^20, 4-wave workgroups): 636.8 MB fetched + 23.1 MB written per step.
+BATCH_TRAFFIC_BYTES_PER_PRP = 3.355
+TRAFFIC_SOURCE = ("rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE, calibrated on a 1 GiB copy "
+                  "(profiles/k_batch_persistent_2p63_w2p33.md)")
 PRESIEVE_BYTES_PER_SEGMENT = 512
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
 
@@ -349,6 +351,7 @@ def main():
             "unit": "TMUL32/s", "frac": ach / peak,
             "traffic": pers["prps_per_step"] * BATCH_TRAFFIC_BYTES_PER_PRP,
             "traffic_algorithmic": ((HI - LO) // 8192) * PRESIEVE_BYTES_PER_SEGMENT,
+            "traffic_source": TRAFFIC_SOURCE,
             "per_unit": f"{mul32_path(BITS)} MUL32 per tested prime (A(b) = 99b-123 at b=64: "
                         "PRP + Wieferich + Wall-Sun-Sun in one kernel, counted as the "
                         "reference's algorithm spends them)",
