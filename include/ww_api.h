@@ -436,7 +436,7 @@ int ww_server_destroy(ww_server *s);
    range as the plain form gives it), d_records (record_cap records, pad = range index;
    order unspecified) and d_record_count (one uint32_t: records produced, which may exceed
    record_cap). It calls no hipMalloc, hipFree or allocating library routine: the span
-   table, segment prefix and kernel state (about 160 KB) live in one fixed device array
+   table, segment prefix and kernel state (about 200 KB) live in one fixed device array
    per GPU, whose address is fetched when a context is opened on that GPU. Every call on
    a GPU shares that array, so calls that overlap on one GPU (from contexts on different
    streams) race; calls on one context are ordered on its stream. It runs without the wide
