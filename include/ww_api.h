@@ -135,6 +135,39 @@ int ww_get_deep_bound(ww_ctx *ctx, uint32_t *bound);
 int ww_set_presieve_bound(ww_ctx *ctx, uint32_t bound);
 int ww_get_presieve_bound(ww_ctx *ctx, uint32_t *bound);
 
+/* Smallest range width that gets the bucket sieve in batches created afterwards (see
+   ww_set_bucket_sieve). */
+#define WW_DEFAULT_BUCKET_MIN_WIDTH (1ull << 30)
+
+/* The bucket sieve of the persistent form, for ranges at least min_width wide in
+   batches created afterwards (UINT64_MAX: none). No counterpart in the reference. Such a
+   range is sieved by every prime up to sqrt(hi - 1): the deep sieve and the wide pre-sieve
+   take the primes up to the pre-sieve bound, and one more kernel drops the hits of the
+   primes above it, up to 2^32, into per-chunk buckets that the pre-sieve applies. Every
+   tier spares (and flags) the base-2 pseudoprime class of each prime, so a candidate that no
+   tier dropped or flagged is a prime and skips the strong PRP test; a flagged one is tested
+   as before. The PRP list, checksums and records are unchanged. It takes effect where the
+   deep bound is at least 1024, the pre-sieve bound at least 2^20 and the pre-sieve segment
+   2^20 bits (the defaults); elsewhere such a batch runs as without it. Costs per range
+   about 4.7 bytes of device memory per integer of width (4.5 GB for a 2^33-wide range) and,
+   on first use, an 813 MB table of the odd primes below 2^32 built on the device.
+   Default WW_DEFAULT_BUCKET_MIN_WIDTH. */
+int ww_set_bucket_sieve(ww_ctx *ctx, uint64_t min_width);
+int ww_get_bucket_sieve(ww_ctx *ctx, uint64_t *min_width);
+
+/* The context's table of the odd primes below 2^32 (built on first use, 203,280,220
+   entries): copies entries [first, first + n) to out (host) and writes the entry count to
+   *total. Verification; blocking. */
+int ww_prime_table(ww_ctx *ctx, uint64_t first, uint64_t n, uint32_t *out, uint64_t *total);
+
+/* The bucket sieve's walk of one sieving prime, on the host with the code the device runs
+   (verification): the multiples n = ell*k in [g0, g0 + 2 nbits) with k prime to 210 (g0
+   odd, 2^20 < ell < 2^32 prime), as bit indices (n - g0) / 2 with a 1 in rescue[] where
+   2^(n-1) = 1 (mod ell) (the prime's base-2 pseudoprime class). Writes up to cap of each;
+   *count = the number of hits. */
+int ww_cpu_bucket_hits(uint64_t g0, uint64_t nbits, uint32_t ell, uint64_t *bits,
+                       uint8_t *rescue, uint64_t cap, uint64_t *count);
+
 /* Launch shape of the persistent kernel for launches issued afterwards: waves per
    workgroup (2, 4 or 8) and the wide pre-sieve's segment, the LDS bitmap one workgroup
    marks at a time, as a bit count (15..20: 2^15 to 2^20 odd numbers). No counterpart in
@@ -283,8 +316,9 @@ int ww_batch_create(ww_ctx *ctx, const ww_range *ranges, uint32_t n, uint32_t re
 int ww_batch_set_grid(ww_batch *b, int blocks_per_cu, uint32_t claim);
 
 /* Stream-ordered search of every range of the batch: the wide pre-sieve of its segments
-   (when the context's pre-sieve bound is set), then one persistent kernel that clears the
-   batch's sums and whose waves claim 8192-integer segments from a shared cursor.
+   (when the context's pre-sieve bound is set; with the bucket sieve first for the ranges
+   it covers), then one persistent kernel that clears the batch's sums and whose waves
+   claim 8192-integer segments from a shared cursor.
    Allocates nothing and records no memset, so it may run under graph capture. */
 int ww_batch_launch(ww_ctx *ctx, ww_batch *b);
 
@@ -304,6 +338,19 @@ int ww_batch_time(ww_ctx *ctx, ww_batch *b, double *ms);
 /* The same launch split: the pre-sieve kernel alone and the persistent search kernel
    alone (their sum is ww_batch_time). Either pointer may be null. */
 int ww_batch_kernel_times(ww_ctx *ctx, ww_batch *b, double *presieve_ms, double *search_ms);
+
+/* The batch's bucket sieve: ranges it covers, whether the last launch used it (the
+   context's sieve tiers allowed it), entries per bucket, device bytes of its buckets, and
+   chunks whose buckets overflowed over all launches so far (those chunks' candidates were
+   all PRP-tested: exact, only slower). */
+typedef struct {
+    uint32_t spans;
+    uint32_t active;
+    uint32_t bucket_cap;
+    uint32_t overflows;
+    uint64_t bytes;
+} ww_bucket_info;
+int ww_batch_buckets(ww_ctx *ctx, ww_batch *b, ww_bucket_info *out);
 
 /* Waits for the batch's launches on its context's stream (not for the whole device),
    then frees it. Destroy batches before their context. */

@@ -21,7 +21,7 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wall", "-Wno-unused-command-line-argument"]
 
-LIB_SOURCES = ["ww_kernels.hip", "ww_batch.hip", "ww_presieve.hip", "ww_api.cpp",
+LIB_SOURCES = ["ww_kernels.hip", "ww_batch.hip", "ww_presieve.hip", "ww_bucket.hip", "ww_api.cpp",
                "ww_server.cpp", "ww_cpu.cpp", "ww_selftest.cpp", "ww_peak.hip",
                "ww_noalloc.hip"]
 HOST_SOURCES = ["wwocl_main.cpp", "wwocl_checks.cpp"]

@@ -21,6 +21,7 @@
 #include "../../include/ww_api.h"
 #include "ww_arith.h"
 #include "ww_batch.h"
+#include "ww_bucket.h"
 static_assert(sizeof(ww_deep_prime) == sizeof(ww::DeepPrime), "deep table layout");
 #include "ww_kernels.h"
 
@@ -71,6 +72,13 @@ struct ww_ctx {
     uint32_t presieve_chunk = ww::kPresieveMaxChunkSegs;
     ww_tuning last_tuning{}; /* the tuner's (or a loaded cache's) best, for ww_tuning_save */
     uint64_t default_chunk = 0;
+    /* bucket sieve: every odd prime below 2^32 (built on first use), the number of table
+       primes in each 2^21-integer block before it (for index bounds), and the smallest
+       range width that gets the bucket sieve in batches created from now on */
+    uint32_t *d_primes32 = nullptr;
+    uint64_t nprimes32 = 0;
+    std::vector<uint64_t> primes32_prefix;
+    uint64_t bucket_min_width = WW_DEFAULT_BUCKET_MIN_WIDTH;
     /* event pairs of launches whose durations have not been collected yet */
     std::vector<hipEvent_t> ev_sieve, ev_tests;
     double ms_sieve = 0, ms_tests = 0;
@@ -159,6 +167,95 @@ void wide_span(ww_ctx *c, uint32_t *first, uint32_t *count)
     while (e < c->wide_ells.size() && c->wide_ells[e] <= c->presieve_bound) ++e;
     *first = a;
     *count = e - a;
+}
+
+/* Builds the table of every odd prime below 2^32 on the device (once per context): a
+   counting pass of k_primes32, the exclusive prefix of the per-block counts on the host,
+   then the writing pass. */
+int ensure_primes32(ww_ctx *c)
+{
+    if (c->d_primes32) return WW_OK;
+    constexpr uint32_t kBlocks = 2048; /* 2^21 integers each */
+    std::vector<uint32_t> small;
+    {
+        std::vector<uint8_t> comp(1u << 16, 0);
+        for (uint32_t i = 3; i < (1u << 16); i += 2) {
+            if (comp[i]) continue;
+            small.push_back(i);
+            for (uint32_t j = i * i; j < (1u << 16); j += 2 * i) comp[j] = 1;
+        }
+    }
+    uint32_t *d_small = nullptr, *d_counts = nullptr;
+    uint64_t *d_off = nullptr;
+    std::vector<uint32_t> counts(kBlocks);
+    std::vector<uint64_t> off(kBlocks + 1, 0);
+    hipError_t e = hipMalloc(&d_small, small.size() * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc(&d_counts, kBlocks * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc(&d_off, kBlocks * sizeof(uint64_t));
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(d_small, small.data(), small.size() * sizeof(uint32_t),
+                           hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess)
+        e = ww::launch_primes32(d_small, (uint32_t)small.size(), kBlocks, d_counts, nullptr, nullptr,
+                                c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(counts.data(), d_counts, kBlocks * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                           c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) {
+        for (uint32_t b = 0; b < kBlocks; ++b) off[b + 1] = off[b] + counts[b];
+        e = hipMalloc(&c->d_primes32, off[kBlocks] * sizeof(uint32_t));
+    }
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(d_off, off.data(), kBlocks * sizeof(uint64_t), hipMemcpyHostToDevice,
+                           c->stream);
+    if (e == hipSuccess)
+        e = ww::launch_primes32(d_small, (uint32_t)small.size(), kBlocks, d_counts, d_off,
+                                c->d_primes32, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (d_small) (void)hipFree(d_small);
+    if (d_counts) (void)hipFree(d_counts);
+    if (d_off) (void)hipFree(d_off);
+    if (e != hipSuccess) {
+        if (c->d_primes32) (void)hipFree(c->d_primes32);
+        c->d_primes32 = nullptr;
+        return fail(WW_E_HIP, "prime table: %s", hipGetErrorString(e));
+    }
+    c->nprimes32 = off[kBlocks];
+    c->primes32_prefix = off;
+    return WW_OK;
+}
+
+/* isqrt(x) for 64-bit x. */
+uint64_t isqrt64(uint64_t x)
+{
+    uint64_t r = (uint64_t)std::sqrt((double)x);
+    while (r > 0 && (r > 0xffffffffull || r * r > x)) --r;
+    while (r < 0xffffffffull && (r + 1) * (r + 1) <= x) ++r;
+    return r;
+}
+
+/* Whether the context's sieve tiers allow known primes (no strong PRP test for candidates
+   sieved up to sqrt(hi)): the deep sieve reaches the pre-sieve's lowest prime (1024), the
+   pre-sieve reaches 2^20 (the bucket sieve's lowest prime, which its capacities assume) and
+   marks 2^20-bit chunks (the buckets' unit). */
+bool bucket_config_ok(const ww_ctx *c)
+{
+    return c->ndeep && c->deep_bound >= ww::kPresieveMinPrime &&
+           c->presieve_bound >= (1u << ww::kBucketChunkBits) &&
+           c->presieve_chunk == (1u << (ww::kBucketChunkBits - 12));
+}
+
+/* Table index of the first prime above the pre-sieve bound: the odd primes up to 113 (29)
+   and the wide table's primes up to the bound. */
+uint64_t bucket_first_index(const ww_ctx *c)
+{
+    uint64_t n = 29;
+    for (uint32_t l : c->wide_ells) {
+        if (l > c->presieve_bound) break;
+        ++n;
+    }
+    return n;
 }
 
 } // namespace
@@ -373,6 +470,7 @@ int ww_close(ww_ctx *c)
     for (hipEvent_t e : c->ev_tests) (void)hipEventDestroy(e);
     if (c->d_deep) (void)hipFree(c->d_deep);
     if (c->d_wide) (void)hipFree(c->d_wide);
+    if (c->d_primes32) (void)hipFree(c->d_primes32);
     if (c->d_fstate) (void)hipFree(c->d_fstate);
     if (c->d_fsum) (void)hipFree(c->d_fsum);
     if (c->d_fres) (void)hipFree(c->d_fres);
@@ -446,6 +544,41 @@ int ww_get_presieve_bound(ww_ctx *c, uint32_t *bound)
     WW_LOCK(c);
     if (!c || !bound) return fail(WW_E_ARG, "null argument");
     *bound = c->presieve_bound;
+    return WW_OK;
+}
+
+int ww_set_bucket_sieve(ww_ctx *c, uint64_t min_width)
+{
+    WW_LOCK(c);
+    if (!c) return fail(WW_E_ARG, "null context");
+    c->bucket_min_width = min_width;
+    return WW_OK;
+}
+
+int ww_get_bucket_sieve(ww_ctx *c, uint64_t *min_width)
+{
+    WW_LOCK(c);
+    if (!c || !min_width) return fail(WW_E_ARG, "null argument");
+    *min_width = c->bucket_min_width;
+    return WW_OK;
+}
+
+int ww_prime_table(ww_ctx *c, uint64_t first, uint64_t n, uint32_t *out, uint64_t *total)
+{
+    WW_LOCK(c);
+    int rc = use(c);
+    if (rc) return rc;
+    if ((n && !out) || !total) return fail(WW_E_ARG, "null argument");
+    if ((rc = ensure_primes32(c))) return rc;
+    *total = c->nprimes32;
+    if (first > c->nprimes32 || n > c->nprimes32 - first)
+        return fail(WW_E_ARG, "entries [%llu, +%llu) beyond the table's %llu", (unsigned long long)first,
+                    (unsigned long long)n, (unsigned long long)c->nprimes32);
+    if (n) {
+        HIP_TRY(hipMemcpyAsync(out, c->d_primes32 + first, n * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                               c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+    }
     return WW_OK;
 }
 
@@ -769,7 +902,18 @@ struct ww_batch {
     uint32_t graph_ndeep = 0;
     uint64_t graph_wide = 0;
     const void *graph_wide_table = nullptr; /* the table the graph reads (rebuilds move it) */
+    bool graph_buckets = false;
     hipEvent_t ev0 = nullptr, evm = nullptr, ev1 = nullptr;
+    /* bucket sieve (ranges at least the context's bucket_min_width wide when the batch was
+       made): the bucket-span table (host copy and device), the buckets, and the words of
+       candidates that need the strong PRP test (64 per segment, like d_pre) */
+    std::vector<ww::BucketSpan> bspans;
+    ww::BucketSpan *d_bspans = nullptr;
+    uint32_t *d_bent = nullptr, *d_bcnt = nullptr;
+    uint32_t bcap = 0;
+    uint32_t *d_bover = nullptr; /* chunks whose buckets overflowed, summed over launches */
+    uint64_t *d_prer = nullptr;
+    uint64_t bucket_first = 0; /* table index the device's bucket-span table was made for */
 };
 
 namespace {
@@ -790,22 +934,63 @@ bool batch_presieves(ww_ctx *c, ww_batch *b)
     return count && b->d_pre;
 }
 
-int batch_enqueue_presieve(ww_ctx *c, ww_batch *b)
+/* Whether this launch uses the bucket sieve (known primes skip the strong PRP test): the
+   batch has bucket spans, pre-sieves, and the context's sieve tiers allow it. */
+bool batch_buckets(ww_ctx *c, ww_batch *b)
+{
+    return !b->bspans.empty() && b->d_prer && batch_presieves(c, b) && bucket_config_ok(c);
+}
+
+/* The bucket-span table for the context's pre-sieve bound (the sieving primes start above
+   it): built on the host and copied with a wait, so never under capture. Builds the prime
+   table on first use. */
+int batch_prepare_buckets(ww_ctx *c, ww_batch *b)
+{
+    int rc = ensure_primes32(c);
+    if (rc) return rc;
+    const uint64_t first = bucket_first_index(c);
+    if (first == b->bucket_first) return WW_OK;
+    for (ww::BucketSpan &bs : b->bspans) {
+        /* all table primes of the 2^21-integer blocks up to the top's; the kernel stops at
+           the first prime above the top */
+        const uint64_t blk = ((uint64_t)bs.top >> 21) + 1;
+        bs.pidx0 = first;
+        bs.pidx1 = c->primes32_prefix[std::min<uint64_t>(blk, c->primes32_prefix.size() - 1)];
+        if (bs.pidx1 < bs.pidx0) bs.pidx1 = bs.pidx0;
+    }
+    HIP_TRY(hipMemcpyAsync(b->d_bspans, b->bspans.data(), b->bspans.size() * sizeof(ww::BucketSpan),
+                           hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    b->bucket_first = first;
+    return WW_OK;
+}
+
+int batch_enqueue_presieve(ww_ctx *c, ww_batch *b, bool buckets)
 {
     uint32_t first, count;
     wide_span(c, &first, &count);
     const uint64_t chunks = (b->nseg + c->presieve_chunk - 1) / c->presieve_chunk;
     const int pgrid = (int)std::min<uint64_t>(chunks, (uint64_t)c->cus);
+    if (!buckets) {
+        HIP_TRY(ww::launch_presieve(b->d_spans, b->d_prefix, b->n, c->d_wide + first, count,
+                                    b->d_pre, pgrid > 0 ? pgrid : 1, c->stream, c->presieve_chunk));
+        return WW_OK;
+    }
+    HIP_TRY(ww::launch_bucket_fill(b->d_spans, b->d_prefix, b->d_bspans, (uint32_t)b->bspans.size(),
+                                   c->d_primes32, b->d_bent, b->d_bcnt, b->bcap, ww::kBucketGroups,
+                                   c->stream));
+    ww::BucketArgs bk{b->d_bspans, b->d_bent, b->d_bcnt, b->bcap, b->d_bover};
     HIP_TRY(ww::launch_presieve(b->d_spans, b->d_prefix, b->n, c->d_wide + first, count, b->d_pre,
-                                pgrid > 0 ? pgrid : 1, c->stream, c->presieve_chunk));
+                                pgrid > 0 ? pgrid : 1, c->stream, c->presieve_chunk, b->d_prer, &bk));
     return WW_OK;
 }
 
-int batch_enqueue_search(ww_ctx *c, ww_batch *b, int grid, bool pre)
+int batch_enqueue_search(ww_ctx *c, ww_batch *b, int grid, bool pre, bool buckets)
 {
     HIP_TRY(ww::launch_batch(b->d_spans, b->d_prefix, b->n, b->d_state, b->d_sums, b->n, b->d_res,
                              b->res_cap, grid, b->claim, c->d_deep, c->ndeep,
-                             pre ? b->d_pre : nullptr, c->stream, c->waves));
+                             pre ? b->d_pre : nullptr, c->stream, c->waves,
+                             buckets ? b->d_prer : nullptr));
     return WW_OK;
 }
 
@@ -878,6 +1063,50 @@ int ww_batch_create(ww_ctx *c, const ww_range *ranges, uint32_t n, uint32_t resu
     b->nseg = prefix[n];
     if (e == hipSuccess && c->presieve_bound && b->nseg)
         e = hipMalloc(&b->d_pre, b->nseg * ww::kPresieveWordsPerSeg * sizeof(uint64_t));
+    /* bucket spans: ranges at least bucket_min_width wide (and at most kBucketMaxChunks
+       chunks), with rows of buckets sized for primes from 2^20 to sqrt(hi - 1): per
+       2^20-bit chunk, 2^20 x 48/105 (the cofactor wheel) x sum(1/l) hits, sum(1/l) from
+       Mertens' ln ln top - ln ln 2^20 (within 0.005 of the true sum), spread over
+       kBucketGroups buckets; capacity 1.25x the mean plus 96, and an overflow only costs
+       that chunk its known primes */
+    if (e == hipSuccess && c->presieve_bound && b->nseg) {
+        uint64_t rows = 0;
+        double mean = 0;
+        for (uint32_t i = 0; i < n; ++i) {
+            const uint64_t nseg = prefix[i + 1] - prefix[i];
+            if (ranges[i].hi - ranges[i].lo < c->bucket_min_width || nseg == 0) continue;
+            const uint64_t bit0 = prefix[i] * 4096;
+            const uint64_t cf = bit0 >> ww::kBucketChunkBits;
+            const uint64_t nch = ((bit0 + nseg * 4096 - 1) >> ww::kBucketChunkBits) - cf + 1;
+            if (nch > ww::kBucketMaxChunks) continue;
+            ww::BucketSpan bs{};
+            bs.span = i;
+            bs.bit0 = bit0;
+            bs.chunk_first = (uint32_t)cf;
+            bs.nchunks = (uint32_t)nch;
+            bs.top = (uint32_t)isqrt64(ranges[i].hi - 1);
+            bs.cnt_base = rows * ww::kBucketGroups;
+            rows += nch;
+            const double lt = std::log((double)bs.top), l20 = 20.0 * std::log(2.0);
+            if (lt > l20) mean = std::max(mean, std::ldexp(1.0, 20) * 48.0 / 105.0 *
+                                                    std::log(lt / l20) / ww::kBucketGroups);
+            b->bspans.push_back(bs);
+            spans[i].pad = (uint32_t)b->bspans.size();
+        }
+        if (!b->bspans.empty()) {
+            b->bcap = (uint32_t)std::ceil(1.25 * mean + 96);
+            for (ww::BucketSpan &bs : b->bspans) bs.ent_base = bs.cnt_base * b->bcap;
+            const size_t nb = b->bspans.size();
+            e = hipMalloc(&b->d_bspans, nb * sizeof(ww::BucketSpan));
+            if (e == hipSuccess) e = hipMalloc(&b->d_bcnt, rows * ww::kBucketGroups * sizeof(uint32_t));
+            if (e == hipSuccess)
+                e = hipMalloc(&b->d_bent, rows * ww::kBucketGroups * (size_t)b->bcap * sizeof(uint32_t));
+            if (e == hipSuccess) e = hipMalloc(&b->d_bover, sizeof(uint32_t));
+            if (e == hipSuccess) e = hipMemsetAsync(b->d_bover, 0, sizeof(uint32_t), c->stream);
+            if (e == hipSuccess)
+                e = hipMalloc(&b->d_prer, b->nseg * ww::kPresieveWordsPerSeg * sizeof(uint64_t));
+        }
+    }
     if (e == hipSuccess)
         e = hipMemcpyAsync(b->d_spans, spans.data(), n * sizeof(BatchSpan), hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess)
@@ -910,10 +1139,12 @@ int ww_batch_launch(ww_ctx *c, ww_batch *b)
     if (rc) return rc;
     if (!b) return fail(WW_E_ARG, "null batch");
     const bool pre = batch_presieves(c, b);
+    const bool buckets = batch_buckets(c, b);
+    if (buckets && (rc = batch_prepare_buckets(c, b))) return rc;
     HIP_TRY(hipEventRecord(b->ev0, c->stream));
-    if (pre && (rc = batch_enqueue_presieve(c, b))) return rc;
+    if (pre && (rc = batch_enqueue_presieve(c, b, buckets))) return rc;
     HIP_TRY(hipEventRecord(b->evm, c->stream));
-    if ((rc = batch_enqueue_search(c, b, batch_grid(c, b), pre))) return rc;
+    if ((rc = batch_enqueue_search(c, b, batch_grid(c, b), pre, buckets))) return rc;
     HIP_TRY(hipEventRecord(b->ev1, c->stream));
     return WW_OK;
 }
@@ -929,22 +1160,25 @@ int ww_batch_launch_graph(ww_ctx *c, ww_batch *b)
     wide_span(c, &wfirst, &wcount);
     const uint64_t wide = ((uint64_t)wfirst << 32) | wcount;
     const bool pre = batch_presieves(c, b);
+    const bool buckets = batch_buckets(c, b);
+    if (buckets && (rc = batch_prepare_buckets(c, b))) return rc;
     if (b->exec && (b->graph_grid != grid || b->graph_claim != b->claim ||
                     b->graph_ndeep != c->ndeep || b->graph_wide != wide ||
                     b->graph_wide_table != c->d_wide || b->graph_waves != c->waves ||
-                    b->graph_chunk != c->presieve_chunk))
+                    b->graph_chunk != c->presieve_chunk || b->graph_buckets != buckets))
         drop_graphs(b);
     if (!b->exec) {
         if (pre && (rc = capture(c, &b->graph_pre, &b->exec_pre,
-                                 [&] { return batch_enqueue_presieve(c, b); }))) {
+                                 [&] { return batch_enqueue_presieve(c, b, buckets); }))) {
             drop_graphs(b);
             return rc;
         }
         if ((rc = capture(c, &b->graph, &b->exec,
-                          [&] { return batch_enqueue_search(c, b, grid, pre); }))) {
+                          [&] { return batch_enqueue_search(c, b, grid, pre, buckets); }))) {
             drop_graphs(b);
             return rc;
         }
+        b->graph_buckets = buckets;
         b->graph_grid = grid;
         b->graph_waves = c->waves;
         b->graph_chunk = c->presieve_chunk;
@@ -1016,6 +1250,27 @@ int ww_batch_kernel_times(ww_ctx *c, ww_batch *b, double *presieve_ms, double *s
     return WW_OK;
 }
 
+int ww_batch_buckets(ww_ctx *c, ww_batch *b, ww_bucket_info *out)
+{
+    WW_LOCK(c);
+    int rc = use(c);
+    if (rc) return rc;
+    if (!b || !out) return fail(WW_E_ARG, "null argument");
+    memset(out, 0, sizeof *out);
+    out->spans = (uint32_t)b->bspans.size();
+    out->active = batch_buckets(c, b) ? 1u : 0u;
+    out->bucket_cap = b->bcap;
+    uint64_t rows = 0;
+    for (const ww::BucketSpan &bs : b->bspans) rows += bs.nchunks;
+    out->bytes = rows * ww::kBucketGroups * ((uint64_t)b->bcap + 1) * sizeof(uint32_t);
+    if (b->d_bover) {
+        HIP_TRY(hipMemcpyAsync(&out->overflows, b->d_bover, sizeof(uint32_t), hipMemcpyDeviceToHost,
+                               c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+    }
+    return WW_OK;
+}
+
 int ww_batch_destroy(ww_batch *b)
 {
     if (!b) return WW_OK;
@@ -1028,6 +1283,11 @@ int ww_batch_destroy(ww_batch *b)
     if (b->evm) (void)hipEventDestroy(b->evm);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
     if (b->d_pre) (void)hipFree(b->d_pre);
+    if (b->d_prer) (void)hipFree(b->d_prer);
+    if (b->d_bspans) (void)hipFree(b->d_bspans);
+    if (b->d_bent) (void)hipFree(b->d_bent);
+    if (b->d_bcnt) (void)hipFree(b->d_bcnt);
+    if (b->d_bover) (void)hipFree(b->d_bover);
     if (b->d_res) (void)hipFree(b->d_res);
     if (b->d_state) (void)hipFree(b->d_state);
     if (b->d_sums) (void)hipFree(b->d_sums);

@@ -45,6 +45,25 @@ struct WidePrime {
     uint64_t m, bar_ell, bar_m;
 };
 
+/* One range of a batch that the bucket sieve covers (sieving primes above the wide
+   pre-sieve, up to sqrt(hi - 1)); its BatchSpan's pad holds 1 + its index in this table.
+   Its buckets are rows of `groups` (the fill's workgroups) per 2^20-bit chunk: row j,
+   workgroup w holds up to cap entries from entries[ent_base + (j groups + w) cap] on, and
+   its count at counts[cnt_base + j groups + w]. */
+struct BucketSpan {
+    uint64_t ent_base, cnt_base;
+    uint64_t pidx0, pidx1; /* sieving primes: table entries [pidx0, pidx1) up to top */
+    uint64_t bit0;         /* global odd-number index of the span's first segment */
+    uint32_t span;         /* index of the span in the span table */
+    uint32_t chunk_first;  /* bit0 >> 20 */
+    uint32_t nchunks;      /* chunks holding the span's segments */
+    uint32_t top;          /* isqrt(hi - 1): the largest prime to sieve by */
+};
+/* Chunks one bucket span may cover (a 2^34-wide range and one boundary chunk). */
+constexpr uint32_t kBucketMaxChunks = 8200;
+/* Workgroups of the bucket fill, and so buckets per chunk. */
+constexpr uint32_t kBucketGroups = 256;
+
 /* Words of the pre-sieve per segment (one per lane of a wave). */
 constexpr uint64_t kPresieveWordsPerSeg = 64;
 /* The pre-sieve skips primes up to this (their pseudoprime classes are too dense) ... */
@@ -65,12 +84,41 @@ bool batch_waves_ok(int waves);
 /* Resident workgroups per CU of the persistent kernel at that shape (0 on error). */
 int batch_blocks_per_cu(int waves = kDefaultBatchWaves);
 
+/* Buckets of a launch for k_presieve: the bucket spans, their entries and counts, the
+   entries per bucket, and a counter of chunks whose buckets overflowed. */
+struct BucketArgs {
+    const BucketSpan *spans;
+    const uint32_t *entries;
+    const uint32_t *counts;
+    uint32_t cap;
+    uint32_t *overflows;
+};
+
 /* The wide pre-sieve over all segments of the spans: out[64 s + w] = odd numbers to drop
    in word w of segment s (multiples of tab's primes, minus their pseudoprime classes).
-   No allocation, no memset: safe to capture into a graph. */
+   With `outr` it also writes outr[64 s + w] = the odd numbers that still need the strong
+   PRP test: those in a pseudoprime class of a sieving prime, and all of them in pieces
+   that are not sieved up to sqrt(hi) (no bucket span, short pieces, overflowed lists);
+   the pieces of bucket spans then also get the buckets of `bk` applied (chunk_segs must
+   be 256). No allocation, no memset: safe to capture into a graph. */
 hipError_t launch_presieve(const BatchSpan *spans, const uint64_t *prefix, uint32_t nspans,
                            const WidePrime *tab, uint32_t ntab, uint64_t *out, int grid,
-                           hipStream_t s, uint32_t chunk_segs = kPresieveMaxChunkSegs);
+                           hipStream_t s, uint32_t chunk_segs = kPresieveMaxChunkSegs,
+                           uint64_t *outr = nullptr, const BucketArgs *bk = nullptr);
+
+/* Every odd prime below 2^32 into out (203,280,220 entries), ascending: one launch with
+   offsets = null writes the prime count of each 2^21-integer block to counts, a second
+   with their exclusive prefix sums in offsets writes the primes. small = the odd primes
+   below 2^16. */
+hipError_t launch_primes32(const uint32_t *small, uint32_t nsmall, uint32_t nblocks,
+                           uint32_t *counts, const uint64_t *offsets, uint32_t *out,
+                           hipStream_t s);
+
+/* The bucket fill of the bucket spans (kBucketGroups workgroups; see BucketSpan). */
+hipError_t launch_bucket_fill(const BatchSpan *spans, const uint64_t *prefix,
+                              const BucketSpan *bspans, uint32_t nb, const uint32_t *primes,
+                              uint32_t *entries, uint32_t *counts, uint32_t cap,
+                              uint32_t groups, hipStream_t s);
 
 /* Writes the one-span table of the range [lo, hi) (segments from lo | 1) into span and
    prefix (device scratch), so that a single range needs no host copy. */
@@ -85,13 +133,16 @@ hipError_t launch_presieve_range(uint64_t lo, uint64_t hi, BatchSpan *span, uint
 /* Clears the state and the first `nslots` slot sums (0: none, so sums accumulate over
    launches), then runs the persistent kernel over all spans; each wave claims up to
    `claim` consecutive segments at a time. `pre` (may be null) is the wide pre-sieve of
-   the same spans. `waves` is the work-group shape (2, 4 or 8 waves).
+   the same spans. `waves` is the work-group shape (2, 4 or 8 waves). `prer` (may be
+   null: every candidate is tested) is the pre-sieve's words of candidates that need the
+   strong PRP test; the others are known primes. It requires the deep sieve up to at least
+   1024 (the pre-sieve's lowest prime), whose pseudoprime classes it flags as well.
    No allocation, no memset: safe to capture into a graph. */
 hipError_t launch_batch(const BatchSpan *spans, const uint64_t *prefix, uint32_t nspans,
                         BatchState *st, SlotSum *sums, uint32_t nslots, ww_result *res,
                         uint32_t res_cap, int grid, uint32_t claim, const DeepPrime *deep,
                         uint32_t ndeep, const uint64_t *pre, hipStream_t s,
-                        int waves = kDefaultBatchWaves);
+                        int waves = kDefaultBatchWaves, const uint64_t *prer = nullptr);
 
 /* Adds one single-range persistent launch to the plain form's counters and records: its
    checksum and tested count to the totals, and its records (up to bcap) after the ones

@@ -95,6 +95,57 @@ __device__ __forceinline__ void tests_round(u32 n, u32 &qn, u64 *wq, u32 *ws, u3
     }
 }
 
+/* Wave compaction of a segment's candidates as compact_by_class does it, split in two:
+   the candidates in `need` (those the strong PRP test must decide) go first, grouped by the
+   trailing zeros of n - 1 in the order `desc` gives; the others, known primes, follow as one
+   group tagged with kKnown, so that PRP rounds over them skip the test. The five counts
+   share one 64-bit scan in 12-bit fields (a segment holds at most kWaveCandCap < 4096
+   candidates). Returns the number of entries written, wave-uniform. Called by all 64
+   lanes. */
+constexpr u32 kKnown = 1u << 13;
+
+__device__ __forceinline__ u32 compact_split(u64 bits, u64 need, u64 G, u32 lane, uint16_t *wc,
+                                             u32 base, u32 tag, bool desc)
+{
+    const u32 g8 = (u32)((G - 1) >> 1) & 7u;
+    u64 cls[4];
+    cls[0] = 0x5555555555555555ull << ((1u - g8) & 1u); /* g + i odd: t = 1 */
+    cls[1] = 0x1111111111111111ull << ((2u - g8) & 3u); /* = 2 mod 4: t = 2 */
+    cls[2] = 0x0101010101010101ull << ((4u - g8) & 7u); /* = 4 mod 8: t = 3 */
+    cls[3] = 0x0101010101010101ull << ((0u - g8) & 7u); /* = 0 mod 8: t >= 4 */
+    const u64 test = bits & need, known = bits & ~need;
+    u64 cnt = (u64)__popcll(known) << 48;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) cnt |= (u64)__popcll(test & cls[k]) << (12 * k);
+    u64 incl = cnt;
+    for (int d = 1; d < 64; d <<= 1) {
+        u64 v = __shfl_up(incl, d, 64);
+        if (lane >= (u32)d) incl += v;
+    }
+    const u64 tot = uniform64(__shfl(incl, 63, 64));
+    const u64 excl = incl - cnt;
+    u32 at = base;
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) {
+        const int k = desc ? 3 - kk : kk;
+        u32 pos = at + (u32)((excl >> (12 * k)) & 0xfffu);
+        u64 b = test & cls[k];
+        while (b) {
+            wc[pos++] = (uint16_t)(tag | (lane * 64 + (u32)__builtin_ctzll(b)));
+            b &= b - 1;
+        }
+        at += (u32)((tot >> (12 * k)) & 0xfffu);
+    }
+    u32 pos = at + (u32)((excl >> 48) & 0xfffu);
+    u64 b = known;
+    while (b) {
+        wc[pos++] = (uint16_t)(kKnown | tag | (lane * 64 + (u32)__builtin_ctzll(b)));
+        b &= b - 1;
+    }
+    at += (u32)((tot >> 48) & 0xfffu);
+    return at - base;
+}
+
 /* Waves per SIMD each shape is compiled for (96 VGPRs at 5, 128 at 4). Eight-wave
    workgroups (54 KB of LDS each) fit two per CU, four waves per SIMD. */
 template <int W> struct BatchShape {
@@ -108,7 +159,8 @@ __attribute__((amdgpu_waves_per_eu(BatchShape<W>::kWavesPerEU))) void k_batch(co
                                                              BatchState *st, SlotSum *sums,
                                                              ww_result *res, u32 res_cap,
                                                              u32 claim, const DeepPrime *deep,
-                                                             u32 ndeep, const u64 *pre)
+                                                             u32 ndeep, const u64 *pre,
+                                                             const u64 *prer)
 {
     __shared__ __align__(16) uint16_t cand[W][kBatchCandCap];
     __shared__ u64 deep_bm[W][64];
@@ -156,7 +208,13 @@ __attribute__((amdgpu_waves_per_eu(BatchShape<W>::kWavesPerEU))) void k_batch(co
             const u32 e = wc[in ? j : j0];
             const bool newest = (e >> 12) & 1;
             const u64 n = (newest ? g_new : g_old) + 2ull * (e & 0xfffu);
-            const bool prp = spsp2_wave(n, in);
+            /* known primes pass; the rest take the test (skipped when none is in the round) */
+            const bool test = in && !(e & kKnown);
+            bool prp = in;
+            if (__ballot(test)) {
+                const bool r = spsp2_wave(n, test); /* all lanes: it takes wave-wide bounds */
+                prp = test ? r : prp;
+            }
             const u64 bal = __ballot(prp);
             if (prp) {
                 const u32 at = qn + (u32)__popcll(bal & ((1ull << lane) - 1));
@@ -201,7 +259,7 @@ __attribute__((amdgpu_waves_per_eu(BatchShape<W>::kWavesPerEU))) void k_batch(co
            the front of the buffer (rem <= j0 unless j0 = 0, so the moves never overlap) */
         {
             u32 e = 0;
-            if (lane < rem) e = wc[j0 + lane] & 0xfffu;
+            if (lane < rem) e = wc[j0 + lane] & ~(1u << 12);
             wave_sync_lds();
             if (lane < rem) wc[lane] = (uint16_t)e;
             wave_sync_lds();
@@ -213,7 +271,12 @@ __attribute__((amdgpu_waves_per_eu(BatchShape<W>::kWavesPerEU))) void k_batch(co
 
         /* sieve, then compact candidate positions into LDS behind the leftovers */
         u64 bits = sieve_word(g_new, lane * (u32)kIntsPerWord, sp.hi);
-        if (pre) bits &= ~pre[(sp.pre_seg + (s - prefix[a])) * kPresieveWordsPerSeg + lane];
+        u64 need = ~0ull; /* candidates the strong PRP test must decide */
+        if (pre) {
+            const u64 wi = (sp.pre_seg + (s - prefix[a])) * kPresieveWordsPerSeg + lane;
+            bits &= ~pre[wi];
+            if (prer) need = prer[wi];
+        }
         if (ndeep) {
             /* the end of the candidate buffer is free until the compaction below: it holds
                the deep sieve's pseudoprime classes meanwhile (the leftovers sit at the
@@ -223,14 +286,17 @@ __attribute__((amdgpu_waves_per_eu(BatchShape<W>::kWavesPerEU))) void k_batch(co
             bm[lane] = 0;
             rm[lane] = 0;
             wave_sync_lds();
-            bits &= ~deep_sieve(bm, rm, g_new, deep, ndeep, lane);
+            u64 classes;
+            bits &= ~deep_sieve(bm, rm, g_new, deep, ndeep, lane, &classes);
+            need |= classes;
         }
-        /* compacted behind the leftovers, grouped by the trailing zeros of n - 1, in the
-           opposite order to the previous segment's, whose last group the leftovers are */
+        /* compacted behind the leftovers: the candidates to test grouped by the trailing
+           zeros of n - 1, in the opposite order to the previous segment's, whose last group
+           the leftovers are; then the known primes */
         kb = rem;
         j0 = 0;
         desc = !desc;
-        ncand = rem + compact_by_class(bits, g_new, lane, wc, rem, 1u << 12, desc);
+        ncand = rem + compact_split(bits, need, g_new, lane, wc, rem, 1u << 12, desc);
         wave_sync_lds();
     }
     acc_flush(acc, sums, lane);
@@ -298,7 +364,7 @@ int batch_blocks_per_cu(int waves)
 hipError_t launch_batch(const BatchSpan *spans, const u64 *prefix, u32 nspans, BatchState *st,
                         SlotSum *sums, u32 nslots, ww_result *res, u32 res_cap, int grid,
                         u32 claim, const DeepPrime *deep, u32 ndeep, const u64 *pre,
-                        hipStream_t s, int waves)
+                        hipStream_t s, int waves, const u64 *prer)
 {
     if (!batch_waves_ok(waves)) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_batch_clear, dim3(nslots > 256 ? (nslots + 255) / 256 : 1), dim3(256), 0, s,
@@ -308,13 +374,13 @@ hipError_t launch_batch(const BatchSpan *spans, const u64 *prefix, u32 nspans, B
     const u32 c = claim < 1 ? 1u : claim;
     if (waves == 2)
         hipLaunchKernelGGL(k_batch<2>, dim3(grid), dim3(128), 0, s, spans, prefix, nspans, st, sums,
-                           res, res_cap, c, deep, ndeep, pre);
+                           res, res_cap, c, deep, ndeep, pre, pre ? prer : nullptr);
     else if (waves == 8)
         hipLaunchKernelGGL(k_batch<8>, dim3(grid), dim3(512), 0, s, spans, prefix, nspans, st, sums,
-                           res, res_cap, c, deep, ndeep, pre);
+                           res, res_cap, c, deep, ndeep, pre, pre ? prer : nullptr);
     else
         hipLaunchKernelGGL(k_batch<4>, dim3(grid), dim3(256), 0, s, spans, prefix, nspans, st, sums,
-                           res, res_cap, c, deep, ndeep, pre);
+                           res, res_cap, c, deep, ndeep, pre, pre ? prer : nullptr);
     return hipGetLastError();
 }
 

@@ -1,0 +1,185 @@
+/*
+ * ww_bucket.h -- arithmetic of the bucket sieve: the sieving primes above the wide
+ * pre-sieve, up to the square root of a range's end.
+ *
+ * Compiles for the device (the fill kernel, ww_bucket.hip) and for the host (the
+ * verification entry point ww_cpu_bucket_hits), so the CPU tests pin the very code the
+ * kernel runs.
+ *
+ * For a sieving prime l and the odd numbers G0 + 2b of a range (G0 odd), the multiples
+ * n = l*k are visited only for k prime to 210 = 2*3*5*7: n is odd and prime to 3, 5 and 7
+ * exactly when k is, and any other n is no candidate (every candidate is odd and prime to
+ * 3..113). That skips 57/105 of the odd multiples. Each visited n also gets its base-2
+ * pseudoprime test modulo l: x = 2^(n-1) mod l. A composite n = l*k that is a base-2
+ * Fermat pseudoprime (every strong one is) has x = 1, i.e. ord_l(2) | n - 1; such an n is
+ * not dropped but flagged for the strong PRP test. By Fermat, 2^(n-1) = 2^(lk-1) =
+ * (2^(l-1))^k 2^(k-1) = 2^(k-1) (mod l), so x starts as 2^(k0-1) mod l and each wheel
+ * step k -> k + g multiplies it by 2^g.
+ */
+#ifndef WW_BUCKET_H
+#define WW_BUCKET_H
+
+#include "ww_arith.h"
+
+namespace ww {
+
+/* Cofactor wheel modulo 210: its 48 residues prime to 210, ascending. */
+constexpr u32 kWheelMod = 210;
+constexpr u32 kWheelRes = 48;
+
+struct WheelTables {
+    uint16_t res[kWheelRes];  /* the residues */
+    uint8_t gap[kWheelRes];   /* res[i + 1] - res[i], and 210 + res[0] - res[47] */
+    uint8_t next[kWheelMod];  /* index of the first residue >= r, 48 when none */
+};
+
+constexpr u32 wheel_gcd(u32 a, u32 b) { return b ? wheel_gcd(b, a % b) : a; }
+
+constexpr WheelTables make_wheel()
+{
+    WheelTables t{};
+    u32 n = 0;
+    for (u32 r = 0; r < kWheelMod; ++r)
+        if (wheel_gcd(r, kWheelMod) == 1) t.res[n++] = (uint16_t)r;
+    for (u32 i = 0; i < kWheelRes; ++i) {
+        const u32 nx = i + 1 < kWheelRes ? t.res[i + 1] : kWheelMod + t.res[0];
+        t.gap[i] = (uint8_t)(nx - t.res[i]);
+    }
+    for (u32 r = 0; r < kWheelMod; ++r) {
+        u32 i = 0;
+        while (i < kWheelRes && t.res[i] < r) ++i;
+        t.next[r] = (uint8_t)i;
+    }
+    return t;
+}
+
+/* Largest multiple of the wheel's gaps: 2^gap < 2^11 keeps x * 2^gap below 2^43. */
+constexpr u32 kWheelMaxGap = 10;
+
+/* x = q d + r for x < 2^64 and 2^20 <= d < 2^32, with dinv = 1/d rounded to double. The
+   double quotient is within 3 * 2^-53 of x/d relatively, and x/d < 2^44, so it is off by
+   less than 2^-7: truncated it is the true quotient or one away from it, and the signed
+   remainder x - q d (exact mod 2^64, its true value lies in (-d, 2d)) decides the one
+   correction. */
+WW_HD void divmod_u64_u32(u64 x, u32 d, double dinv, u64 *q, u32 *r)
+{
+    u64 qe = (u64)((double)x * dinv);
+    int64_t rs = (int64_t)(x - qe * (u64)d);
+    if (rs < 0) {
+        rs += d;
+        --qe;
+    } else if (rs >= (int64_t)d) {
+        rs -= d;
+        ++qe;
+    }
+    *q = qe;
+    *r = (u32)rs;
+}
+
+/* x 2^v mod l for x < l < 2^32 and v <= 32 (linv = 1/l rounded): the quotient is below
+   2^v, and the double quotient of the exact double x 2^v is within 2^-51 of it relatively,
+   so off by less than 2^-19; truncated it is the true quotient or one away, and one signed
+   correction either way gives the residue. */
+WW_HD u32 mulpow2_u32(u32 x, u32 v, u32 l, double linv)
+{
+    const u64 t = (u64)x << v;
+    const u64 q = (u64)(ldexp((double)x, (int)v) * linv);
+    int64_t r = (int64_t)(t - q * (u64)l);
+    if (r < 0) r += l;
+    else if (r >= (int64_t)l) r -= l;
+    return (u32)r;
+}
+
+/* -l^-1 mod 2^32 for odd l: l * l = 1 (mod 8), and each Newton step y(2 - l y) doubles
+   the correct low bits (3, 6, 12, 24, 48). */
+WW_HD u32 neg_inv_u32(u32 l)
+{
+    u32 y = l;
+    y *= 2u - l * y;
+    y *= 2u - l * y;
+    y *= 2u - l * y;
+    y *= 2u - l * y;
+    return 0u - y;
+}
+
+/* Montgomery product a b 2^-32 mod l for a, b < l < 2^32 odd, linv = -l^-1 mod 2^32.
+   t + m l = 0 (mod 2^32), so (t + m l) / 2^32 = hi(t) + hi(m l) + carry, and the low words
+   add up to 0 or 2^32: the carry is 1 exactly when lo(t) != 0. The sum is below
+   l^2 / 2^32 + l < 2l, so one subtraction completes it. */
+WW_HD u32 mont_mul_u32(u32 a, u32 b, u32 l, u32 linv)
+{
+    const u64 t = (u64)a * b;
+    const u32 m = (u32)t * linv;
+    const u64 u = (t >> 32) + (((u64)m * l) >> 32) + ((u32)t != 0u ? 1u : 0u);
+    return u >= l ? (u32)(u - l) : (u32)u;
+}
+
+/* 2^e mod l for e < 2^32 and 2^20 <= l < 2^32 prime: Montgomery squarings over 4-bit
+   windows of e from the top, each window's 2^w entered by one exact multiplication. */
+WW_HD u32 pow2_mod_u32(u32 e, u32 l, double linv)
+{
+    const u32 li = neg_inv_u32(l);
+    const u32 r1 = mulpow2_u32(1u, 32u, l, linv); /* 2^32 mod l: Montgomery 1 */
+    u32 x = mulpow2_u32(r1, e >> 28, l, linv);
+    for (int s = 24; s >= 0; s -= 4) {
+        x = mont_mul_u32(x, x, l, li);
+        x = mont_mul_u32(x, x, l, li);
+        x = mont_mul_u32(x, x, l, li);
+        x = mont_mul_u32(x, x, l, li);
+        x = mulpow2_u32(x, (e >> s) & 15u, l, linv);
+    }
+    return mont_mul_u32(x, 1u, l, li);
+}
+
+/* Where the bucket sieve starts for the sieving prime l (2^20 < l < 2^32) over the odd
+   numbers G0 + 2b (G0 odd): the first multiple n = l k >= G0 with k prime to 210, as its
+   bit b (the true difference n - G0 is below 11 l, so mod-2^64 arithmetic gives it even
+   where n passes 2^64), the wheel index of k, and x = 2^(n-1) mod l. */
+struct BucketStart {
+    u64 b;
+    u32 idx;
+    u32 x;
+};
+
+WW_HD BucketStart bucket_start(u64 G0, u32 l, double linv, const uint16_t *res,
+                               const uint8_t *next)
+{
+    u64 q;
+    u32 r;
+    divmod_u64_u32(G0, l, linv, &q, &r);
+    const u64 kmin = q + (r != 0u ? 1u : 0u); /* ceil(G0 / l), below 2^44 + 1 */
+    u64 kq = kmin / kWheelMod;
+    u32 idx = next[(u32)(kmin - kq * kWheelMod)];
+    if (idx == kWheelRes) {
+        idx = 0;
+        ++kq;
+    }
+    const u64 k = kq * kWheelMod + res[idx];
+    BucketStart s;
+    s.b = ((u64)l * k - G0) >> 1;
+    s.idx = idx;
+    /* x = 2^(k-1) mod l, the exponent reduced mod l - 1 */
+    u64 q2;
+    u32 e;
+    divmod_u64_u32(k - 1, l - 1, 1.0 / (double)(l - 1), &q2, &e);
+    s.x = pow2_mod_u32(e, l, linv);
+    return s;
+}
+
+/* The next visited multiple: k -> k + gap, b -> b + gap l / 2, x -> x 2^gap. */
+WW_HD void bucket_step(u64 &b, u32 &idx, u32 &x, u32 l, double linv, const uint8_t *gap)
+{
+    const u32 g = gap[idx];
+    b += (u64)(g >> 1) * l;
+    x = mulpow2_u32(x, g, l, linv);
+    idx = idx + 1 == kWheelRes ? 0u : idx + 1;
+}
+
+/* Bucket entries: the odd-number index of the hit in its 2^20-bit chunk, and the top bit
+   set when the hit is in the prime's pseudoprime class (x = 1). */
+constexpr u32 kBucketChunkBits = 20;
+constexpr u32 kBucketRescue = 0x80000000u;
+
+} // namespace ww
+
+#endif

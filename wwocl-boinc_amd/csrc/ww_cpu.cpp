@@ -5,6 +5,7 @@
  *                                   with the bounds a wave would share as arguments
  *   ww_deep_table, ww_presieve_table  the deep-sieve and pre-sieve primes with their
  *                                   base-2 pseudoprime classes
+ *   ww_cpu_bucket_hits              ww_bucket.h's walk of one bucket-sieve prime
  * The drop-in's validation and self-test (isPrime, isPRP, the CPU range loop) are in
  * ww_selftest.cpp, in a formulation independent of this arithmetic.
  */
@@ -12,6 +13,7 @@
 
 #include "../../include/ww_api.h"
 #include "ww_arith.h"
+#include "ww_bucket.h"
 
 using namespace ww;
 
@@ -140,6 +142,27 @@ uint32_t ww_presieve_table(uint32_t above, uint32_t bound, ww_wide_prime *out, u
         d.bar_m = ~0ull / d.m; /* floor(2^64 / m), m no power of 2 */
     }
     return n;
+}
+
+int ww_cpu_bucket_hits(uint64_t g0, uint64_t nbits, uint32_t ell, uint64_t *bits,
+                       uint8_t *rescue, uint64_t cap, uint64_t *count)
+{
+    if (!count || (cap && (!bits || !rescue))) return ww_set_error(WW_E_ARG, "null buffer");
+    if ((g0 & 1) == 0 || ell <= (1u << 20) || (ell & 1) == 0)
+        return ww_set_error(WW_E_ARG, "g0 must be odd and ell an odd prime above 2^20");
+    static constexpr WheelTables wt = make_wheel();
+    const double linv = 1.0 / (double)ell;
+    BucketStart s = bucket_start(g0, ell, linv, wt.res, wt.next);
+    uint64_t n = 0;
+    for (u64 b = s.b; b < nbits; ++n) {
+        if (n < cap) {
+            bits[n] = b;
+            rescue[n] = s.x == 1u;
+        }
+        bucket_step(b, s.idx, s.x, ell, linv, wt.gap);
+    }
+    *count = n;
+    return WW_OK;
 }
 
 } // extern "C"

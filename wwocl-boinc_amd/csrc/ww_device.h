@@ -159,9 +159,11 @@ __device__ __forceinline__ bool spsp2_wave(u64 n, bool act)
    prime's base-2 pseudoprime class (which holds every strong pseudoprime that the prime
    divides, so no list member is dropped). Lanes share the primes round-robin; multiples
    go to bm, pseudoprime classes to rm (both 64 words of LDS, zeroed by the caller), so
-   each prime's offsets are computed once. Called by all 64 lanes. */
+   each prime's offsets are computed once. With `classes`, also returns the lane's word of
+   pseudoprime-class members (the candidates this sieve leaves to the strong PRP test).
+   Called by all 64 lanes. */
 __device__ __forceinline__ u64 deep_sieve(u64 *bm, u64 *rm, u64 group, const DeepPrime *deep,
-                                          u32 ndeep, u32 lane)
+                                          u32 ndeep, u32 lane, u64 *classes = nullptr)
 {
     for (u32 j = lane; j < ndeep; j += 64) {
         const DeepPrime d = deep[j];
@@ -175,7 +177,9 @@ __device__ __forceinline__ u64 deep_sieve(u64 *bm, u64 *rm, u64 group, const Dee
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    const u64 drop = bm[lane] & ~rm[lane];
+    const u64 rw = rm[lane];
+    const u64 drop = bm[lane] & ~rw;
+    if (classes) *classes = rw;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");

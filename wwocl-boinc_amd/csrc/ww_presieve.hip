@@ -18,9 +18,15 @@
  * hundred per chunk; the table starts above 1024 for that); if the list overflows, the
  * chunk drops nothing (exact, only slower). Pieces shorter than min(16, chunk_segs)
  * segments are not pre-sieved.
+ *
+ * With the bucket sieve (ww_bucket.hip) the chunk's buckets are applied after those
+ * conservative clears, as exact marks, and a second set of words is written: the odd
+ * numbers that still need the strong PRP test, i.e. the pseudoprime-class members of every
+ * tier, or every odd number of a piece that is not sieved up to sqrt(hi).
  */
 #include "ww_arith.h"
 #include "ww_batch.h"
+#include "ww_bucket.h"
 
 namespace ww {
 
@@ -40,14 +46,70 @@ __device__ __forceinline__ u64 barrett64(u64 x, u64 m, u64 bar)
     return r >= m ? r - m : r;
 }
 
+/* Largest a with prefix[a] <= s (the span holding global segment s). */
+__device__ __forceinline__ u32 span_of(const u64 *prefix, u32 nspans, u64 s)
+{
+    u32 a = 0, b = nspans;
+    while (b - a > 1) {
+        const u32 mid = (a + b) >> 1;
+        if (prefix[mid] <= s) a = mid;
+        else b = mid;
+    }
+    return a;
+}
+
+/* Applies the buckets of one chunk row of a bucket span to the LDS bitmap: marks the hits
+   and appends the pseudoprime-class hits to the rescue list; sets *over when a bucket
+   overflowed (entries were lost). Called by every thread; each wave takes every 16th
+   bucket and reads it with 16 loads in flight per lane. */
+__device__ __forceinline__ void apply_buckets(const BucketArgs &bk, const BucketSpan &bs, u32 j,
+                                              u32 *bm, u32 *rescue, u32 *rcount, u32 *over)
+{
+    constexpr int kUnroll = 16;
+    const u32 lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (u32 w = wave; w < kBucketGroups; w += kPreThreads / 64) {
+        u32 n = __builtin_amdgcn_readfirstlane(bk.counts[bs.cnt_base + (u64)j * kBucketGroups + w]);
+        if (n > bk.cap) {
+            if (lane == 0) *over = 1;
+            n = bk.cap;
+        }
+        const u32 *run = bk.entries + bs.ent_base + ((u64)j * kBucketGroups + w) * bk.cap;
+        for (u32 i0 = 0; i0 < n; i0 += 64 * kUnroll) {
+            u32 e[kUnroll];
+#pragma unroll
+            for (int u = 0; u < kUnroll; ++u) {
+                const u32 i = i0 + 64 * u + lane;
+                e[u] = i < n ? run[i] : 0xffffffffu;
+            }
+#pragma unroll
+            for (int u = 0; u < kUnroll; ++u) {
+                if (e[u] == 0xffffffffu) continue;
+                const u32 at = e[u] & ((1u << kBucketChunkBits) - 1);
+                if (e[u] & kBucketRescue) {
+                    const u32 k = atomicAdd(rcount, 1u);
+                    if (k < kRescueCap) rescue[k] = at;
+                } else {
+                    atomicOr(&bm[at >> 5], 1u << (at & 31));
+                }
+            }
+        }
+    }
+}
+
 __global__ __launch_bounds__(kPreThreads) void k_presieve(const BatchSpan *spans,
                                                           const u64 *prefix, u32 nspans,
                                                           const WidePrime *tab, u32 ntab,
-                                                          u64 *out, u32 chunk_segs)
+                                                          u64 *out, u32 chunk_segs, u64 *outr,
+                                                          BucketArgs bk)
 {
     __shared__ __align__(8) u32 bm[kChunkWords]; /* read back as u64 words below */
     __shared__ u32 rescue[kRescueCap];
     __shared__ u32 rcount;
+    __shared__ u32 lost; /* the rescue list overflowed with the buckets' class hits */
+    __shared__ u32 over; /* a bucket of the current piece overflowed */
+    /* segments of the chunk whose candidates all need the strong PRP test (not sieved up
+       to sqrt(hi)); only kept when outr is given */
+    __shared__ uint8_t all_prp[kChunkSegs];
 
     const u64 total = prefix[nspans];
     const u64 nchunks = (total + chunk_segs - 1) / chunk_segs;
@@ -57,17 +119,17 @@ __global__ __launch_bounds__(kPreThreads) void k_presieve(const BatchSpan *spans
         const u64 s0 = c * chunk_segs;
         const u64 s1 = s0 + chunk_segs < total ? s0 + chunk_segs : total;
         for (u32 w = threadIdx.x; w < chunk_words; w += kPreThreads) bm[w] = 0;
-        if (threadIdx.x == 0) rcount = 0;
+        if (threadIdx.x < kChunkSegs) all_prp[threadIdx.x] = 0;
+        if (threadIdx.x == 0) {
+            rcount = 0;
+            lost = 0;
+            over = 0;
+        }
         __syncthreads();
 
         /* the chunk's pieces, each inside one range (the same values in every thread) */
         for (u64 s = s0; s < s1;) {
-            u32 a = 0, b = nspans;
-            while (b - a > 1) {
-                const u32 mid = (a + b) >> 1;
-                if (prefix[mid] <= s) a = mid;
-                else b = mid;
-            }
+            const u32 a = span_of(prefix, nspans, s);
             const u64 e = prefix[a + 1] < s1 ? prefix[a + 1] : s1;
             const BatchSpan sp = spans[a];
             const u64 G = (sp.lo | 1) + (sp.seg0 + (s - prefix[a])) * kIntsPerWave;
@@ -75,7 +137,11 @@ __global__ __launch_bounds__(kPreThreads) void k_presieve(const BatchSpan *spans
             const u32 len = (u32)(e - s) * 4096;
             /* a short piece (a small range, or the end of one) is left to the strong PRP
                test: its words stay zero */
-            const u32 nprimes = e - s >= min_piece ? ntab : 0;
+            const bool sieved = e - s >= min_piece;
+            const u32 nprimes = sieved ? ntab : 0;
+            if (outr && (!sieved || !bk.spans || sp.pad == 0))
+                for (u32 k = threadIdx.x; k < (u32)(e - s); k += kPreThreads)
+                    all_prp[(u32)(s - s0) + k] = 1;
             for (u32 j = threadIdx.x; j < nprimes; j += kPreThreads) {
                 const WidePrime d = tab[j];
                 const u64 ell = d.ell;
@@ -104,10 +170,54 @@ __global__ __launch_bounds__(kPreThreads) void k_presieve(const BatchSpan *spans
             for (u32 k = threadIdx.x; k < nr; k += kPreThreads)
                 atomicAnd(&bm[rescue[k] >> 5], ~(1u << (rescue[k] & 31)));
         __syncthreads();
+        /* the buckets of the chunk's bucket-span pieces: exact marks (after the conservative
+           clears above), their pseudoprime-class hits joining the rescue list */
+        if (outr && bk.spans && nr <= kRescueCap) {
+            for (u64 s = s0; s < s1;) {
+                const u32 a = span_of(prefix, nspans, s);
+                const u64 e = prefix[a + 1] < s1 ? prefix[a + 1] : s1;
+                const u32 pad = spans[a].pad;
+                if (pad && e - s >= min_piece) {
+                    const BucketSpan bs = bk.spans[pad - 1];
+                    const u32 j = (u32)(s0 >> (kBucketChunkBits - 12)) - bs.chunk_first;
+                    apply_buckets(bk, bs, j, bm, rescue, &rcount, &over);
+                    __syncthreads();
+                    if (over) { /* the same in every thread after the barrier */
+                        for (u32 k = threadIdx.x; k < (u32)(e - s); k += kPreThreads)
+                            all_prp[(u32)(s - s0) + k] = 1;
+                        __syncthreads();
+                        if (threadIdx.x == 0) {
+                            atomicAdd(bk.overflows, 1u);
+                            over = 0;
+                        }
+                    }
+                    __syncthreads();
+                }
+                s = e;
+            }
+            __syncthreads();
+            if (threadIdx.x == 0 && rcount > kRescueCap) lost = 1;
+        }
+        __syncthreads();
         const u64 *bm64 = reinterpret_cast<const u64 *>(bm);
         const u32 nw = (u32)(s1 - s0) * 64;
         for (u32 w = threadIdx.x; w < nw; w += kPreThreads)
             out[s0 * 64 + w] = nr <= kRescueCap ? bm64[w] : 0;
+        if (outr) {
+            /* the words of candidates that still need the strong PRP test: the rescue list
+               (pseudoprime classes of every tier), or everything where the chunk is not
+               fully sieved */
+            const bool all = nr > kRescueCap || lost;
+            const u32 nrr = rcount < kRescueCap ? rcount : kRescueCap;
+            __syncthreads();
+            for (u32 w = threadIdx.x; w < chunk_words; w += kPreThreads) bm[w] = 0;
+            __syncthreads();
+            for (u32 k = threadIdx.x; k < nrr; k += kPreThreads)
+                atomicOr(&bm[rescue[k] >> 5], 1u << (rescue[k] & 31));
+            __syncthreads();
+            for (u32 w = threadIdx.x; w < nw; w += kPreThreads)
+                outr[s0 * 64 + w] = (all || all_prp[w >> 6]) ? ~0ull : bm64[w];
+        }
         __syncthreads();
     }
 }
@@ -151,11 +261,17 @@ bool presieve_chunk_ok(u32 chunk_segs)
 
 hipError_t launch_presieve(const BatchSpan *spans, const u64 *prefix, u32 nspans,
                            const WidePrime *tab, u32 ntab, u64 *out, int grid, hipStream_t s,
-                           u32 chunk_segs)
+                           u32 chunk_segs, u64 *outr, const BucketArgs *bk)
 {
     if (!presieve_chunk_ok(chunk_segs)) return hipErrorInvalidValue;
+    BucketArgs b{};
+    if (bk && bk->spans) {
+        if (!outr || chunk_segs != (1u << (kBucketChunkBits - 12)) || !bk->overflows)
+            return hipErrorInvalidValue;
+        b = *bk;
+    }
     hipLaunchKernelGGL(k_presieve, dim3(grid), dim3(kPreThreads), 0, s, spans, prefix, nspans,
-                       tab, ntab, out, chunk_segs);
+                       tab, ntab, out, chunk_segs, outr, b);
     return hipGetLastError();
 }
 

@@ -104,6 +104,11 @@ class ServerStats(ctypes.Structure):
                 ("peak_active", U64)]
 
 
+class BucketInfo(ctypes.Structure):
+    _fields_ = [("spans", U32), ("active", U32), ("bucket_cap", U32), ("overflows", U32),
+                ("bytes", U64)]
+
+
 class DeviceInfo(ctypes.Structure):
     _fields_ = [("name", ctypes.c_char * 256), ("arch", ctypes.c_char * 64),
                 ("compute_units", ctypes.c_int), ("clock_khz", ctypes.c_int),
@@ -188,6 +193,15 @@ _SIGNATURES = [
                                                 ctypes.c_void_p, ctypes.c_void_p, U32,
                                                 ctypes.c_void_p]),
     ("ww_cpu_range", ctypes.c_int, [U64, U64, ctypes.POINTER(U64)]),
+    ("ww_set_bucket_sieve", ctypes.c_int, [ctypes.c_void_p, U64]),
+    ("ww_get_bucket_sieve", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(U64)]),
+    ("ww_prime_table", ctypes.c_int, [ctypes.c_void_p, U64, U64, ctypes.POINTER(U32),
+                                      ctypes.POINTER(U64)]),
+    ("ww_cpu_bucket_hits", ctypes.c_int, [U64, U64, U32, ctypes.POINTER(U64),
+                                          ctypes.POINTER(ctypes.c_uint8), U64,
+                                          ctypes.POINTER(U64)]),
+    ("ww_batch_buckets", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p,
+                                        ctypes.POINTER(BucketInfo)]),
     ("ww_deep_table", U32, [U32, ctypes.POINTER(DeepPrime), U32]),
     ("ww_presieve_table", U32, [U32, U32, ctypes.POINTER(WidePrime), U32]),
 ]
@@ -295,6 +309,18 @@ def cpu_spsp2(n: int, top: int = -1, tmax: int = -1) -> bool:
     return bool(r)
 
 
+def cpu_bucket_hits(g0: int, nbits: int, ell: int, cap: int = 1 << 16):
+    """Host build of the bucket sieve's walk of one prime: [(bit, in pseudoprime class)]
+    for the multiples ell*k of [g0, g0 + 2 nbits) with k prime to 210."""
+    bits = (U64 * max(cap, 1))()
+    resc = (ctypes.c_uint8 * max(cap, 1))()
+    n = U64()
+    _check(load().ww_cpu_bucket_hits(g0, nbits, ell, bits, resc, cap, ctypes.byref(n)))
+    if n.value > cap:
+        raise WWError(WW_E_CAPACITY, f"{n.value} hits, buffer {cap}")
+    return [(bits[i], bool(resc[i])) for i in range(n.value)]
+
+
 def cpu_range(low: int, high: int):
     out = (U64 * 3)()
     _check(load().ww_cpu_range(low, high, out))
@@ -341,6 +367,23 @@ class Context:
             self.close()
         except Exception:
             pass
+
+    def set_bucket_sieve(self, min_width: int):
+        """Ranges at least min_width wide get the bucket sieve in batches made afterwards
+        (2**64 - 1: none)."""
+        _check(self.lib.ww_set_bucket_sieve(self.h, min_width))
+
+    def bucket_sieve(self) -> int:
+        v = U64()
+        _check(self.lib.ww_get_bucket_sieve(self.h, ctypes.byref(v)))
+        return v.value
+
+    def prime_table(self, first: int, n: int):
+        """(entries [first, first + n) of the device's odd-prime table, its length)."""
+        out = (U32 * max(n, 1))()
+        total = U64()
+        _check(self.lib.ww_prime_table(self.h, first, n, out, ctypes.byref(total)))
+        return list(out[:n]), total.value
 
     def set_deep_bound(self, bound: int):
         _check(self.lib.ww_set_deep_bound(self.h, bound))
@@ -525,6 +568,14 @@ class Batch:
         _check(self.lib.ww_batch_kernel_times(self.ctx.h, self.h, ctypes.byref(pre),
                                               ctypes.byref(search)))
         return pre.value, search.value
+
+    def buckets(self) -> dict:
+        """The batch's bucket sieve: ranges covered, whether launches use it, bucket
+        capacity, device bytes and overflowed chunks so far."""
+        bi = BucketInfo()
+        _check(self.lib.ww_batch_buckets(self.ctx.h, self.h, ctypes.byref(bi)))
+        return {"spans": bi.spans, "active": bool(bi.active), "bucket_cap": bi.bucket_cap,
+                "overflows": bi.overflows, "bytes": bi.bytes}
 
     def read(self):
         """[(prp_count, checksum, sorted records)] per range, in table order."""
