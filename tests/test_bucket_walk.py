@@ -1,0 +1,79 @@
+"""The bucket sieve's walk of one sieving prime (csrc/ww_bucket.h), built for the host
+through ww_cpu_bucket_hits: for a prime l above 2^20 and the odd numbers g0 + 2b, it must
+visit exactly the multiples n = l*k with k prime to 210 (in order, as bit indices
+(n - g0) / 2), and flag exactly those with 2^(n-1) = 1 (mod l), the prime's base-2
+pseudoprime class. The expected lists come from Python's exact integers: every multiple of
+l in the window, a gcd with 210, and pow(2, n - 1, l). Primes of small multiplicative order
+(factors of 2^31 - 1, 2^37 - 1, 2^41 - 1, ...) make the class frequent, windows near 2^64
+make the multiples pass 2^64. No device needed."""
+import math
+import random
+
+import pytest
+
+import oracle
+import wwlib
+
+
+def brute(g0, nbits, ell):
+    out = []
+    k = -(-g0 // ell)
+    while ell * k < g0 + 2 * nbits:
+        if math.gcd(k, 210) == 1:
+            n = ell * k
+            out.append(((n - g0) // 2, pow(2, n - 1, ell) == 1))
+        k += 1
+    return out
+
+
+# primes above 2^20 with 2 of small order: 2^31 - 1 (31), 616318177 | 2^37 - 1 (37),
+# 164511353 | 2^41 - 1 (41), 2099863 | 2^43 - 1 (43), 13264529 | 2^47 - 1 (47),
+# 20394401 | 2^53 - 1 (53), 6700417 | 2^32 + 1 (64)
+SMALL_ORDER = [2147483647, 616318177, 164511353, 2099863, 13264529, 20394401, 6700417]
+
+
+def test_small_order_primes_flag_their_class():
+    for ell in SMALL_ORDER:
+        order = next(d for d in range(1, 65) if pow(2, d, ell) == 1)
+        assert order <= 64
+        for g0 in (1, (1 << 20) + 1, (1 << 63) + 1, (1 << 64) - (1 << 30) + 1, 10**15 + 1,
+                   ((1 << 64) - 700 * ell) | 1):
+            got = wwlib.cpu_bucket_hits(g0, 700 * ell, ell, cap=1 << 16)
+            assert got == brute(g0, 700 * ell, ell), (ell, g0)
+            assert any(r for _, r in got), (ell, g0)  # ~1 in 11 hits is in the class
+
+
+def test_random_primes_and_windows():
+    rng = random.Random(5)
+
+    def rprime(lo, hi):
+        while True:
+            x = rng.randrange(lo, hi) | 1
+            if oracle.is_prime(x):
+                return x
+
+    for _ in range(60):
+        ell = rprime((1 << 20) + 1, 1 << 32)
+        g0 = rng.choice([rng.randrange(1, 1 << 64) | 1,
+                         ((1 << 64) - rng.randrange(1, 1 << 34)) | 1,
+                         rng.randrange(1, 1 << 40) | 1])
+        nbits = rng.choice([1, 100, ell * 20, ell * 300])
+        assert wwlib.cpu_bucket_hits(g0, nbits, ell, cap=1 << 16) == brute(g0, nbits, ell), \
+            (g0, nbits, ell)
+
+
+def test_extreme_primes():
+    # the largest prime below 2^32 over the top of the domain, the smallest above 2^20
+    g0 = ((1 << 64) - (1 << 33)) | 1
+    assert wwlib.cpu_bucket_hits(g0, 1 << 32, 4294967291) == brute(g0, 1 << 32, 4294967291)
+    g0 = (1 << 63) + 1
+    got = wwlib.cpu_bucket_hits(g0, 1 << 32, 1048583, cap=1 << 16)
+    assert got == brute(g0, 1 << 32, 1048583)
+    assert len(got) > 1800  # 2^33 / l odd multiples, 48 of each 105 visited
+
+
+def test_arguments():
+    with pytest.raises(wwlib.WWError):
+        wwlib.cpu_bucket_hits(2, 100, 2147483647)  # even start
+    with pytest.raises(wwlib.WWError):
+        wwlib.cpu_bucket_hits(1, 100, 1 << 19)  # below the bucket tier

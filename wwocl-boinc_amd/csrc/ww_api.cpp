@@ -909,6 +909,9 @@ struct ww_batch {
        candidates that need the strong PRP test (64 per segment, like d_pre) */
     std::vector<ww::BucketSpan> bspans;
     ww::BucketSpan *d_bspans = nullptr;
+    uint32_t nbands = 0;
+    uint32_t *d_band_ent = nullptr, *d_band_cnt = nullptr;
+    uint32_t band_cap = 0;
     uint32_t *d_bent = nullptr, *d_bcnt = nullptr;
     uint32_t bcap = 0;
     uint32_t *d_bover = nullptr; /* chunks whose buckets overflowed, summed over launches */
@@ -976,9 +979,9 @@ int batch_enqueue_presieve(ww_ctx *c, ww_batch *b, bool buckets)
                                     b->d_pre, pgrid > 0 ? pgrid : 1, c->stream, c->presieve_chunk));
         return WW_OK;
     }
-    HIP_TRY(ww::launch_bucket_fill(b->d_spans, b->d_prefix, b->d_bspans, (uint32_t)b->bspans.size(),
-                                   c->d_primes32, b->d_bent, b->d_bcnt, b->bcap, ww::kBucketGroups,
-                                   c->stream));
+    const ww::BucketStore st{b->d_band_ent, b->d_band_cnt, b->band_cap, b->d_bent, b->d_bcnt, b->bcap};
+    HIP_TRY(ww::launch_bucket_sieve(b->d_spans, b->d_prefix, b->d_bspans, (uint32_t)b->bspans.size(),
+                                    b->nbands, c->d_primes32, st, c->stream));
     ww::BucketArgs bk{b->d_bspans, b->d_bent, b->d_bcnt, b->bcap, b->d_bover};
     HIP_TRY(ww::launch_presieve(b->d_spans, b->d_prefix, b->n, c->d_wide + first, count, b->d_pre,
                                 pgrid > 0 ? pgrid : 1, c->stream, c->presieve_chunk, b->d_prer, &bk));
@@ -1064,13 +1067,14 @@ int ww_batch_create(ww_ctx *c, const ww_range *ranges, uint32_t n, uint32_t resu
     if (e == hipSuccess && c->presieve_bound && b->nseg)
         e = hipMalloc(&b->d_pre, b->nseg * ww::kPresieveWordsPerSeg * sizeof(uint64_t));
     /* bucket spans: ranges at least bucket_min_width wide (and at most kBucketMaxChunks
-       chunks), with rows of buckets sized for primes from 2^20 to sqrt(hi - 1): per
-       2^20-bit chunk, 2^20 x 48/105 (the cofactor wheel) x sum(1/l) hits, sum(1/l) from
-       Mertens' ln ln top - ln ln 2^20 (within 0.005 of the true sum), spread over
-       kBucketGroups buckets; capacity 1.25x the mean plus 96, and an overflow only costs
-       that chunk its known primes */
+       chunks). Buckets are sized for primes from 2^20 to sqrt(hi - 1): per 2^20-bit chunk,
+       2^20 x 48/105 (the cofactor wheel) x sum(1/l) hits, sum(1/l) from Mertens'
+       ln ln top - ln ln 2^20 (within 0.005 of the true sum). A chunk part's bucket (one of
+       kBucketSplitParts) holds 1.15x its share of that mean plus 512; a band bucket (16
+       chunks, one per fill workgroup) 1.25x its mean plus 256. An overflow only costs the
+       chunks concerned their known primes. */
     if (e == hipSuccess && c->presieve_bound && b->nseg) {
-        uint64_t rows = 0;
+        uint64_t chunks = 0, bands = 0;
         double mean = 0;
         for (uint32_t i = 0; i < n; ++i) {
             const uint64_t nseg = prefix[i + 1] - prefix[i];
@@ -1084,23 +1088,34 @@ int ww_batch_create(ww_ctx *c, const ww_range *ranges, uint32_t n, uint32_t resu
             bs.bit0 = bit0;
             bs.chunk_first = (uint32_t)cf;
             bs.nchunks = (uint32_t)nch;
+            bs.nbands = (uint32_t)((nch + ww::kBucketBandChunks - 1) / ww::kBucketBandChunks);
             bs.top = (uint32_t)isqrt64(ranges[i].hi - 1);
-            bs.cnt_base = rows * ww::kBucketGroups;
-            rows += nch;
+            bs.cnt_base = chunks * ww::kBucketSplitParts;
+            bs.band_cnt_base = bands * ww::kBucketGroups;
+            chunks += nch;
+            bands += bs.nbands;
             const double lt = std::log((double)bs.top), l20 = 20.0 * std::log(2.0);
-            if (lt > l20) mean = std::max(mean, std::ldexp(1.0, 20) * 48.0 / 105.0 *
-                                                    std::log(lt / l20) / ww::kBucketGroups);
+            if (lt > l20) mean = std::max(mean, std::ldexp(1.0, 20) * 48.0 / 105.0 * std::log(lt / l20));
             b->bspans.push_back(bs);
             spans[i].pad = (uint32_t)b->bspans.size();
         }
         if (!b->bspans.empty()) {
-            b->bcap = (uint32_t)std::ceil(1.25 * mean + 96);
-            for (ww::BucketSpan &bs : b->bspans) bs.ent_base = bs.cnt_base * b->bcap;
+            b->bcap = (uint32_t)std::ceil(1.15 * mean / ww::kBucketSplitParts + 512);
+            b->band_cap = (uint32_t)std::ceil(1.25 * mean * ww::kBucketBandChunks / ww::kBucketGroups + 256);
+            b->nbands = (uint32_t)bands;
+            for (ww::BucketSpan &bs : b->bspans) {
+                bs.ent_base = bs.cnt_base * b->bcap;
+                bs.band_ent_base = bs.band_cnt_base * b->band_cap;
+            }
             const size_t nb = b->bspans.size();
+            const size_t band_buckets = bands * ww::kBucketGroups;
             e = hipMalloc(&b->d_bspans, nb * sizeof(ww::BucketSpan));
-            if (e == hipSuccess) e = hipMalloc(&b->d_bcnt, rows * ww::kBucketGroups * sizeof(uint32_t));
+            if (e == hipSuccess) e = hipMalloc(&b->d_band_cnt, band_buckets * sizeof(uint32_t));
             if (e == hipSuccess)
-                e = hipMalloc(&b->d_bent, rows * ww::kBucketGroups * (size_t)b->bcap * sizeof(uint32_t));
+                e = hipMalloc(&b->d_band_ent, band_buckets * (size_t)b->band_cap * sizeof(uint32_t));
+            const size_t parts = chunks * ww::kBucketSplitParts;
+            if (e == hipSuccess) e = hipMalloc(&b->d_bcnt, parts * sizeof(uint32_t));
+            if (e == hipSuccess) e = hipMalloc(&b->d_bent, parts * (size_t)b->bcap * sizeof(uint32_t));
             if (e == hipSuccess) e = hipMalloc(&b->d_bover, sizeof(uint32_t));
             if (e == hipSuccess) e = hipMemsetAsync(b->d_bover, 0, sizeof(uint32_t), c->stream);
             if (e == hipSuccess)
@@ -1260,9 +1275,11 @@ int ww_batch_buckets(ww_ctx *c, ww_batch *b, ww_bucket_info *out)
     out->spans = (uint32_t)b->bspans.size();
     out->active = batch_buckets(c, b) ? 1u : 0u;
     out->bucket_cap = b->bcap;
-    uint64_t rows = 0;
-    for (const ww::BucketSpan &bs : b->bspans) rows += bs.nchunks;
-    out->bytes = rows * ww::kBucketGroups * ((uint64_t)b->bcap + 1) * sizeof(uint32_t);
+    uint64_t chunks = 0;
+    for (const ww::BucketSpan &bs : b->bspans) chunks += bs.nchunks;
+    out->bytes = (chunks * ww::kBucketSplitParts * ((uint64_t)b->bcap + 1) +
+                  (uint64_t)b->nbands * ww::kBucketGroups * ((uint64_t)b->band_cap + 1)) *
+                 sizeof(uint32_t);
     if (b->d_bover) {
         HIP_TRY(hipMemcpyAsync(&out->overflows, b->d_bover, sizeof(uint32_t), hipMemcpyDeviceToHost,
                                c->stream));
@@ -1287,6 +1304,8 @@ int ww_batch_destroy(ww_batch *b)
     if (b->d_bspans) (void)hipFree(b->d_bspans);
     if (b->d_bent) (void)hipFree(b->d_bent);
     if (b->d_bcnt) (void)hipFree(b->d_bcnt);
+    if (b->d_band_ent) (void)hipFree(b->d_band_ent);
+    if (b->d_band_cnt) (void)hipFree(b->d_band_cnt);
     if (b->d_bover) (void)hipFree(b->d_bover);
     if (b->d_res) (void)hipFree(b->d_res);
     if (b->d_state) (void)hipFree(b->d_state);

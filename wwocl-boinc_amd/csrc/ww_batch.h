@@ -47,22 +47,36 @@ struct WidePrime {
 
 /* One range of a batch that the bucket sieve covers (sieving primes above the wide
    pre-sieve, up to sqrt(hi - 1)); its BatchSpan's pad holds 1 + its index in this table.
-   Its buckets are rows of `groups` (the fill's workgroups) per 2^20-bit chunk: row j,
-   workgroup w holds up to cap entries from entries[ent_base + (j groups + w) cap] on, and
-   its count at counts[cnt_base + j groups + w]. */
+   The fill writes each hit to a band (kBucketBandChunks consecutive 2^20-bit chunks) of
+   its workgroup: workgroup w, band b holds up to band_cap entries from
+   band_entries[band_ent_base + (w nbands + b) band_cap] on (a workgroup's buckets are
+   contiguous), its count at band_counts[band_cnt_base + w nbands + b]. The split then
+   gathers each chunk's entries in kBucketSplitParts parts (one per split workgroup of the
+   band): chunk j, part p holds up to cap entries from entries[ent_base + (j parts + p) cap]
+   on, its count at counts[cnt_base + j parts + p]. */
 struct BucketSpan {
-    uint64_t ent_base, cnt_base;
+    uint64_t ent_base, cnt_base;           /* per (chunk, part) */
+    uint64_t band_ent_base, band_cnt_base; /* per (fill workgroup, band) */
     uint64_t pidx0, pidx1; /* sieving primes: table entries [pidx0, pidx1) up to top */
     uint64_t bit0;         /* global odd-number index of the span's first segment */
     uint32_t span;         /* index of the span in the span table */
     uint32_t chunk_first;  /* bit0 >> 20 */
     uint32_t nchunks;      /* chunks holding the span's segments */
     uint32_t top;          /* isqrt(hi - 1): the largest prime to sieve by */
+    uint32_t nbands;       /* ceil(nchunks / kBucketBandChunks) */
+    uint32_t pad;
 };
 /* Chunks one bucket span may cover (a 2^34-wide range and one boundary chunk). */
 constexpr uint32_t kBucketMaxChunks = 8200;
-/* Workgroups of the bucket fill, and so buckets per chunk. */
-constexpr uint32_t kBucketGroups = 256;
+/* Chunks per band: a fill workgroup keeps one partly written line per band open, and 2^33
+   bits make 256 bands, so the open lines of the workgroups of one XCD fit in its L2. */
+constexpr uint32_t kBucketBandChunks = 16;
+constexpr uint32_t kBucketMaxBands = (kBucketMaxChunks + kBucketBandChunks - 1) / kBucketBandChunks;
+/* Workgroups of the bucket fill (two per CU), and so band buckets per band. */
+constexpr uint32_t kBucketGroups = 512;
+/* Split workgroups per band, each taking an equal share of the fill workgroups' buckets;
+   and so buckets per chunk. */
+constexpr uint32_t kBucketSplitParts = 4;
 
 /* Words of the pre-sieve per segment (one per lane of a wave). */
 constexpr uint64_t kPresieveWordsPerSeg = 64;
@@ -114,11 +128,18 @@ hipError_t launch_primes32(const uint32_t *small, uint32_t nsmall, uint32_t nblo
                            uint32_t *counts, const uint64_t *offsets, uint32_t *out,
                            hipStream_t s);
 
-/* The bucket fill of the bucket spans (kBucketGroups workgroups; see BucketSpan). */
-hipError_t launch_bucket_fill(const BatchSpan *spans, const uint64_t *prefix,
-                              const BucketSpan *bspans, uint32_t nb, const uint32_t *primes,
-                              uint32_t *entries, uint32_t *counts, uint32_t cap,
-                              uint32_t groups, hipStream_t s);
+/* The bucket sieve of the bucket spans (see BucketSpan): the fill (kBucketGroups
+   workgroups) into bands, then the split into per-chunk buckets (kBucketSplitParts
+   workgroups per band, `nbands` bands in all). */
+struct BucketStore {
+    uint32_t *band_entries, *band_counts;
+    uint32_t band_cap;
+    uint32_t *entries, *counts;
+    uint32_t cap;
+};
+hipError_t launch_bucket_sieve(const BatchSpan *spans, const uint64_t *prefix,
+                               const BucketSpan *bspans, uint32_t nb, uint32_t nbands,
+                               const uint32_t *primes, const BucketStore &st, hipStream_t s);
 
 /* Writes the one-span table of the range [lo, hi) (segments from lo | 1) into span and
    prefix (device scratch), so that a single range needs no host copy. */

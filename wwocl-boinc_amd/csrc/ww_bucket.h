@@ -56,11 +56,11 @@ constexpr WheelTables make_wheel()
 /* Largest multiple of the wheel's gaps: 2^gap < 2^11 keeps x * 2^gap below 2^43. */
 constexpr u32 kWheelMaxGap = 10;
 
-/* x = q d + r for x < 2^64 and 2^20 <= d < 2^32, with dinv = 1/d rounded to double. The
-   double quotient is within 3 * 2^-53 of x/d relatively, and x/d < 2^44, so it is off by
-   less than 2^-7: truncated it is the true quotient or one away from it, and the signed
-   remainder x - q d (exact mod 2^64, its true value lies in (-d, 2d)) decides the one
-   correction. */
+/* x = q d + r for x < 2^64 and 2^20 <= d < 2^32, with dinv = 1/d rounded to double (or
+   inv_below(d)). The double quotient is within 2^-49 of x/d relatively, and x/d < 2^44, so
+   it is off by less than 2^-5: truncated it is the true quotient or one away from it, and
+   the signed remainder x - q d (exact mod 2^64, its true value lies in (-d, 2d)) decides
+   the one correction. */
 WW_HD void divmod_u64_u32(u64 x, u32 d, double dinv, u64 *q, u32 *r)
 {
     u64 qe = (u64)((double)x * dinv);
@@ -76,18 +76,21 @@ WW_HD void divmod_u64_u32(u64 x, u32 d, double dinv, u64 *q, u32 *r)
     *r = (u32)rs;
 }
 
-/* x 2^v mod l for x < l < 2^32 and v <= 32 (linv = 1/l rounded): the quotient is below
-   2^v, and the double quotient of the exact double x 2^v is within 2^-51 of it relatively,
-   so off by less than 2^-19; truncated it is the true quotient or one away, and one signed
-   correction either way gives the residue. */
+/* 1/l rounded to double and nudged down by 2^-50 relatively: every quotient estimate
+   below is then at most the true quotient and, for quotients below 2^32, at most one
+   below it (its relative error stays within 2^-49). */
+WW_HD double inv_below(u32 l) { return (1.0 / (double)l) * (1.0 - 0x1p-50); }
+
+/* x 2^v mod l for x < l < 2^32, v <= 32 and x 2^v < 2^63, with linv = inv_below(l). The
+   quotient is below 2^v <= 2^32, and the estimate from the exact double x 2^v is the true
+   quotient or one less, so the remainder x 2^v - q l lies in [0, 2l) and one conditional
+   subtraction finishes it. */
 WW_HD u32 mulpow2_u32(u32 x, u32 v, u32 l, double linv)
 {
     const u64 t = (u64)x << v;
-    const u64 q = (u64)(ldexp((double)x, (int)v) * linv);
-    int64_t r = (int64_t)(t - q * (u64)l);
-    if (r < 0) r += l;
-    else if (r >= (int64_t)l) r -= l;
-    return (u32)r;
+    const u32 q = (u32)(ldexp((double)x, (int)v) * linv);
+    const u64 r = t - (u64)q * l;
+    return (u32)r - (r >= (u64)l ? l : 0u);
 }
 
 /* -l^-1 mod 2^32 for odd l: l * l = 1 (mod 8), and each Newton step y(2 - l y) doubles
@@ -104,18 +107,23 @@ WW_HD u32 neg_inv_u32(u32 l)
 
 /* Montgomery product a b 2^-32 mod l for a, b < l < 2^32 odd, linv = -l^-1 mod 2^32.
    t + m l = 0 (mod 2^32), so (t + m l) / 2^32 = hi(t) + hi(m l) + carry, and the low words
-   add up to 0 or 2^32: the carry is 1 exactly when lo(t) != 0. The sum is below
-   l^2 / 2^32 + l < 2l, so one subtraction completes it. */
+   add up to 0 or 2^32: the carry is 1 exactly when lo(t) != 0. hi(t) + carry stays below
+   2^32 (t < l^2), and the full sum, below l^2 / 2^32 + l < 2l, may pass 2^32 once: then, or
+   when it is at least l, one subtraction of l finishes it (mod 2^32 arithmetic). */
 WW_HD u32 mont_mul_u32(u32 a, u32 b, u32 l, u32 linv)
 {
     const u64 t = (u64)a * b;
-    const u32 m = (u32)t * linv;
-    const u64 u = (t >> 32) + (((u64)m * l) >> 32) + ((u32)t != 0u ? 1u : 0u);
-    return u >= l ? (u32)(u - l) : (u32)u;
+    const u32 tl = (u32)t, th = (u32)(t >> 32);
+    const u32 m = tl * linv;
+    const u32 mh = (u32)(((u64)m * l) >> 32);
+    const u32 s = th + (tl != 0u ? 1u : 0u);
+    const u32 u = s + mh;
+    return (u < s || u >= l) ? u - l : u;
 }
 
-/* 2^e mod l for e < 2^32 and 2^20 <= l < 2^32 prime: Montgomery squarings over 4-bit
-   windows of e from the top, each window's 2^w entered by one exact multiplication. */
+/* 2^e mod l for e < 2^32 and 2^20 <= l < 2^32 prime (linv = inv_below(l)): Montgomery
+   squarings over 4-bit windows of e from the top, each window's 2^w entered by one exact
+   multiplication. */
 WW_HD u32 pow2_mod_u32(u32 e, u32 l, double linv)
 {
     const u32 li = neg_inv_u32(l);
@@ -144,6 +152,7 @@ struct BucketStart {
 WW_HD BucketStart bucket_start(u64 G0, u32 l, double linv, const uint16_t *res,
                                const uint8_t *next)
 {
+    /* linv = inv_below(l) */
     u64 q;
     u32 r;
     divmod_u64_u32(G0, l, linv, &q, &r);
@@ -161,7 +170,7 @@ WW_HD BucketStart bucket_start(u64 G0, u32 l, double linv, const uint16_t *res,
     /* x = 2^(k-1) mod l, the exponent reduced mod l - 1 */
     u64 q2;
     u32 e;
-    divmod_u64_u32(k - 1, l - 1, 1.0 / (double)(l - 1), &q2, &e);
+    divmod_u64_u32(k - 1, l - 1, inv_below(l - 1), &q2, &e);
     s.x = pow2_mod_u32(e, l, linv);
     return s;
 }

@@ -36,6 +36,22 @@ constexpr u32 kPrimeBlockBits = 1u << 20; /* odd numbers per block of the prime 
 constexpr u32 kPrimeBlockWords = kPrimeBlockBits / 32;
 constexpr u32 kPrimeWordsPerThread = kPrimeBlockWords / kPrimeThreads; /* 32 */
 
+/* A slot in the LDS counter cnt[key] for every active lane, as atomicAdd(&cnt[key], 1)
+   would give it. The lanes sharing the lowest active lane's key (all of them when a wave's
+   hits cluster, as a small prime's do) take theirs with one atomic; the others add their
+   own, so no 64-way conflict on one counter serialises the wave. */
+__device__ __forceinline__ u32 wave_slot(u32 *cnt, u32 key)
+{
+    const u32 k0 = __builtin_amdgcn_readfirstlane(key);
+    const bool same = key == k0;
+    const u64 m = __ballot(same);
+    const u32 below = __builtin_amdgcn_mbcnt_hi((u32)(m >> 32), __builtin_amdgcn_mbcnt_lo((u32)m, 0u));
+    u32 base = 0;
+    if (same && below == 0) base = atomicAdd(&cnt[k0], (u32)__popcll(m));
+    base = __builtin_amdgcn_readfirstlane(base); /* the lowest active lane took it */
+    return same ? base + below : atomicAdd(&cnt[key], 1u);
+}
+
 /* Exclusive prefix sum of one value per thread over a 1024-thread workgroup; returns the
    thread's prefix and stores the total in *total (same in every thread). */
 __device__ __forceinline__ u32 block_scan_1024(u32 v, u32 *wsum, u32 *total)
@@ -112,19 +128,21 @@ __global__ __launch_bounds__(kPrimeThreads) void k_primes32(const u32 *small, u3
 
 /* The bucket fill. For every bucket span f, every workgroup walks its share of the
    span's sieving primes (blocks of 64 consecutive table entries, round-robin over the
-   workgroups, up to the first prime above the span's top) and writes each hit
-   to its (chunk, workgroup) bucket: entry = odd-number index in the chunk, top bit set
-   when the hit is in the prime's pseudoprime class. The counts are written for every
-   (chunk, workgroup) pair of the span, so nothing needs clearing between launches; a count
-   above `cap` means entries were lost and k_presieve treats that chunk as not sieved. */
+   workgroups, up to the first prime above the span's top) and writes each hit to its
+   workgroup's bucket of the hit's band: entry = odd-number index in the chunk, the chunk's
+   index in the band above it, and the top bit when the hit is in the prime's pseudoprime
+   class. The slots come from the workgroup's LDS cursor per band, so a band bucket fills
+   front to back and only one partly written line per band is open at a time. The counts
+   are written for every (band, workgroup) pair of the span, so nothing needs clearing
+   between launches; a count above band_cap means entries were lost (the split passes that
+   on to the band's chunks). */
 __global__ __launch_bounds__(kFillThreads) void k_bucket_fill(const BatchSpan *spans,
                                                               const u64 *prefix,
                                                               const BucketSpan *bspans,
                                                               u32 nb, const u32 *primes,
-                                                              u32 *entries, u32 *counts,
-                                                              u32 cap)
+                                                              BucketStore st)
 {
-    __shared__ u32 cnt[kBucketMaxChunks];
+    __shared__ u32 cur[kBucketMaxBands];
     __shared__ uint16_t res[kWheelRes];
     __shared__ uint8_t gap[kWheelRes];
     __shared__ uint8_t nxt[kWheelMod];
@@ -138,17 +156,19 @@ __global__ __launch_bounds__(kFillThreads) void k_bucket_fill(const BatchSpan *s
         if (t < kWheelMod) nxt[t] = wt.next[t];
     }
     const u32 groups = gridDim.x, wg = blockIdx.x;
+    const u32 lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (u32 f = 0; f < nb; ++f) {
         const BucketSpan bs = bspans[f];
         const BatchSpan sp = spans[bs.span];
         const u64 G0 = (sp.lo | 1) + sp.seg0 * kIntsPerWave;
         const u64 nbits = (prefix[bs.span + 1] - prefix[bs.span]) * 4096;
-        for (u32 j = threadIdx.x; j < bs.nchunks; j += kFillThreads) cnt[j] = 0;
+        for (u32 j = threadIdx.x; j < bs.nbands; j += kFillThreads) cur[j] = 0;
+        const u64 wbase = bs.band_ent_base + (u64)wg * bs.nbands * st.band_cap;
+        const u32 bstride = st.band_cap;
         __syncthreads();
         /* wave v of workgroup w takes the 64-prime blocks w + groups (v + 16 r), r = 0, 1,
            ...: neighbouring workgroups get neighbouring blocks, so their hit counts per
-           chunk differ by about one block's, and each wave reads 256 contiguous bytes */
-        const u32 lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+           band differ by about one block's, and each wave reads 256 contiguous bytes */
         for (u64 blk = bs.pidx0 + 64 * ((u64)wg + (u64)groups * wave); blk < bs.pidx1;
              blk += 64 * (u64)groups * (kFillThreads / 64)) {
             if (primes[blk] > bs.top) break; /* ascending: the rest is above the top too */
@@ -156,25 +176,134 @@ __global__ __launch_bounds__(kFillThreads) void k_bucket_fill(const BatchSpan *s
             if (i >= bs.pidx1) continue;
             const u32 l = primes[i];
             if (l > bs.top) continue;
-            const double linv = 1.0 / (double)l;
+            const double linv = inv_below(l);
             BucketStart s = bucket_start(G0, l, linv, res, nxt);
-            u64 b = s.b;
+            /* walk in global odd-number indices; band buckets of this workgroup at
+               wbase + band * bstride */
+            u64 B = bs.bit0 + s.b;
+            const u64 Bend = bs.bit0 + nbits;
             u32 idx = s.idx, x = s.x;
-            while (b < nbits) {
-                const u64 B = bs.bit0 + b;
+            while (B < Bend) {
                 const u32 j = (u32)(B >> kBucketChunkBits) - bs.chunk_first;
-                const u32 slot = atomicAdd(&cnt[j], 1u);
-                if (slot < cap)
-                    entries[bs.ent_base + ((u64)j * groups + wg) * cap + slot] =
-                        ((u32)B & ((1u << kBucketChunkBits) - 1)) | (x == 1u ? kBucketRescue : 0u);
-                bucket_step(b, idx, x, l, linv, gap);
+                const u32 band = j / kBucketBandChunks;
+                const u32 slot = wave_slot(cur, band);
+                if (slot < st.band_cap)
+                    st.band_entries[wbase + (u64)band * bstride + slot] =
+                        ((u32)B & ((1u << kBucketChunkBits) - 1)) |
+                        ((j % kBucketBandChunks) << kBucketChunkBits) | (x == 1u ? kBucketRescue : 0u);
+                bucket_step(B, idx, x, l, linv, gap);
             }
         }
         __syncthreads();
-        for (u32 j = threadIdx.x; j < bs.nchunks; j += kFillThreads)
-            counts[bs.cnt_base + (u64)j * groups + wg] = cnt[j];
+        for (u32 j = threadIdx.x; j < bs.nbands; j += kFillThreads)
+            st.band_counts[bs.band_cnt_base + (u64)wg * bs.nbands + j] = cur[j];
         __syncthreads();
     }
+}
+
+/* The split: kBucketSplitParts workgroups per band of a bucket span, part p reading the
+   band's buckets of fill workgroups p, p + parts, ... and appending their entries to its
+   own bucket of each chunk of the band. A wave works on its own: it loads 16 entries per
+   lane (16 loads in flight), and sorts them 64 at a time into 16 LDS bins of its own, one
+   per chunk (slot from a per-bin LDS counter). A bin that reaches 64 entries is written out
+   as one coalesced 256-byte store at a slot block taken from the workgroup's counter of
+   that chunk, and its rest moves to the front; the partial bins are written at the end. A
+   chunk part whose entries do not all fit, or whose band lost entries in the fill, gets a
+   count above cap. */
+constexpr u32 kSplitWaves = kFillThreads / 64;
+constexpr u32 kSplitBin = 128; /* fewer than 64 left after a flush, plus 64 per step */
+
+__device__ __forceinline__ void split_flush(u32 *bin, u32 *bcnt, u32 jj, u32 n, u32 *cnt,
+                                            u32 *out, u32 cap, u32 lane)
+{
+    /* write the first n (<= 64) entries of bin jj to the chunk's bucket, move the rest of
+       the bin (bcnt[jj] - n, below 64) to the front */
+    const u32 have = bcnt[jj];
+    u32 base = 0;
+    if (lane == 0) base = atomicAdd(&cnt[jj], n);
+    base = __builtin_amdgcn_readfirstlane(base);
+    const u32 v = lane < n ? bin[jj * kSplitBin + lane] : 0u;
+    const u32 rest = have - n;
+    const u32 mv = lane < rest ? bin[jj * kSplitBin + n + lane] : 0u;
+    if (lane < n && base + lane < cap) out[(u64)jj * kBucketSplitParts * cap + base + lane] = v;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (lane < rest) bin[jj * kSplitBin + lane] = mv;
+    if (lane == 0) bcnt[jj] = rest;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__global__ __launch_bounds__(kFillThreads) void k_bucket_split(const BucketSpan *bspans, u32 nb,
+                                                               BucketStore st, u32 groups)
+{
+    __shared__ u32 cnt[kBucketBandChunks];
+    __shared__ u32 lost;
+    __shared__ u32 bins[kSplitWaves][kBucketBandChunks * kSplitBin];
+    __shared__ u32 bcnts[kSplitWaves][kBucketBandChunks];
+    constexpr int kUnroll = 16;
+    /* this workgroup's band and part: the bands of the spans in table order */
+    const u32 part = blockIdx.x % kBucketSplitParts;
+    u32 f = 0, band = blockIdx.x / kBucketSplitParts;
+    while (f < nb && band >= bspans[f].nbands) band -= bspans[f++].nbands;
+    if (f >= nb) return;
+    const BucketSpan bs = bspans[f];
+    const u32 lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    u32 *bin = bins[wave];
+    u32 *bcnt = bcnts[wave];
+    if (threadIdx.x < kBucketBandChunks) cnt[threadIdx.x] = 0;
+    if (threadIdx.x == 0) lost = 0;
+    if (lane < kBucketBandChunks) bcnt[lane] = 0;
+    __syncthreads();
+    const u64 first_chunk = (u64)band * kBucketBandChunks;
+    /* chunk jj's bucket of this part: out + jj * parts * cap */
+    u32 *out = st.entries + bs.ent_base + (first_chunk * kBucketSplitParts + part) * st.cap;
+    for (u32 w = part + kBucketSplitParts * wave; w < groups; w += kBucketSplitParts * kSplitWaves) {
+        u32 n = __builtin_amdgcn_readfirstlane(st.band_counts[bs.band_cnt_base + (u64)w * bs.nbands + band]);
+        if (n > st.band_cap) {
+            if (lane == 0) lost = 1;
+            n = st.band_cap;
+        }
+        const u32 *run = st.band_entries + bs.band_ent_base + ((u64)w * bs.nbands + band) * st.band_cap;
+        for (u32 i0 = 0; i0 < n; i0 += 64 * kUnroll) {
+            u32 e[kUnroll];
+#pragma unroll
+            for (int u = 0; u < kUnroll; ++u) {
+                const u32 i = i0 + 64 * u + lane;
+                e[u] = i < n ? run[i] : 0xffffffffu;
+            }
+#pragma unroll
+            for (int u = 0; u < kUnroll; ++u) {
+                if (i0 + 64 * u >= n) break; /* wave-uniform */
+                if (e[u] != 0xffffffffu) {
+                    const u32 jj = (e[u] >> kBucketChunkBits) & (kBucketBandChunks - 1);
+                    const u32 slot = wave_slot(bcnt, jj);
+                    bin[jj * kSplitBin + slot] = e[u] & (((1u << kBucketChunkBits) - 1) | kBucketRescue);
+                }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                /* bins that reached 64 entries */
+                u64 full = __ballot(lane < kBucketBandChunks && bcnt[lane] >= 64);
+                while (full) {
+                    const u32 jj = (u32)__builtin_ctzll(full);
+                    split_flush(bin, bcnt, jj, 64, cnt, out, st.cap, lane);
+                    full &= full - 1;
+                }
+            }
+        }
+    }
+    /* the partial bins */
+    for (u32 jj = 0; jj < kBucketBandChunks; ++jj) {
+        const u32 have = __builtin_amdgcn_readfirstlane(bcnt[jj]);
+        if (have) split_flush(bin, bcnt, jj, have, cnt, out, st.cap, lane);
+    }
+    __syncthreads();
+    if (threadIdx.x < kBucketBandChunks && first_chunk + threadIdx.x < bs.nchunks)
+        st.counts[bs.cnt_base + (first_chunk + threadIdx.x) * kBucketSplitParts + part] =
+            lost ? st.cap + 1 : cnt[threadIdx.x];
 }
 
 } // namespace
@@ -187,12 +316,16 @@ hipError_t launch_primes32(const u32 *small, u32 nsmall, u32 nblocks, u32 *count
     return hipGetLastError();
 }
 
-hipError_t launch_bucket_fill(const BatchSpan *spans, const u64 *prefix, const BucketSpan *bspans,
-                              u32 nb, const u32 *primes, u32 *entries, u32 *counts, u32 cap,
-                              u32 groups, hipStream_t s)
+hipError_t launch_bucket_sieve(const BatchSpan *spans, const u64 *prefix, const BucketSpan *bspans,
+                               u32 nb, u32 nbands, const u32 *primes, const BucketStore &st,
+                               hipStream_t s)
 {
-    hipLaunchKernelGGL(k_bucket_fill, dim3(groups), dim3(kFillThreads), 0, s, spans, prefix,
-                       bspans, nb, primes, entries, counts, cap);
+    hipLaunchKernelGGL(k_bucket_fill, dim3(kBucketGroups), dim3(kFillThreads), 0, s, spans, prefix,
+                       bspans, nb, primes, st);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || nbands == 0) return e;
+    hipLaunchKernelGGL(k_bucket_split, dim3(nbands * kBucketSplitParts), dim3(kFillThreads), 0, s,
+                       bspans, nb, st, (u32)kBucketGroups);
     return hipGetLastError();
 }
 

@@ -58,27 +58,27 @@ __device__ __forceinline__ u32 span_of(const u64 *prefix, u32 nspans, u64 s)
     return a;
 }
 
-/* Applies the buckets of one chunk row of a bucket span to the LDS bitmap: marks the hits
-   and appends the pseudoprime-class hits to the rescue list; sets *over when a bucket
-   overflowed (entries were lost). Called by every thread; each wave takes every 16th
-   bucket and reads it with 16 loads in flight per lane. */
+/* Applies the buckets of chunk j of a bucket span (one per split part) to the LDS bitmap:
+   marks the hits and appends the pseudoprime-class hits to the rescue list; sets *over when
+   a bucket overflowed (entries were lost). Called by every thread, with 16 loads in flight
+   per lane. */
 __device__ __forceinline__ void apply_buckets(const BucketArgs &bk, const BucketSpan &bs, u32 j,
                                               u32 *bm, u32 *rescue, u32 *rcount, u32 *over)
 {
     constexpr int kUnroll = 16;
-    const u32 lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (u32 w = wave; w < kBucketGroups; w += kPreThreads / 64) {
-        u32 n = __builtin_amdgcn_readfirstlane(bk.counts[bs.cnt_base + (u64)j * kBucketGroups + w]);
+    for (u32 p = 0; p < kBucketSplitParts; ++p) {
+        const u64 at0 = (u64)j * kBucketSplitParts + p;
+        u32 n = bk.counts[bs.cnt_base + at0];
         if (n > bk.cap) {
-            if (lane == 0) *over = 1;
+            if (threadIdx.x == 0) *over = 1;
             n = bk.cap;
         }
-        const u32 *run = bk.entries + bs.ent_base + ((u64)j * kBucketGroups + w) * bk.cap;
-        for (u32 i0 = 0; i0 < n; i0 += 64 * kUnroll) {
+        const u32 *run = bk.entries + bs.ent_base + at0 * bk.cap;
+        for (u32 i0 = 0; i0 < n; i0 += kPreThreads * kUnroll) {
             u32 e[kUnroll];
 #pragma unroll
             for (int u = 0; u < kUnroll; ++u) {
-                const u32 i = i0 + 64 * u + lane;
+                const u32 i = i0 + kPreThreads * u + threadIdx.x;
                 e[u] = i < n ? run[i] : 0xffffffffu;
             }
 #pragma unroll
