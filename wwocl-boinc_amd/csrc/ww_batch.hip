@@ -24,6 +24,9 @@ namespace ww {
 namespace {
 
 constexpr int kQueueCap = 128; /* < 64 left over + <= 64 appended per PRP round */
+/* candidates awaiting the strong PRP test outside the candidate buffer (p, range slot):
+   fewer than 64 between rounds, plus one segment's when they fit */
+constexpr u32 kPspCap = 96;
 /* candidates of one segment (at most kWaveCandCap) behind fewer than 64 leftovers */
 constexpr int kBatchCandCap = kWaveCandCap + 64;
 static_assert(kBatchCandCap - 256 >= 64, "rescue bitmap must not reach the leftovers");
@@ -96,16 +99,19 @@ __device__ __forceinline__ void tests_round(u32 n, u32 &qn, u64 *wq, u32 *ws, u3
 }
 
 /* Wave compaction of a segment's candidates as compact_by_class does it, split in two:
-   the candidates in `need` (those the strong PRP test must decide) go first, grouped by the
-   trailing zeros of n - 1 in the order `desc` gives; the others, known primes, follow as one
-   group tagged with kKnown, so that PRP rounds over them skip the test. The five counts
-   share one 64-bit scan in 12-bit fields (a segment holds at most kWaveCandCap < 4096
-   candidates). Returns the number of entries written, wave-uniform. Called by all 64
-   lanes. */
+   the candidates in `need` (those the strong PRP test must decide) and the others, known
+   primes, which go to the candidate buffer as one group tagged with kKnown, so that PRP
+   rounds over them skip the test. The candidates to test go to the PRP queue pq/ps (as
+   p = G + 2 i with the range slot) when they all fit behind its pn entries; otherwise they
+   go to the candidate buffer first, grouped by the trailing zeros of n - 1 in the order
+   `desc` gives. The five counts share one 64-bit scan in 12-bit fields (a segment holds at
+   most kWaveCandCap < 4096 candidates). Returns the number of entries written to the
+   buffer, wave-uniform, and advances pn. Called by all 64 lanes. */
 constexpr u32 kKnown = 1u << 13;
 
 __device__ __forceinline__ u32 compact_split(u64 bits, u64 need, u64 G, u32 lane, uint16_t *wc,
-                                             u32 base, u32 tag, bool desc)
+                                             u32 base, u32 tag, bool desc, u64 *pq, u32 *ps,
+                                             u32 &pn, u32 slot)
 {
     const u32 g8 = (u32)((G - 1) >> 1) & 7u;
     u64 cls[4];
@@ -124,17 +130,32 @@ __device__ __forceinline__ u32 compact_split(u64 bits, u64 need, u64 G, u32 lane
     }
     const u64 tot = uniform64(__shfl(incl, 63, 64));
     const u64 excl = incl - cnt;
+    const u32 ptot = (u32)((tot & 0xfffu) + ((tot >> 12) & 0xfffu) + ((tot >> 24) & 0xfffu) +
+                           ((tot >> 36) & 0xfffu));
     u32 at = base;
-#pragma unroll
-    for (int kk = 0; kk < 4; ++kk) {
-        const int k = desc ? 3 - kk : kk;
-        u32 pos = at + (u32)((excl >> (12 * k)) & 0xfffu);
-        u64 b = test & cls[k];
+    if (pn + ptot <= kPspCap) {
+        u32 pos = pn + (u32)((excl & 0xfffu) + ((excl >> 12) & 0xfffu) + ((excl >> 24) & 0xfffu) +
+                             ((excl >> 36) & 0xfffu));
+        u64 b = test;
         while (b) {
-            wc[pos++] = (uint16_t)(tag | (lane * 64 + (u32)__builtin_ctzll(b)));
+            pq[pos] = G + 2ull * (lane * 64 + (u32)__builtin_ctzll(b));
+            ps[pos] = slot;
+            ++pos;
             b &= b - 1;
         }
-        at += (u32)((tot >> (12 * k)) & 0xfffu);
+        pn += ptot;
+    } else {
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) {
+            const int k = desc ? 3 - kk : kk;
+            u32 pos = at + (u32)((excl >> (12 * k)) & 0xfffu);
+            u64 b = test & cls[k];
+            while (b) {
+                wc[pos++] = (uint16_t)(tag | (lane * 64 + (u32)__builtin_ctzll(b)));
+                b &= b - 1;
+            }
+            at += (u32)((tot >> (12 * k)) & 0xfffu);
+        }
     }
     u32 pos = at + (u32)((excl >> 48) & 0xfffu);
     u64 b = known;
@@ -166,15 +187,19 @@ __attribute__((amdgpu_waves_per_eu(BatchShape<W>::kWavesPerEU))) void k_batch(co
     __shared__ u64 deep_bm[W][64];
     __shared__ u64 qp[W][kQueueCap];
     __shared__ u32 qs[W][kQueueCap];
+    __shared__ u64 pp[W][kPspCap];
+    __shared__ u32 psl[W][kPspCap];
 
     const u32 lane = threadIdx.x & 63;
     const u32 wave = threadIdx.x >> 6;
     uint16_t *wc = cand[wave];
     u64 *wq = qp[wave];
     u32 *ws = qs[wave];
+    u64 *pq = pp[wave];
+    u32 *ps = psl[wave];
     const u64 total = prefix[nspans];
 
-    u32 qn = 0;
+    u32 qn = 0, pn = 0;
     WaveAcc acc;
     acc.slot = 0xffffffffu;
     acc.cks = acc.cnt = 0;
@@ -201,15 +226,42 @@ __attribute__((amdgpu_waves_per_eu(BatchShape<W>::kWavesPerEU))) void k_batch(co
             continue;
         }
         const u32 rem = j0 < ncand ? ncand - j0 : 0;
-        if (rem >= 64 || (rem > 0 && (done || j0 < kb))) {
-            /* strong PRP, 64 candidates per round; survivors join the test queue */
-            const u32 j = j0 + lane;
-            const bool in = j < ncand;
-            const u32 e = wc[in ? j : j0];
-            const bool newest = (e >> 12) & 1;
-            const u64 n = (newest ? g_new : g_old) + 2ull * (e & 0xfffu);
-            /* known primes pass; the rest take the test (skipped when none is in the round) */
-            const bool test = in && !(e & kKnown);
+        const bool from_buf = rem >= 64 || (rem > 0 && (done || j0 < kb));
+        if (from_buf || pn >= 64 || (done && pn > 0)) {
+            /* 64 candidates per round, from the candidate buffer or else from the PRP
+               queue; known primes pass, the rest take the strong PRP test (skipped when
+               none is in the round); survivors join the test queue */
+            u64 n;
+            u32 sl;
+            bool in, test;
+            if (from_buf) {
+                const u32 j = j0 + lane;
+                in = j < ncand;
+                const u32 e = wc[in ? j : j0];
+                const bool newest = (e >> 12) & 1;
+                n = (newest ? g_new : g_old) + 2ull * (e & 0xfffu);
+                sl = newest ? s_new : s_old;
+                test = in && !(e & kKnown);
+                j0 += 64;
+            } else {
+                const u32 nr = pn >= 64 ? 64 : pn, rest = pn - nr;
+                in = lane < nr;
+                n = in ? pq[lane] : 0;
+                sl = in ? ps[lane] : 0;
+                test = in;
+                u64 pr = 0;
+                u32 sr = 0;
+                if (lane < rest) {
+                    pr = pq[nr + lane];
+                    sr = ps[nr + lane];
+                }
+                wave_sync_lds();
+                if (lane < rest) {
+                    pq[lane] = pr;
+                    ps[lane] = sr;
+                }
+                pn = rest;
+            }
             bool prp = in;
             if (__ballot(test)) {
                 const bool r = spsp2_wave(n, test); /* all lanes: it takes wave-wide bounds */
@@ -219,10 +271,9 @@ __attribute__((amdgpu_waves_per_eu(BatchShape<W>::kWavesPerEU))) void k_batch(co
             if (prp) {
                 const u32 at = qn + (u32)__popcll(bal & ((1ull << lane) - 1));
                 wq[at] = n;
-                ws[at] = newest ? s_new : s_old;
+                ws[at] = sl;
             }
             qn += (u32)__popcll(bal);
-            j0 += 64;
             wave_sync_lds();
             continue;
         }
@@ -290,13 +341,14 @@ __attribute__((amdgpu_waves_per_eu(BatchShape<W>::kWavesPerEU))) void k_batch(co
             bits &= ~deep_sieve(bm, rm, g_new, deep, ndeep, lane, &classes);
             need |= classes;
         }
-        /* compacted behind the leftovers: the candidates to test grouped by the trailing
-           zeros of n - 1, in the opposite order to the previous segment's, whose last group
-           the leftovers are; then the known primes */
+        /* compacted behind the leftovers: the candidates to test to the PRP queue when they
+           fit, else here grouped by the trailing zeros of n - 1, in the opposite order to
+           the previous segment's, whose last group the leftovers are; then the known
+           primes */
         kb = rem;
         j0 = 0;
         desc = !desc;
-        ncand = rem + compact_split(bits, need, g_new, lane, wc, rem, 1u << 12, desc);
+        ncand = rem + compact_split(bits, need, g_new, lane, wc, rem, 1u << 12, desc, pq, ps, pn, s_new);
         wave_sync_lds();
     }
     acc_flush(acc, sums, lane);
