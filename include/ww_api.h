@@ -155,6 +155,11 @@ int ww_get_presieve_bound(ww_ctx *ctx, uint32_t *bound);
 int ww_set_bucket_sieve(ww_ctx *ctx, uint64_t min_width);
 int ww_get_bucket_sieve(ww_ctx *ctx, uint64_t *min_width);
 
+/* Scale (0 < scale <= 16, default 1) of the bucket capacities of batches created
+   afterwards. Below 1 buckets overflow: the chunks concerned then test all their
+   candidates, which is exact but slower, so this is for testing that path. */
+int ww_set_bucket_capacity(ww_ctx *ctx, double scale);
+
 /* The context's table of the odd primes below 2^32 (built on first use, 203,280,220
    entries): copies entries [first, first + n) to out (host) and writes the entry count to
    *total. Verification; blocking. */

@@ -79,6 +79,7 @@ struct ww_ctx {
     uint64_t nprimes32 = 0;
     std::vector<uint64_t> primes32_prefix;
     uint64_t bucket_min_width = WW_DEFAULT_BUCKET_MIN_WIDTH;
+    double bucket_scale = 1.0;
     /* event pairs of launches whose durations have not been collected yet */
     std::vector<hipEvent_t> ev_sieve, ev_tests;
     double ms_sieve = 0, ms_tests = 0;
@@ -560,6 +561,15 @@ int ww_get_bucket_sieve(ww_ctx *c, uint64_t *min_width)
     WW_LOCK(c);
     if (!c || !min_width) return fail(WW_E_ARG, "null argument");
     *min_width = c->bucket_min_width;
+    return WW_OK;
+}
+
+int ww_set_bucket_capacity(ww_ctx *c, double scale)
+{
+    WW_LOCK(c);
+    if (!c) return fail(WW_E_ARG, "null context");
+    if (!(scale > 0 && scale <= 16)) return fail(WW_E_ARG, "bucket capacity scale %g outside (0, 16]", scale);
+    c->bucket_scale = scale;
     return WW_OK;
 }
 
@@ -1100,8 +1110,9 @@ int ww_batch_create(ww_ctx *c, const ww_range *ranges, uint32_t n, uint32_t resu
             spans[i].pad = (uint32_t)b->bspans.size();
         }
         if (!b->bspans.empty()) {
-            b->bcap = (uint32_t)std::ceil(1.15 * mean / ww::kBucketSplitParts + 512);
-            b->band_cap = (uint32_t)std::ceil(1.25 * mean * ww::kBucketBandChunks / ww::kBucketGroups + 256);
+            const double sc = c->bucket_scale;
+            b->bcap = (uint32_t)std::ceil(sc * (1.15 * mean / ww::kBucketSplitParts + 512));
+            b->band_cap = (uint32_t)std::ceil(sc * (1.25 * mean * ww::kBucketBandChunks / ww::kBucketGroups + 256));
             b->nbands = (uint32_t)bands;
             for (ww::BucketSpan &bs : b->bspans) {
                 bs.ent_base = bs.cnt_base * b->bcap;

@@ -195,6 +195,7 @@ _SIGNATURES = [
     ("ww_cpu_range", ctypes.c_int, [U64, U64, ctypes.POINTER(U64)]),
     ("ww_set_bucket_sieve", ctypes.c_int, [ctypes.c_void_p, U64]),
     ("ww_get_bucket_sieve", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(U64)]),
+    ("ww_set_bucket_capacity", ctypes.c_int, [ctypes.c_void_p, ctypes.c_double]),
     ("ww_prime_table", ctypes.c_int, [ctypes.c_void_p, U64, U64, ctypes.POINTER(U32),
                                       ctypes.POINTER(U64)]),
     ("ww_cpu_bucket_hits", ctypes.c_int, [U64, U64, U32, ctypes.POINTER(U64),
@@ -372,6 +373,11 @@ class Context:
         """Ranges at least min_width wide get the bucket sieve in batches made afterwards
         (2**64 - 1: none)."""
         _check(self.lib.ww_set_bucket_sieve(self.h, min_width))
+
+    def set_bucket_capacity(self, scale: float):
+        """Scale of the bucket capacities of batches made afterwards (below 1: overflows,
+        for testing the exact fallback)."""
+        _check(self.lib.ww_set_bucket_capacity(self.h, scale))
 
     def bucket_sieve(self) -> int:
         v = U64()
