@@ -2,11 +2,13 @@
 
 One step = the whole range [2^63, 2^63 + 2^33) (196,705,536 tested primes) through the HIP
 path, inputs generated on the device, the checksum accumulated on the device. The headline
-`value` is the persistent form (the wide pre-sieve k_presieve, then one graph-launched
-k_batch: sieve, PRP, Wieferich and Wall-Sun-Sun in one kernel); the plain form (per 2^32-wide chunk: clear the list, sieve +
-PRP, then the test kernel) runs beside it and must give the same step checksum. Also: the
-4096-range batched workload, the autotuner, the serving form's latency, and the reference's
-own CPU path on a bounded sample. Prints one JSON line.
+`value` is the persistent form: the sieve stage (the bucket sieve's fill and split, then the
+wide pre-sieve k_presieve, which applies the buckets), then one graph-launched k_batch
+(sieve by the small primes, strong PRP of the candidates not known to be prime, Wieferich
+and Wall-Sun-Sun in one kernel). The plain form (per 2^32-wide chunk: clear the list, sieve
++ PRP of every candidate, then the test kernel) runs beside it and must give the same step
+checksum. Also: the 4096-range batched workload, the autotuner, the serving form's latency,
+and the reference's own CPU path on a bounded sample. Prints one JSON line.
 
     python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--no-cpu] [--no-batch]
                     [--form both|persistent|plain] [--serve-seconds 10]
@@ -35,14 +37,15 @@ BITS = 64  # every p in the range has bit length 64
 # chunk (the 8-byte list read, register spills and the result atomics).
 TRAFFIC_BYTES_PER_ENTRY = 8.37
 # k_batch keeps candidates and PRPs on chip (LDS queues). Its algorithmic HBM traffic is the
-# wide pre-sieve's 64 words (512 B) per 8192-integer segment, read once; the rest is register
-# spill reloads, the deep-sieve table, span lookups and the cursor/slot atomics (each atomic
-# reads as a 32-byte write). Measured the same way at the defaults (deep bound 2048,
-# pre-sieve 2^20, 4-wave workgroups): 636.8 MB fetched + 23.1 MB written per step.
-BATCH_TRAFFIC_BYTES_PER_PRP = 3.355
+# pre-sieve's two sets of 64 words (512 B each) per 8192-integer segment, read once: the
+# odd numbers to drop and those that still need the strong PRP test. The rest is register
+# spill reloads, the deep-sieve table, span lookups and the cursor/slot atomics. Measured
+# the same way at the defaults (deep bound 2048, pre-sieve 2^20, bucket sieve on, 4-wave
+# workgroups): see profiles/k_batch_persistent_2p63_w2p33.md.
+BATCH_TRAFFIC_BYTES_PER_PRP = 6.540
 TRAFFIC_SOURCE = ("rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE, calibrated on a 1 GiB copy "
                   "(profiles/k_batch_persistent_2p63_w2p33.md)")
-PRESIEVE_BYTES_PER_SEGMENT = 512
+PRESIEVE_BYTES_PER_SEGMENT = 1024  # drop words + needs-PRP words
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
 
 
@@ -161,7 +164,8 @@ def run_plain(ctx, steps, warmup):
 
 
 def run_persistent(ctx, steps, warmup):
-    """Persistent form: the whole range in one graph-launched k_presieve + k_batch."""
+    """Persistent form: the whole range as two graphs, the sieve stage (k_bucket_fill,
+    k_bucket_split, k_presieve) and k_batch."""
     with wwlib.Batch(ctx, [(LO, HI)]) as b:
         res = None
         for _ in range(max(warmup, 1)):
@@ -179,12 +183,17 @@ def run_persistent(ctx, steps, warmup):
         ctx.finish()
         elapsed = time.perf_counter() - t0
         last = b.read()[0]
+        buckets = b.buckets()
     assert last[:2] == res[:2], "persistent form is not deterministic"
+    assert buckets["active"] and buckets["overflows"] == 0, buckets
     prps, cks = res[0], res[1]
     return {"value": prps * steps / elapsed, "ms_per_step": elapsed / steps * 1e3,
             "prps_per_step": prps, "step_checksum": f"{cks:016X}",
             "kernel_ms_per_step": kms / steps,
-            "kernels_ms_per_step": {"k_presieve": pre_ms / steps, "k_batch": search_ms / steps}}
+            "kernels_ms_per_step": {"sieve (k_bucket_fill + k_bucket_split + k_presieve)":
+                                    pre_ms / steps, "k_batch": search_ms / steps},
+            "bucket_sieve": {"bucket_bytes": buckets["bytes"],
+                             "overflowed_chunks": buckets["overflows"]}}
 
 
 def run_batched_workload(ctx):
@@ -335,7 +344,7 @@ def main():
         "vs_baseline": None,
         "dtype": "u64",
         "data": "synthetic: the number line [2^63, 2^63+2^33), generated on the device",
-        "config": {"workload": ("persistent form (k_presieve + one k_batch, graph-launched)"
+        "config": {"workload": ("persistent form (sieve stage + one k_batch, graph-launched)"
                                 if pers else
                                 "plain form (2^32-wide chunks)") +
                                ", range [2^63, 2^63+2^33)",
@@ -351,10 +360,14 @@ def main():
             "unit": "TMUL32/s", "frac": ach / peak,
             "traffic": pers["prps_per_step"] * BATCH_TRAFFIC_BYTES_PER_PRP,
             "traffic_algorithmic": ((HI - LO) // 8192) * PRESIEVE_BYTES_PER_SEGMENT,
+            "traffic_algorithmic_note": "the pre-sieve's drop words and needs-PRP words, "
+                                        "512 B each per 8192-integer segment",
             "traffic_source": TRAFFIC_SOURCE,
             "per_unit": f"{mul32_path(BITS)} MUL32 per tested prime (A(b) = 99b-123 at b=64: "
                         "PRP + Wieferich + Wall-Sun-Sun in one kernel, counted as the "
-                        "reference's algorithm spends them)",
+                        "reference's algorithm spends them; this build proves most primes "
+                        "by sieving to sqrt(hi) and runs the PRP test only where that "
+                        "cannot)",
             "executed_per_unit": f"{mul32_path_executed(BITS)} MUL32 per tested prime "
                                  "(78b-117: this build's ladders)",
             "avg_launch_ms": pers["kernels_ms_per_step"]["k_batch"], "launches": args.steps,
@@ -366,7 +379,7 @@ def main():
             "unit": "GB/s", "frac": hbm / 1e9 / HBM_PEAK_GBS,
             "traffic": pers["prps_per_step"] * BATCH_TRAFFIC_BYTES_PER_PRP,
             "note": "not the bound: the kernel keeps candidates and PRPs on chip and reads "
-                    "only the pre-sieve words (512 B per 8192 integers) from HBM"}
+                    "only the pre-sieve words (1 KB per 8192 integers) from HBM"}
     if plain is not None:
         tk = plain["tests_kernel"]
         key = "roofline" if pers is None else "roofline_plain_tests_kernel"
