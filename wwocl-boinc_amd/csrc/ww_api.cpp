@@ -1078,7 +1078,7 @@ int ww_batch_create(ww_ctx *c, const ww_range *ranges, uint32_t n, uint32_t resu
         e = hipMalloc(&b->d_pre, b->nseg * ww::kPresieveWordsPerSeg * sizeof(uint64_t));
     /* bucket spans: ranges at least bucket_min_width wide (and at most kBucketMaxChunks
        chunks). Buckets are sized for primes from 2^20 to sqrt(hi - 1): per 2^20-bit chunk,
-       2^20 x 48/105 (the cofactor wheel) x sum(1/l) hits, sum(1/l) from Mertens'
+       2^20 x 5760/15015 (the cofactor wheel) x sum(1/l) hits, sum(1/l) from Mertens'
        ln ln top - ln ln 2^20 (within 0.005 of the true sum). A chunk part's bucket (one of
        kBucketSplitParts) holds 1.15x its share of that mean plus 512; a band bucket (16
        chunks, one per fill workgroup) 1.25x its mean plus 256. An overflow only costs the
@@ -1105,7 +1105,7 @@ int ww_batch_create(ww_ctx *c, const ww_range *ranges, uint32_t n, uint32_t resu
             chunks += nch;
             bands += bs.nbands;
             const double lt = std::log((double)bs.top), l20 = 20.0 * std::log(2.0);
-            if (lt > l20) mean = std::max(mean, std::ldexp(1.0, 20) * 48.0 / 105.0 * std::log(lt / l20));
+            if (lt > l20) mean = std::max(mean, std::ldexp(1.0, 20) * 5760.0 / 15015.0 * std::log(lt / l20));
             b->bspans.push_back(bs);
             spans[i].pad = (uint32_t)b->bspans.size();
         }

@@ -7,10 +7,10 @@
  * kernel runs.
  *
  * For a sieving prime l and the odd numbers G0 + 2b of a range (G0 odd), the multiples
- * n = l*k are visited only for k prime to 210 = 2*3*5*7: n is odd and prime to 3, 5 and 7
- * exactly when k is, and any other n is no candidate (every candidate is odd and prime to
- * 3..113). That skips 57/105 of the odd multiples. Each visited n also gets its base-2
- * pseudoprime test modulo l: x = 2^(n-1) mod l. A composite n = l*k that is a base-2
+ * n = l*k are visited only for k prime to 30030 = 2*3*5*7*11*13: n is odd and prime to
+ * 3..13 exactly when k is, and any other n is no candidate (every candidate is odd and
+ * prime to 3..113). That skips 9255 of every 15015 odd multiples. Each visited n also
+ * gets its base-2 pseudoprime test modulo l: x = 2^(n-1) mod l. A composite n = l*k that is a base-2
  * Fermat pseudoprime (every strong one is) has x = 1, i.e. ord_l(2) | n - 1; such an n is
  * not dropped but flagged for the strong PRP test. By Fermat, 2^(n-1) = 2^(lk-1) =
  * (2^(l-1))^k 2^(k-1) = 2^(k-1) (mod l), so x starts as 2^(k0-1) mod l and each wheel
@@ -23,38 +23,49 @@
 
 namespace ww {
 
-/* Cofactor wheel modulo 210: its 48 residues prime to 210, ascending. */
-constexpr u32 kWheelMod = 210;
-constexpr u32 kWheelRes = 48;
+/* Cofactor wheel modulo 30030: its 5760 residues prime to 30030, ascending, the gaps
+   between them, and the rank structure that gives the index of the first residue >= r
+   (a bitmap of the residues and the count of those below each 32-bit word). */
+constexpr u32 kWheelMod = 30030;
+constexpr u32 kWheelRes = 5760;
+constexpr u32 kWheelWords = (kWheelMod + 31) / 32;
 
 struct WheelTables {
-    uint16_t res[kWheelRes];  /* the residues */
-    uint8_t gap[kWheelRes];   /* res[i + 1] - res[i], and 210 + res[0] - res[47] */
-    uint8_t next[kWheelMod];  /* index of the first residue >= r, 48 when none */
+    uint16_t res[kWheelRes];     /* the residues */
+    uint8_t gap[kWheelRes];      /* res[i + 1] - res[i], and 30030 + res[0] - res[5759] */
+    uint32_t bits[kWheelWords];  /* bit r % 32 of word r / 32: r is a residue */
+    uint16_t rank[kWheelWords];  /* residues below 32 w */
 };
-
-constexpr u32 wheel_gcd(u32 a, u32 b) { return b ? wheel_gcd(b, a % b) : a; }
 
 constexpr WheelTables make_wheel()
 {
     WheelTables t{};
     u32 n = 0;
     for (u32 r = 0; r < kWheelMod; ++r)
-        if (wheel_gcd(r, kWheelMod) == 1) t.res[n++] = (uint16_t)r;
+        if (r % 2 && r % 3 && r % 5 && r % 7 && r % 11 && r % 13) {
+            t.res[n++] = (uint16_t)r;
+            t.bits[r / 32] |= 1u << (r % 32);
+        }
     for (u32 i = 0; i < kWheelRes; ++i) {
         const u32 nx = i + 1 < kWheelRes ? t.res[i + 1] : kWheelMod + t.res[0];
         t.gap[i] = (uint8_t)(nx - t.res[i]);
     }
-    for (u32 r = 0; r < kWheelMod; ++r) {
-        u32 i = 0;
-        while (i < kWheelRes && t.res[i] < r) ++i;
-        t.next[r] = (uint8_t)i;
+    u32 below = 0;
+    for (u32 w = 0; w < kWheelWords; ++w) {
+        t.rank[w] = (uint16_t)below;
+        for (u32 b = 0; b < 32; ++b) below += (t.bits[w] >> b) & 1u;
     }
     return t;
 }
 
-/* Largest multiple of the wheel's gaps: 2^gap < 2^11 keeps x * 2^gap below 2^43. */
-constexpr u32 kWheelMaxGap = 10;
+/* The wheel's largest gap: 2^22 x < 2^54 keeps every x * 2^gap in mulpow2_u32's range. */
+constexpr u32 kWheelMaxGap = 22;
+
+/* Index of the first residue >= r (r < 30030), kWheelRes when there is none. */
+WW_HD u32 wheel_index(const uint32_t *bits, const uint16_t *rank, u32 r)
+{
+    return rank[r >> 5] + (u32)__builtin_popcount(bits[r >> 5] & ((1u << (r & 31)) - 1u));
+}
 
 /* x = q d + r for x < 2^64 and 2^20 <= d < 2^32, with dinv = 1/d rounded to double (or
    inv_below(d)). The double quotient is within 2^-49 of x/d relatively, and x/d < 2^44, so
@@ -140,8 +151,8 @@ WW_HD u32 pow2_mod_u32(u32 e, u32 l, double linv)
 }
 
 /* Where the bucket sieve starts for the sieving prime l (2^20 < l < 2^32) over the odd
-   numbers G0 + 2b (G0 odd): the first multiple n = l k >= G0 with k prime to 210, as its
-   bit b (the true difference n - G0 is below 11 l, so mod-2^64 arithmetic gives it even
+   numbers G0 + 2b (G0 odd): the first multiple n = l k >= G0 with k prime to 30030, as its
+   bit b (the true difference n - G0 is below 22 l, so mod-2^64 arithmetic gives it even
    where n passes 2^64), the wheel index of k, and x = 2^(n-1) mod l. */
 struct BucketStart {
     u64 b;
@@ -150,7 +161,7 @@ struct BucketStart {
 };
 
 WW_HD BucketStart bucket_start(u64 G0, u32 l, double linv, const uint16_t *res,
-                               const uint8_t *next)
+                               const uint32_t *bits, const uint16_t *rank)
 {
     /* linv = inv_below(l) */
     u64 q;
@@ -158,7 +169,7 @@ WW_HD BucketStart bucket_start(u64 G0, u32 l, double linv, const uint16_t *res,
     divmod_u64_u32(G0, l, linv, &q, &r);
     const u64 kmin = q + (r != 0u ? 1u : 0u); /* ceil(G0 / l), below 2^44 + 1 */
     u64 kq = kmin / kWheelMod;
-    u32 idx = next[(u32)(kmin - kq * kWheelMod)];
+    u32 idx = wheel_index(bits, rank, (u32)(kmin - kq * kWheelMod));
     if (idx == kWheelRes) {
         idx = 0;
         ++kq;

@@ -145,15 +145,18 @@ __global__ __launch_bounds__(kFillThreads) void k_bucket_fill(const BatchSpan *s
     __shared__ u32 cur[kBucketMaxBands];
     __shared__ uint16_t res[kWheelRes];
     __shared__ uint8_t gap[kWheelRes];
-    __shared__ uint8_t nxt[kWheelMod];
+    __shared__ u32 wbits[kWheelWords];
+    __shared__ uint16_t wrank[kWheelWords];
     {
-        constexpr WheelTables wt = make_wheel();
-        const u32 t = threadIdx.x;
-        if (t < kWheelRes) {
+        static constexpr WheelTables wt = make_wheel();
+        for (u32 t = threadIdx.x; t < kWheelRes; t += kFillThreads) {
             res[t] = wt.res[t];
             gap[t] = wt.gap[t];
         }
-        if (t < kWheelMod) nxt[t] = wt.next[t];
+        for (u32 t = threadIdx.x; t < kWheelWords; t += kFillThreads) {
+            wbits[t] = wt.bits[t];
+            wrank[t] = wt.rank[t];
+        }
     }
     const u32 groups = gridDim.x, wg = blockIdx.x;
     const u32 lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -177,7 +180,7 @@ __global__ __launch_bounds__(kFillThreads) void k_bucket_fill(const BatchSpan *s
             const u32 l = primes[i];
             if (l > bs.top) continue;
             const double linv = inv_below(l);
-            BucketStart s = bucket_start(G0, l, linv, res, nxt);
+            BucketStart s = bucket_start(G0, l, linv, res, wbits, wrank);
             /* walk in global odd-number indices; band buckets of this workgroup at
                wbase + band * bstride */
             u64 B = bs.bit0 + s.b;

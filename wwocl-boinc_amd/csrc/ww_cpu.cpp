@@ -152,7 +152,7 @@ int ww_cpu_bucket_hits(uint64_t g0, uint64_t nbits, uint32_t ell, uint64_t *bits
         return ww_set_error(WW_E_ARG, "g0 must be odd and ell an odd prime above 2^20");
     static constexpr WheelTables wt = make_wheel();
     const double linv = inv_below(ell);
-    BucketStart s = bucket_start(g0, ell, linv, wt.res, wt.next);
+    BucketStart s = bucket_start(g0, ell, linv, wt.res, wt.bits, wt.rank);
     uint64_t n = 0;
     for (u64 b = s.b; b < nbits; ++n) {
         if (n < cap) {
