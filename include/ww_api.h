@@ -165,6 +165,10 @@ int ww_set_bucket_capacity(ww_ctx *ctx, double scale);
    *total. Verification; blocking. */
 int ww_prime_table(ww_ctx *ctx, uint64_t first, uint64_t n, uint32_t *out, uint64_t *total);
 
+/* The same entries of the context's order table: ord_l(2) of each table prime l above 2^20
+   (the class test of the bucket sieve), 0 for the smaller ones. Verification; blocking. */
+int ww_prime_orders(ww_ctx *ctx, uint64_t first, uint64_t n, uint32_t *out, uint64_t *total);
+
 /* The bucket sieve's walk of one sieving prime, on the host with the code the device runs
    (verification): the multiples n = ell*k in [g0, g0 + 2 nbits) with k prime to 30030 (g0
    odd, 2^20 < ell < 2^32 prime), as bit indices (n - g0) / 2 with a 1 in rescue[] where
@@ -172,6 +176,10 @@ int ww_prime_table(ww_ctx *ctx, uint64_t first, uint64_t n, uint32_t *out, uint6
    *count = the number of hits. */
 int ww_cpu_bucket_hits(uint64_t g0, uint64_t nbits, uint32_t ell, uint64_t *bits,
                        uint8_t *rescue, uint64_t cap, uint64_t *count);
+
+/* ord_ell(2) for an odd prime 2^20 < ell < 2^32, on the host with the code the order table's
+   kernel runs (verification). */
+int ww_cpu_order2(uint32_t ell, uint32_t *order);
 
 /* Launch shape of the persistent kernel for launches issued afterwards: waves per
    workgroup (2, 4 or 8) and the wide pre-sieve's segment, the LDS bitmap one workgroup

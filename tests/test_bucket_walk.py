@@ -5,7 +5,9 @@ visit exactly the multiples n = l*k with k prime to 30030 (in order, as bit indi
 pseudoprime class. The expected lists come from Python's exact integers: every multiple of
 l in the window, a gcd with 30030, and pow(2, n - 1, l). Primes of small multiplicative order
 (factors of 2^31 - 1, 2^37 - 1, 2^41 - 1, ...) make the class frequent, windows near 2^64
-make the multiples pass 2^64. No device needed."""
+make the multiples pass 2^64. The walk's class test uses the prime's base-2 order, which
+ww_cpu_order2 computes with the order table's arithmetic; it is checked against the
+definition of the order. No device needed."""
 import math
 import random
 
@@ -72,8 +74,79 @@ def test_extreme_primes():
     assert len(got) > 1500  # 2^32 / l odd multiples, 5760 of each 15015 visited
 
 
+def prime_factors(n):
+    out, q = [], 2
+    while q * q <= n:
+        if n % q == 0:
+            out.append(q)
+            while n % q == 0:
+                n //= q
+        q += 1 if q == 2 else 2
+    if n > 1:
+        out.append(n)
+    return out
+
+
+def check_order(ell, o):
+    """o is ord_ell(2) by the definition: 2^o = 1, and 2^(o/q) != 1 for every prime q | o."""
+    assert (ell - 1) % o == 0 and pow(2, o, ell) == 1, (ell, o)
+    for q in prime_factors(o):
+        assert pow(2, o // q, ell) != 1, (ell, o, q)
+
+
+def test_order_small_and_structured():
+    # small orders, found by direct enumeration
+    for ell in SMALL_ORDER:
+        o = wwlib.cpu_order2(ell)
+        assert o == next(d for d in range(1, 65) if pow(2, d, ell) == 1)
+    # l - 1 with a large power of two (3 2^30 + 1, 15 2^27 + 1, 7 2^26 + 1, 5 2^25 + 1,
+    # 119 2^23 + 1), safe primes (l - 1 = 2 x prime), the extremes of the bucket tier
+    cases = [3221225473, 2013265921, 469762049, 167772161, 998244353,
+             4294967291, 1048583, 2147483647, 4294967279]
+    p = (1 << 31) + 3  # 11 mod 12, as every safe prime above 5
+    while len(cases) < 20:  # safe primes above 2^31
+        if oracle.is_prime(p) and oracle.is_prime((p - 1) // 2):
+            cases.append(p)
+        p += 12
+    for ell in cases:
+        assert oracle.is_prime(ell)
+        check_order(ell, wwlib.cpu_order2(ell))
+
+
+def test_order_repeated_factors():
+    """Primes whose order drops a factor 3 or 5 more than once (q^2 | (l - 1) / ord), so
+    the reduction must repeat; and primes where 2 is a quadratic residue (l = +-1 mod 8),
+    which halve the order's power of two."""
+    rng = random.Random(7)
+    found = {3: 0, 5: 0, 2: 0}
+    while min(found.values()) < 4:
+        ell = rng.randrange((1 << 20) + 1, 1 << 32) | 1
+        if not oracle.is_prime(ell):
+            continue
+        o = wwlib.cpu_order2(ell)
+        check_order(ell, o)
+        for q in (3, 5):
+            if (ell - 1) % (q * q) == 0 and ((ell - 1) // o) % (q * q) == 0:
+                found[q] += 1
+        if ell % 8 in (1, 7) and ((ell - 1) // o) % 4 == 0:
+            found[2] += 1
+
+
+def test_order_random():
+    rng = random.Random(9)
+    for _ in range(400):
+        ell = rng.randrange((1 << 20) + 1, 1 << 32) | 1
+        while not oracle.is_prime(ell):
+            ell += 2
+        if ell >= 1 << 32:
+            continue
+        check_order(ell, wwlib.cpu_order2(ell))
+
+
 def test_arguments():
     with pytest.raises(wwlib.WWError):
         wwlib.cpu_bucket_hits(2, 100, 2147483647)  # even start
     with pytest.raises(wwlib.WWError):
         wwlib.cpu_bucket_hits(1, 100, 1 << 19)  # below the bucket tier
+    with pytest.raises(wwlib.WWError):
+        wwlib.cpu_order2(1 << 19)

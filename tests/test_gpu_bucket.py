@@ -65,6 +65,36 @@ def test_prime_table(gctx):
         assert got == odd_primes(got[0], got[-1] + 1), at
 
 
+def test_order_table(gctx):
+    """The device's order table against the host's order arithmetic (itself checked against
+    the definition on the CPU): zeros up to 2^20, then slices at the tier's start, at random
+    places and at the end of the table."""
+    _, total = gctx.prime_table(0, 1)
+    first = 82025 - 1  # first table prime above 2^20
+    got, n = gctx.prime_orders(first - 5, 4005)
+    assert n == total and got[:5] == [0] * 5
+    ps, _ = gctx.prime_table(first, 4000)
+    assert ps[0] == 1048583
+    assert got[5:] == [wwlib.cpu_order2(l) for l in ps]
+    rng = random.Random(4)
+    for at in [rng.randrange(first, total - 3000) for _ in range(5)] + [total - 3000]:
+        ps, _ = gctx.prime_table(at, 3000)
+        got, _ = gctx.prime_orders(at, 3000)
+        assert got == [wwlib.cpu_order2(l) for l in ps], at
+    # primes of small order: 2^31 - 1 and 6700417 (a factor of 2^32 + 1), found in the
+    # table by bisection
+    for ell, order in ((2147483647, 31), (6700417, 64)):
+        a, b = 0, total
+        while b - a > 1:
+            mid = (a + b) // 2
+            if gctx.prime_table(mid, 1)[0][0] <= ell:
+                a = mid
+            else:
+                b = mid
+        assert gctx.prime_table(a, 1)[0] == [ell]
+        assert gctx.prime_orders(a, 1)[0] == [order]
+
+
 RANGES = [
     (10**12, 10**12 + (1 << 22)),                              # top 10^6: pre-sieve tier only
     (1 << 63, (1 << 63) + (1 << 23)),                          # sqrt ~ 3.04e9

@@ -72,10 +72,12 @@ struct ww_ctx {
     uint32_t presieve_chunk = ww::kPresieveMaxChunkSegs;
     ww_tuning last_tuning{}; /* the tuner's (or a loaded cache's) best, for ww_tuning_save */
     uint64_t default_chunk = 0;
-    /* bucket sieve: every odd prime below 2^32 (built on first use), the number of table
-       primes in each 2^21-integer block before it (for index bounds), and the smallest
-       range width that gets the bucket sieve in batches created from now on */
+    /* bucket sieve: every odd prime below 2^32 and the base-2 order of each one above 2^20
+       (built on first use), the number of table primes in each 2^21-integer block before
+       it (for index bounds), and the smallest range width that gets the bucket sieve in
+       batches created from now on */
     uint32_t *d_primes32 = nullptr;
+    uint32_t *d_orders32 = nullptr;
     uint64_t nprimes32 = 0;
     std::vector<uint64_t> primes32_prefix;
     uint64_t bucket_min_width = WW_DEFAULT_BUCKET_MIN_WIDTH;
@@ -213,13 +215,26 @@ int ensure_primes32(ww_ctx *c)
     if (e == hipSuccess)
         e = ww::launch_primes32(d_small, (uint32_t)small.size(), kBlocks, d_counts, d_off,
                                 c->d_primes32, c->stream);
+    /* the base-2 orders of the primes above 2^20 */
+    ww::SmallDiv *d_div = nullptr;
+    if (e == hipSuccess) e = hipMalloc(&c->d_orders32, off[kBlocks] * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc(&d_div, ww::kSmallOddPrimes * sizeof(ww::SmallDiv));
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(d_div, ww::small_div_table(), ww::kSmallOddPrimes * sizeof(ww::SmallDiv),
+                           hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess)
+        e = ww::launch_order2(c->d_primes32, off[kBlocks], d_div, ww::kSmallOddPrimes, c->d_orders32,
+                              c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (d_small) (void)hipFree(d_small);
     if (d_counts) (void)hipFree(d_counts);
     if (d_off) (void)hipFree(d_off);
+    if (d_div) (void)hipFree(d_div);
     if (e != hipSuccess) {
         if (c->d_primes32) (void)hipFree(c->d_primes32);
+        if (c->d_orders32) (void)hipFree(c->d_orders32);
         c->d_primes32 = nullptr;
+        c->d_orders32 = nullptr;
         return fail(WW_E_HIP, "prime table: %s", hipGetErrorString(e));
     }
     c->nprimes32 = off[kBlocks];
@@ -472,6 +487,7 @@ int ww_close(ww_ctx *c)
     if (c->d_deep) (void)hipFree(c->d_deep);
     if (c->d_wide) (void)hipFree(c->d_wide);
     if (c->d_primes32) (void)hipFree(c->d_primes32);
+    if (c->d_orders32) (void)hipFree(c->d_orders32);
     if (c->d_fstate) (void)hipFree(c->d_fstate);
     if (c->d_fsum) (void)hipFree(c->d_fsum);
     if (c->d_fres) (void)hipFree(c->d_fres);
@@ -573,7 +589,9 @@ int ww_set_bucket_capacity(ww_ctx *c, double scale)
     return WW_OK;
 }
 
-int ww_prime_table(ww_ctx *c, uint64_t first, uint64_t n, uint32_t *out, uint64_t *total)
+/* Entries [first, first + n) of the prime table (orders = false) or of the order table. */
+static int read_prime_table(ww_ctx *c, bool orders, uint64_t first, uint64_t n, uint32_t *out,
+                            uint64_t *total)
 {
     WW_LOCK(c);
     int rc = use(c);
@@ -585,11 +603,21 @@ int ww_prime_table(ww_ctx *c, uint64_t first, uint64_t n, uint32_t *out, uint64_
         return fail(WW_E_ARG, "entries [%llu, +%llu) beyond the table's %llu", (unsigned long long)first,
                     (unsigned long long)n, (unsigned long long)c->nprimes32);
     if (n) {
-        HIP_TRY(hipMemcpyAsync(out, c->d_primes32 + first, n * sizeof(uint32_t), hipMemcpyDeviceToHost,
-                               c->stream));
+        const uint32_t *src = (orders ? c->d_orders32 : c->d_primes32) + first;
+        HIP_TRY(hipMemcpyAsync(out, src, n * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
         HIP_TRY(hipStreamSynchronize(c->stream));
     }
     return WW_OK;
+}
+
+int ww_prime_table(ww_ctx *c, uint64_t first, uint64_t n, uint32_t *out, uint64_t *total)
+{
+    return read_prime_table(c, false, first, n, out, total);
+}
+
+int ww_prime_orders(ww_ctx *c, uint64_t first, uint64_t n, uint32_t *out, uint64_t *total)
+{
+    return read_prime_table(c, true, first, n, out, total);
 }
 
 int ww_set_launch_shape(ww_ctx *c, uint32_t waves_per_group, uint32_t segment_bits)
@@ -991,7 +1019,7 @@ int batch_enqueue_presieve(ww_ctx *c, ww_batch *b, bool buckets)
     }
     const ww::BucketStore st{b->d_band_ent, b->d_band_cnt, b->band_cap, b->d_bent, b->d_bcnt, b->bcap};
     HIP_TRY(ww::launch_bucket_sieve(b->d_spans, b->d_prefix, b->d_bspans, (uint32_t)b->bspans.size(),
-                                    b->nbands, c->d_primes32, st, c->stream));
+                                    b->nbands, c->d_primes32, c->d_orders32, st, c->stream));
     ww::BucketArgs bk{b->d_bspans, b->d_bent, b->d_bcnt, b->bcap, b->d_bover};
     HIP_TRY(ww::launch_presieve(b->d_spans, b->d_prefix, b->n, c->d_wide + first, count, b->d_pre,
                                 pgrid > 0 ? pgrid : 1, c->stream, c->presieve_chunk, b->d_prer, &bk));

@@ -139,7 +139,15 @@ struct BucketStore {
 };
 hipError_t launch_bucket_sieve(const BatchSpan *spans, const uint64_t *prefix,
                                const BucketSpan *bspans, uint32_t nb, uint32_t nbands,
-                               const uint32_t *primes, const BucketStore &st, hipStream_t s);
+                               const uint32_t *primes, const uint32_t *orders,
+                               const BucketStore &st, hipStream_t s);
+
+/* orders[i] = ord_l(2) for every table prime l = primes[i] above 2^20, 0 for the others
+   (i < n); small = the odd primes below 2^16 with their divisibility data (ww_bucket.h's
+   SmallDiv, nsmall of them). */
+struct SmallDiv;
+hipError_t launch_order2(const uint32_t *primes, uint64_t n, const SmallDiv *small,
+                         uint32_t nsmall, uint32_t *orders, hipStream_t s);
 
 /* Writes the one-span table of the range [lo, hi) (segments from lo | 1) into span and
    prefix (device scratch), so that a single range needs no host copy. */

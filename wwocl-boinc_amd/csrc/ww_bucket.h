@@ -10,11 +10,12 @@
  * n = l*k are visited only for k prime to 30030 = 2*3*5*7*11*13: n is odd and prime to
  * 3..13 exactly when k is, and any other n is no candidate (every candidate is odd and
  * prime to 3..113). That skips 9255 of every 15015 odd multiples. Each visited n also
- * gets its base-2 pseudoprime test modulo l: x = 2^(n-1) mod l. A composite n = l*k that is a base-2
- * Fermat pseudoprime (every strong one is) has x = 1, i.e. ord_l(2) | n - 1; such an n is
- * not dropped but flagged for the strong PRP test. By Fermat, 2^(n-1) = 2^(lk-1) =
- * (2^(l-1))^k 2^(k-1) = 2^(k-1) (mod l), so x starts as 2^(k0-1) mod l and each wheel
- * step k -> k + g multiplies it by 2^g.
+ * gets its base-2 pseudoprime test modulo l. A composite n = l*k that is a base-2 Fermat
+ * pseudoprime (every strong one is) has 2^(n-1) = 1 (mod l); such an n is not dropped but
+ * flagged for the strong PRP test. By Fermat, 2^(n-1) = 2^(lk-1) = (2^(l-1))^k 2^(k-1) =
+ * 2^(k-1) (mod l), so the test is ord_l(2) | k - 1: the walk carries c = (k - 1) mod ord_l(2)
+ * and adds each wheel gap to it, and the orders of all sieving primes come from a table
+ * built once per context (order2_from_factors, from the factors of l - 1).
  */
 #ifndef WW_BUCKET_H
 #define WW_BUCKET_H
@@ -58,7 +59,7 @@ constexpr WheelTables make_wheel()
     return t;
 }
 
-/* The wheel's largest gap: 2^22 x < 2^54 keeps every x * 2^gap in mulpow2_u32's range. */
+/* The wheel's largest gap. */
 constexpr u32 kWheelMaxGap = 22;
 
 /* Index of the first residue >= r (r < 30030), kWheelRes when there is none. */
@@ -67,9 +68,9 @@ WW_HD u32 wheel_index(const uint32_t *bits, const uint16_t *rank, u32 r)
     return rank[r >> 5] + (u32)__builtin_popcount(bits[r >> 5] & ((1u << (r & 31)) - 1u));
 }
 
-/* x = q d + r for x < 2^64 and 2^20 <= d < 2^32, with dinv = 1/d rounded to double (or
-   inv_below(d)). The double quotient is within 2^-49 of x/d relatively, and x/d < 2^44, so
-   it is off by less than 2^-5: truncated it is the true quotient or one away from it, and
+/* x = q d + r for x < 2^64 and d < 2^32 with x/d < 2^44 (any x when d >= 2^20), with
+   dinv = 1/d rounded to double (or inv_below(d)). The double quotient is within 2^-49 of
+   x/d relatively, and x/d < 2^44, so it is off by less than 2^-5: truncated it is the true quotient or one away from it, and
    the signed remainder x - q d (exact mod 2^64, its true value lies in (-d, 2d)) decides
    the one correction. */
 WW_HD void divmod_u64_u32(u64 x, u32 d, double dinv, u64 *q, u32 *r)
@@ -132,7 +133,7 @@ WW_HD u32 mont_mul_u32(u32 a, u32 b, u32 l, u32 linv)
     return (u < s || u >= l) ? u - l : u;
 }
 
-/* 2^e mod l for e < 2^32 and 2^20 <= l < 2^32 prime (linv = inv_below(l)): Montgomery
+/* 2^e mod l for e < 2^32 and 2^20 <= l < 2^32 odd (linv = inv_below(l)): Montgomery
    squarings over 4-bit windows of e from the top, each window's 2^w entered by one exact
    multiplication. */
 WW_HD u32 pow2_mod_u32(u32 e, u32 l, double linv)
@@ -150,17 +151,85 @@ WW_HD u32 pow2_mod_u32(u32 e, u32 l, double linv)
     return mont_mul_u32(x, 1u, l, li);
 }
 
-/* Where the bucket sieve starts for the sieving prime l (2^20 < l < 2^32) over the odd
-   numbers G0 + 2b (G0 odd): the first multiple n = l k >= G0 with k prime to 30030, as its
-   bit b (the true difference n - G0 is below 22 l, so mod-2^64 arithmetic gives it even
-   where n passes 2^64), the wheel index of k, and x = 2^(n-1) mod l. */
+/* The odd primes below 2^16 with the data of an exact divisibility test: for odd q and any
+   32-bit m, q | m iff m qinv mod 2^32 <= qlim, where qinv = q^-1 mod 2^32 and
+   qlim = floor((2^32 - 1) / q) (multiplication by qinv permutes the residues, and maps the
+   multiples m = q j onto j <= qlim); then m / q = m qinv mod 2^32. */
+struct SmallDiv {
+    u32 q, qinv, qlim, pad;
+};
+constexpr u32 kSmallOddPrimes = 6541; /* odd primes below 2^16 */
+
+WW_HD bool small_divides(u32 m, const SmallDiv &t) { return m * t.qinv <= t.qlim; }
+
+/* At most 9 distinct odd primes below 2^16 divide a 32-bit number (3 x 5 x ... x 29 is
+   already above 2^31), and at most one prime above 2^16: ten factor slots. */
+constexpr u32 kOrderFactors = 10;
+
+/* ord_l(2) for a prime 2^20 < l < 2^32, from the distinct odd prime factors f[0..nf) of
+   l - 1 (linv = inv_below(l)). Starting from o = l - 1, a multiple of the order, each odd
+   prime q is divided out of o while 2^(o/q) = 1: since the order divides o, that holds
+   exactly while q divides o more often than the order, so q ends with the order's exponent,
+   whatever the order of the primes. That leaves o = 2^s x (odd part of the order); the
+   order's power of two is the number of squarings that take 2^(odd part) to 1. The slots
+   are visited in an unrolled loop, so the factors stay in registers on the device. */
+WW_HD u32 order2_from_factors(u32 l, double linv, const u32 *f, u32 nf)
+{
+    u32 o = l - 1;
+#pragma unroll
+    for (u32 j = 0; j < kOrderFactors; ++j) {
+        if (j >= nf) continue;
+        const u32 q = f[j];
+        while (o % q == 0 && pow2_mod_u32(o / q, l, linv) == 1u) o /= q;
+    }
+    const u32 s = (u32)__builtin_ctz(l - 1);
+    const u32 odd = o >> s;
+    u32 y = pow2_mod_u32(odd, l, linv), t = 0;
+    while (y != 1u) { /* at most s squarings: 2^(l-1) = 1 */
+        u64 qq;
+        divmod_u64_u32((u64)y * y, l, linv, &qq, &y);
+        ++t;
+    }
+    return odd << t;
+}
+
+/* The distinct odd prime factors of l - 1 (l odd, 2^20 < l < 2^32) by trial division with
+   the odd primes below 2^16 (tab[0..ntab)), stopping once q^2 exceeds the unfactored part
+   m, which is then 1 or a prime (also when the table runs out: every composite below 2^32
+   has a factor below 2^16). Returns the number of factors written to f. */
+WW_HD u32 odd_factors_u32(u32 l, const SmallDiv *tab, u32 ntab, u32 *f)
+{
+    u32 m = (l - 1) >> __builtin_ctz(l - 1), nf = 0;
+    for (u32 j = 0; j < ntab && tab[j].q * tab[j].q <= m; ++j) {
+        if (!small_divides(m, tab[j])) continue;
+        f[nf++] = tab[j].q;
+        do m *= tab[j].qinv; while (small_divides(m, tab[j]));
+    }
+    if (m > 1u) f[nf++] = m;
+    return nf;
+}
+
+/* ord_l(2) for a prime 2^20 < l < 2^32 (the host's way; the table kernel runs the same
+   factor search as one wave-wide loop). Every such order is at least 22: 2^ord > l > 2^20
+   needs ord >= 21, and 2^21 - 1 = 7^2 x 127 x 337 has no prime factor above 2^20. */
+WW_HD u32 order2_u32(u32 l, const SmallDiv *tab, u32 ntab)
+{
+    u32 f[kOrderFactors];
+    const u32 nf = odd_factors_u32(l, tab, ntab, f);
+    return order2_from_factors(l, inv_below(l), f, nf);
+}
+
+/* Where the bucket sieve starts for the sieving prime l (2^20 < l < 2^32, ord = ord_l(2))
+   over the odd numbers G0 + 2b (G0 odd): the first multiple n = l k >= G0 with k prime to
+   30030, as its bit b (the true difference n - G0 is below 22 l, so mod-2^64 arithmetic
+   gives it even where n passes 2^64), the wheel index of k, and c = (k - 1) mod ord. */
 struct BucketStart {
     u64 b;
     u32 idx;
-    u32 x;
+    u32 c;
 };
 
-WW_HD BucketStart bucket_start(u64 G0, u32 l, double linv, const uint16_t *res,
+WW_HD BucketStart bucket_start(u64 G0, u32 l, double linv, u32 ord, const uint16_t *res,
                                const uint32_t *bits, const uint16_t *rank)
 {
     /* linv = inv_below(l) */
@@ -178,27 +247,30 @@ WW_HD BucketStart bucket_start(u64 G0, u32 l, double linv, const uint16_t *res,
     BucketStart s;
     s.b = ((u64)l * k - G0) >> 1;
     s.idx = idx;
-    /* x = 2^(k-1) mod l, the exponent reduced mod l - 1 */
+    /* k - 1 < 2^44 + 2^5 and ord >= 22: the quotient is below 2^44 */
     u64 q2;
-    u32 e;
-    divmod_u64_u32(k - 1, l - 1, inv_below(l - 1), &q2, &e);
-    s.x = pow2_mod_u32(e, l, linv);
+    divmod_u64_u32(k - 1, ord, inv_below(ord), &q2, &s.c);
     return s;
 }
 
-/* The next visited multiple: k -> k + gap, b -> b + gap l / 2, x -> x 2^gap. */
-WW_HD void bucket_step(u64 &b, u32 &idx, u32 &x, u32 l, double linv, const uint8_t *gap)
+/* The next visited multiple: k -> k + gap, b -> b + gap l / 2, c -> (c + gap) mod ord, one
+   subtraction since gap <= 22 <= ord. */
+WW_HD void bucket_step(u64 &b, u32 &idx, u32 &c, u32 l, u32 ord, const uint8_t *gap)
 {
     const u32 g = gap[idx];
     b += (u64)(g >> 1) * l;
-    x = mulpow2_u32(x, g, l, linv);
+    c += g;
+    c = c >= ord ? c - ord : c;
     idx = idx + 1 == kWheelRes ? 0u : idx + 1;
 }
 
 /* Bucket entries: the odd-number index of the hit in its 2^20-bit chunk, and the top bit
-   set when the hit is in the prime's pseudoprime class (x = 1). */
+   set when the hit is in the prime's pseudoprime class (c = 0). */
 constexpr u32 kBucketChunkBits = 20;
 constexpr u32 kBucketRescue = 0x80000000u;
+
+/* The kSmallOddPrimes entries of SmallDiv, built once on the host (ww_cpu.cpp). */
+const SmallDiv *small_div_table();
 
 } // namespace ww
 

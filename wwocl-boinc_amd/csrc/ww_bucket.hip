@@ -13,15 +13,17 @@
  * Primes above 2^20 hit a 2^20-bit chunk at most once, so computing every prime's start
  * per chunk (as k_presieve does for the smaller ones) would be wasted. Instead k_bucket_fill
  * walks each prime once over the whole range and drops its hits into buckets, one per
- * (chunk, workgroup) pair, which k_presieve then applies to the chunk's LDS bitmap. The
+ * (band of 16 chunks, workgroup) pair; k_bucket_split sorts each band's entries into
+ * per-chunk buckets, which k_presieve then applies to the chunk's LDS bitmap. The fill's
  * waves take blocks of 64 consecutive primes round-robin over the workgroups, so every
  * workgroup gets the same mix of small primes (thousands of hits each) and large ones (a
- * handful), and the bucket sizes stay close to their mean. Each workgroup counts its hits per chunk in LDS,
- * which also hands out the slots of its own bucket, so no global atomics are needed and the
- * bucket of (chunk, workgroup) fills sequentially.
+ * handful), and the bucket sizes stay close to their mean. Each workgroup counts its hits
+ * per band in LDS, which also hands out the slots of its own bucket, so no global atomics
+ * are needed and the bucket of (band, workgroup) fills sequentially.
  *
  * The sieving primes (every odd prime below 2^32, 813 MB) are generated once per context on
- * the device by k_primes32, a segmented sieve by the primes below 2^16.
+ * the device by k_primes32, a segmented sieve by the primes below 2^16, and their base-2
+ * orders (another 813 MB) by k_order2.
  */
 #include "ww_batch.h"
 #include "ww_bucket.h"
@@ -140,7 +142,7 @@ __global__ __launch_bounds__(kFillThreads) void k_bucket_fill(const BatchSpan *s
                                                               const u64 *prefix,
                                                               const BucketSpan *bspans,
                                                               u32 nb, const u32 *primes,
-                                                              BucketStore st)
+                                                              const u32 *orders, BucketStore st)
 {
     __shared__ u32 cur[kBucketMaxBands];
     __shared__ uint16_t res[kWheelRes];
@@ -172,20 +174,35 @@ __global__ __launch_bounds__(kFillThreads) void k_bucket_fill(const BatchSpan *s
         /* wave v of workgroup w takes the 64-prime blocks w + groups (v + 16 r), r = 0, 1,
            ...: neighbouring workgroups get neighbouring blocks, so their hit counts per
            band differ by about one block's, and each wave reads 256 contiguous bytes */
-        for (u64 blk = bs.pidx0 + 64 * ((u64)wg + (u64)groups * wave); blk < bs.pidx1;
-             blk += 64 * (u64)groups * (kFillThreads / 64)) {
-            if (primes[blk] > bs.top) break; /* ascending: the rest is above the top too */
+        const u64 stride = 64 * (u64)groups * (kFillThreads / 64);
+        u64 blk = bs.pidx0 + 64 * ((u64)wg + (u64)groups * wave);
+        /* the block's prime and order entries are loaded one block ahead, so their latency
+           hides behind the walk of the block before (lanes past the span's end read its
+           last entry and sit out) */
+        u32 lnext = 0, onext = 0;
+        if (blk < bs.pidx1) {
+            const u64 ic = blk + lane < bs.pidx1 ? blk + lane : bs.pidx1 - 1;
+            lnext = primes[ic];
+            onext = orders[ic];
+        }
+        for (; blk < bs.pidx1; blk += stride) {
             const u64 i = blk + lane;
-            if (i >= bs.pidx1) continue;
-            const u32 l = primes[i];
-            if (l > bs.top) continue;
-            const double linv = inv_below(l);
-            BucketStart s = bucket_start(G0, l, linv, res, wbits, wrank);
+            const u32 l = lnext, ord = onext;
+            /* ascending: once lane 0's prime (blk < pidx1) is above the top, so is the rest */
+            if ((u32)__builtin_amdgcn_readlane((int)l, 0) > bs.top) break;
+            if (blk + stride < bs.pidx1) {
+                const u64 in = blk + stride + lane;
+                const u64 ic = in < bs.pidx1 ? in : bs.pidx1 - 1;
+                lnext = primes[ic];
+                onext = orders[ic];
+            }
+            if (i >= bs.pidx1 || l > bs.top) continue;
+            BucketStart s = bucket_start(G0, l, inv_below(l), ord, res, wbits, wrank);
             /* walk in global odd-number indices; band buckets of this workgroup at
                wbase + band * bstride */
             u64 B = bs.bit0 + s.b;
             const u64 Bend = bs.bit0 + nbits;
-            u32 idx = s.idx, x = s.x;
+            u32 idx = s.idx, c = s.c;
             while (B < Bend) {
                 const u32 j = (u32)(B >> kBucketChunkBits) - bs.chunk_first;
                 const u32 band = j / kBucketBandChunks;
@@ -193,8 +210,8 @@ __global__ __launch_bounds__(kFillThreads) void k_bucket_fill(const BatchSpan *s
                 if (slot < st.band_cap)
                     st.band_entries[wbase + (u64)band * bstride + slot] =
                         ((u32)B & ((1u << kBucketChunkBits) - 1)) |
-                        ((j % kBucketBandChunks) << kBucketChunkBits) | (x == 1u ? kBucketRescue : 0u);
-                bucket_step(B, idx, x, l, linv, gap);
+                        ((j % kBucketBandChunks) << kBucketChunkBits) | (c == 0u ? kBucketRescue : 0u);
+                bucket_step(B, idx, c, l, ord, gap);
             }
         }
         __syncthreads();
@@ -309,7 +326,66 @@ __global__ __launch_bounds__(kFillThreads) void k_bucket_split(const BucketSpan 
             lost ? st.cap + 1 : cnt[threadIdx.x];
 }
 
+/* The order table: orders[i] = ord_l(2) for every table prime l = primes[i] above 2^20 (0
+   for the smaller ones), one lane per prime. The trial division of l - 1 by the odd primes
+   below 2^16 runs as one loop for the whole wave, so the divisors' entries are scalar
+   loads (eight at a time), and ends once q^2 exceeds every lane's unfactored part m. A lane that goes on past
+   its own bound cannot find a new factor: its m is 1 or a prime, which is recorded (and
+   becomes 1) if the loop reaches it, as the host's search records it after its loop. The
+   factors go to register slots, and order2_from_factors (shared with the host) finishes. */
+constexpr int kOrderThreads = 256;
+
+__global__ __launch_bounds__(kOrderThreads) void k_order2(const u32 *primes, u64 n,
+                                                          const SmallDiv *__restrict__ tab,
+                                                          u32 ntab, u32 *orders)
+{
+    const u64 i = (u64)blockIdx.x * kOrderThreads + threadIdx.x;
+    const u32 l = i < n ? primes[i] : 0u;
+    const bool act = l > (1u << kBucketChunkBits);
+    u32 m = act ? (l - 1) >> __builtin_ctz(l - 1) : 1u;
+    u32 f[kOrderFactors];
+    u32 nf = 0;
+#pragma unroll
+    for (u32 k = 0; k < kOrderFactors; ++k) f[k] = 0;
+    /* groups of 8 divisors, loaded together; the exit test takes the group's first */
+    constexpr u32 kGroup = 8;
+    for (u32 j0 = 0; j0 < ntab; j0 += kGroup) {
+        if (__ballot(tab[j0].q * tab[j0].q <= m) == 0) break;
+        SmallDiv g[kGroup];
+#pragma unroll
+        for (u32 u = 0; u < kGroup; ++u) g[u] = tab[j0 + u < ntab ? j0 + u : ntab - 1];
+#pragma unroll
+        for (u32 u = 0; u < kGroup; ++u) {
+            const SmallDiv t = g[u];
+            if (j0 + u < ntab && small_divides(m, t)) {
+#pragma unroll
+                for (u32 k = 0; k < kOrderFactors; ++k)
+                    if (k == nf) f[k] = t.q;
+                ++nf;
+                do m *= t.qinv; while (small_divides(m, t));
+            }
+        }
+    }
+    if (m > 1u) {
+#pragma unroll
+        for (u32 k = 0; k < kOrderFactors; ++k)
+            if (k == nf) f[k] = m;
+        ++nf;
+    }
+    if (i < n) orders[i] = act ? order2_from_factors(l, inv_below(l), f, nf) : 0u;
+}
+
 } // namespace
+
+hipError_t launch_order2(const u32 *primes, u64 n, const SmallDiv *small, u32 nsmall,
+                         u32 *orders, hipStream_t s)
+{
+    const u64 blocks = (n + kOrderThreads - 1) / kOrderThreads;
+    if (blocks == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_order2, dim3((u32)blocks), dim3(kOrderThreads), 0, s, primes, n, small,
+                       nsmall, orders);
+    return hipGetLastError();
+}
 
 hipError_t launch_primes32(const u32 *small, u32 nsmall, u32 nblocks, u32 *counts,
                            const u64 *offsets, u32 *out, hipStream_t s)
@@ -320,11 +396,11 @@ hipError_t launch_primes32(const u32 *small, u32 nsmall, u32 nblocks, u32 *count
 }
 
 hipError_t launch_bucket_sieve(const BatchSpan *spans, const u64 *prefix, const BucketSpan *bspans,
-                               u32 nb, u32 nbands, const u32 *primes, const BucketStore &st,
-                               hipStream_t s)
+                               u32 nb, u32 nbands, const u32 *primes, const u32 *orders,
+                               const BucketStore &st, hipStream_t s)
 {
     hipLaunchKernelGGL(k_bucket_fill, dim3(kBucketGroups), dim3(kFillThreads), 0, s, spans, prefix,
-                       bspans, nb, primes, st);
+                       bspans, nb, primes, orders, st);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || nbands == 0) return e;
     hipLaunchKernelGGL(k_bucket_split, dim3(nbands * kBucketSplitParts), dim3(kFillThreads), 0, s,

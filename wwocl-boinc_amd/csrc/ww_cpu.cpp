@@ -5,7 +5,8 @@
  *                                   with the bounds a wave would share as arguments
  *   ww_deep_table, ww_presieve_table  the deep-sieve and pre-sieve primes with their
  *                                   base-2 pseudoprime classes
- *   ww_cpu_bucket_hits              ww_bucket.h's walk of one bucket-sieve prime
+ *   ww_cpu_bucket_hits, ww_cpu_order2  ww_bucket.h's walk of one bucket-sieve prime and
+ *                                   its base-2 order
  * The drop-in's validation and self-test (isPrime, isPRP, the CPU range loop) are in
  * ww_selftest.cpp, in a formulation independent of this arithmetic.
  */
@@ -28,6 +29,27 @@ static uint32_t powmod32(uint32_t b, uint32_t e, uint32_t m)
 }
 
 int ww_set_error(int code, const char *fmt, ...); /* ww_api.cpp */
+
+namespace ww {
+
+const SmallDiv *small_div_table()
+{
+    static const std::vector<SmallDiv> tab = [] {
+        std::vector<SmallDiv> t;
+        std::vector<uint8_t> comp(1u << 16, 0);
+        for (uint32_t q = 3; q < (1u << 16); q += 2) {
+            if (comp[q]) continue;
+            for (uint32_t j = q * q; j < (1u << 16); j += 2 * q) comp[j] = 1;
+            uint32_t inv = q; /* q q = 1 (mod 8); each Newton step doubles the good bits */
+            for (int k = 0; k < 4; ++k) inv *= 2u - q * inv;
+            t.push_back({q, inv, 0xffffffffu / q, 0});
+        }
+        return t;
+    }();
+    return tab.data();
+}
+
+} // namespace ww
 
 extern "C" {
 
@@ -151,17 +173,26 @@ int ww_cpu_bucket_hits(uint64_t g0, uint64_t nbits, uint32_t ell, uint64_t *bits
     if ((g0 & 1) == 0 || ell <= (1u << 20) || (ell & 1) == 0)
         return ww_set_error(WW_E_ARG, "g0 must be odd and ell an odd prime above 2^20");
     static constexpr WheelTables wt = make_wheel();
-    const double linv = inv_below(ell);
-    BucketStart s = bucket_start(g0, ell, linv, wt.res, wt.bits, wt.rank);
+    const u32 ord = order2_u32(ell, small_div_table(), kSmallOddPrimes);
+    BucketStart s = bucket_start(g0, ell, inv_below(ell), ord, wt.res, wt.bits, wt.rank);
     uint64_t n = 0;
     for (u64 b = s.b; b < nbits; ++n) {
         if (n < cap) {
             bits[n] = b;
-            rescue[n] = s.x == 1u;
+            rescue[n] = s.c == 0u;
         }
-        bucket_step(b, s.idx, s.x, ell, linv, wt.gap);
+        bucket_step(b, s.idx, s.c, ell, ord, wt.gap);
     }
     *count = n;
+    return WW_OK;
+}
+
+int ww_cpu_order2(uint32_t ell, uint32_t *order)
+{
+    if (!order) return ww_set_error(WW_E_ARG, "null output");
+    if (ell <= (1u << 20) || (ell & 1) == 0)
+        return ww_set_error(WW_E_ARG, "ell must be an odd prime above 2^20");
+    *order = order2_u32(ell, small_div_table(), kSmallOddPrimes);
     return WW_OK;
 }
 

@@ -198,9 +198,12 @@ _SIGNATURES = [
     ("ww_set_bucket_capacity", ctypes.c_int, [ctypes.c_void_p, ctypes.c_double]),
     ("ww_prime_table", ctypes.c_int, [ctypes.c_void_p, U64, U64, ctypes.POINTER(U32),
                                       ctypes.POINTER(U64)]),
+    ("ww_prime_orders", ctypes.c_int, [ctypes.c_void_p, U64, U64, ctypes.POINTER(U32),
+                                       ctypes.POINTER(U64)]),
     ("ww_cpu_bucket_hits", ctypes.c_int, [U64, U64, U32, ctypes.POINTER(U64),
                                           ctypes.POINTER(ctypes.c_uint8), U64,
                                           ctypes.POINTER(U64)]),
+    ("ww_cpu_order2", ctypes.c_int, [U32, ctypes.POINTER(U32)]),
     ("ww_batch_buckets", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p,
                                         ctypes.POINTER(BucketInfo)]),
     ("ww_deep_table", U32, [U32, ctypes.POINTER(DeepPrime), U32]),
@@ -322,6 +325,13 @@ def cpu_bucket_hits(g0: int, nbits: int, ell: int, cap: int = 1 << 16):
     return [(bits[i], bool(resc[i])) for i in range(n.value)]
 
 
+def cpu_order2(ell: int) -> int:
+    """Host build of the order table's arithmetic: ord_ell(2) for a prime 2^20 < ell < 2^32."""
+    out = U32()
+    _check(load().ww_cpu_order2(ell, ctypes.byref(out)))
+    return out.value
+
+
 def cpu_range(low: int, high: int):
     out = (U64 * 3)()
     _check(load().ww_cpu_range(low, high, out))
@@ -389,6 +399,14 @@ class Context:
         out = (U32 * max(n, 1))()
         total = U64()
         _check(self.lib.ww_prime_table(self.h, first, n, out, ctypes.byref(total)))
+        return list(out[:n]), total.value
+
+    def prime_orders(self, first: int, n: int):
+        """(entries [first, first + n) of the device's order table, its length): ord_l(2)
+        of each table prime l above 2^20, 0 below."""
+        out = (U32 * max(n, 1))()
+        total = U64()
+        _check(self.lib.ww_prime_orders(self.h, first, n, out, ctypes.byref(total)))
         return list(out[:n]), total.value
 
     def set_deep_bound(self, bound: int):
