@@ -142,8 +142,13 @@ __global__ __launch_bounds__(kPreThreads) void k_presieve(const BatchSpan *spans
             if (outr && (!sieved || !bk.spans || sp.pad == 0))
                 for (u32 k = threadIdx.x; k < (u32)(e - s); k += kPreThreads)
                     all_prp[(u32)(s - s0) + k] = 1;
+            /* each prime's entry is loaded one iteration ahead, so the table's latency hides
+               behind the marks of the prime before */
+            WidePrime dn{};
+            if (threadIdx.x < nprimes) dn = tab[threadIdx.x];
             for (u32 j = threadIdx.x; j < nprimes; j += kPreThreads) {
-                const WidePrime d = tab[j];
+                const WidePrime d = dn;
+                if (j + kPreThreads < nprimes) dn = tab[j + kPreThreads];
                 const u64 ell = d.ell;
                 /* first odd multiple G + 2i of ell: -G mod ell, plus ell when odd */
                 const u64 r = barrett64(G, ell, d.bar_ell);
