@@ -72,8 +72,9 @@ constexpr uint32_t kBucketMaxChunks = 8200;
    bits make 256 bands, so the open lines of the workgroups of one XCD fit in its L2. */
 constexpr uint32_t kBucketBandChunks = 16;
 constexpr uint32_t kBucketMaxBands = (kBucketMaxChunks + kBucketBandChunks - 1) / kBucketBandChunks;
-/* Workgroups of the bucket fill (two per CU), and so band buckets per band. */
-constexpr uint32_t kBucketGroups = 512;
+/* Workgroups of the bucket fill (one per CU: half the open lines of two per CU, 2.93 ->
+   2.68 ms per 2^33 range), and so band buckets per band. */
+constexpr uint32_t kBucketGroups = 256;
 /* Split workgroups per band, each taking an equal share of the fill workgroups' buckets;
    and so buckets per chunk. */
 constexpr uint32_t kBucketSplitParts = 4;
