@@ -140,7 +140,8 @@ int ww_get_presieve_bound(ww_ctx *ctx, uint32_t *bound);
 #define WW_DEFAULT_BUCKET_MIN_WIDTH (1ull << 30)
 
 /* The bucket sieve of the persistent form, for ranges at least min_width wide in
-   batches created afterwards (UINT64_MAX: none). No counterpart in the reference. Such a
+   batches created afterwards and for fused chunks (ww_search_fused) at least that wide
+   (UINT64_MAX: none). No counterpart in the reference. Such a
    range is sieved by every prime up to sqrt(hi - 1): the deep sieve and the wide pre-sieve
    take the primes up to the pre-sieve bound, and one more kernel drops the hits of the
    primes above it, up to 2^32, into per-chunk buckets that the pre-sieve applies. Every
@@ -149,9 +150,10 @@ int ww_get_presieve_bound(ww_ctx *ctx, uint32_t *bound);
    as before. The PRP list, checksums and records are unchanged. It takes effect where the
    deep bound is at least 1024, the pre-sieve bound at least 2^20 and the pre-sieve segment
    2^20 bits (the defaults); elsewhere such a batch runs as without it. Costs per range
-   about 4.7 bytes of device memory per integer of width (4.5 GB for a 2^33-wide range) and,
-   on first use, an 813 MB table of the odd primes below 2^32 built on the device.
-   Default WW_DEFAULT_BUCKET_MIN_WIDTH. */
+   about 0.85 bytes of device memory per integer of width (7.3 GB for a 2^33-wide range;
+   fused chunks keep one such store per context, for the widest chunk so far) and, on
+   first use, the odd primes below 2^32 and their base-2 orders, 813 MB each, built on the
+   device in about 0.5 s. Default WW_DEFAULT_BUCKET_MIN_WIDTH. */
 int ww_set_bucket_sieve(ww_ctx *ctx, uint64_t min_width);
 int ww_get_bucket_sieve(ww_ctx *ctx, uint64_t *min_width);
 
@@ -218,8 +220,10 @@ int ww_search_range(ww_ctx *ctx, uint64_t low, uint64_t high, uint64_t chunk);
    added to the context's counters and results (what ww_read_counters and
    ww_read_results return), but no PRP list is written (its count stays as it is, so
    ww_read_prps and the tests kernel do not apply to such a chunk). Stream-ordered; its
-   time counts as test-kernel time in ww_kernel_times. No counterpart in the reference:
-   a faster body for its chunk loop (main.cpp:1683-1741) when the list is not needed. */
+   time counts as test-kernel time in ww_kernel_times. A chunk at least the bucket width
+   (ww_set_bucket_sieve) is sieved up to sqrt(high - 1) first, and its candidates that no
+   sieve tier flags skip the strong PRP test. No counterpart in the reference: a faster
+   body for its chunk loop (main.cpp:1683-1741) when the list is not needed. */
 int ww_search_fused(ww_ctx *ctx, uint64_t low, uint64_t high);
 
 /* Wait for all queued work (sclFinish / waitOnEvent, simpleCL.c, main.cpp:329-377). */

@@ -164,6 +164,33 @@ def test_strong_pseudoprimes_in_every_tier_are_kept(bctx):
         assert g == oracle.range_results(lo, hi), n
 
 
+@pytest.mark.parametrize("lo,hi", RANGES)
+def test_fused_chunks_with_buckets_vs_oracle(bctx, lo, hi):
+    """ww_search_fused (the drop-in's chunk body) with the bucket sieve: the same counters
+    and records as the oracle, in one chunk and in uneven ones."""
+    want = oracle.range_results(lo, hi)
+    got = bctx.run_range_fused(lo, hi)
+    assert (got.prp_count, got.checksum, got.results) == want, (lo, hi)
+    got = bctx.run_range_fused(lo, hi, chunk=(hi - lo) // 3 + 12345)
+    assert (got.prp_count, got.checksum, got.results) == want, (lo, hi)
+
+
+def test_fused_chunks_keep_strong_pseudoprimes(bctx):
+    cases = [n for _, n in twin_spsp2(1500, 1)] + [n for _, n in twin_spsp2(2100000, 1)]
+    cases += [n for _, n in twin_spsp2(2500000000, 1)] + chernick_spsp2()[:1]
+    for n in cases:
+        lo, hi = n - (1 << 21), n + (1 << 21)
+        got = bctx.run_range_fused(lo, hi)
+        assert (got.prp_count, got.checksum, got.results) == oracle.range_results(lo, hi), n
+
+
+def test_fused_overflowing_buckets_stay_exact(bctx):
+    bctx.set_bucket_capacity(0.1)
+    lo, hi = (1 << 63) + 1001, (1 << 63) + 1001 + (1 << 23)
+    got = bctx.run_range_fused(lo, hi)
+    assert (got.prp_count, got.checksum, got.results) == oracle.range_results(lo, hi)
+
+
 def test_mixed_batch_vs_oracle(bctx):
     """Bucket spans beside ranges too narrow for them, some sharing a pre-sieve chunk."""
     bctx.set_bucket_sieve(1 << 22)

@@ -64,6 +64,16 @@ struct ww_ctx {
     ww::BatchState *d_fstate = nullptr;
     SlotSum *d_fsum = nullptr;
     ww_result *d_fres = nullptr;
+    /* fused chunks with the bucket sieve: band and chunk buckets for up to fb_segs segments
+       (sized for sieving primes up to 2^32 at capacity scale fb_scale), the bucket span,
+       the overflow count and the needs-PRP words */
+    ww::BucketSpan *d_fbspan = nullptr;
+    uint32_t *d_fband_ent = nullptr, *d_fband_cnt = nullptr;
+    uint32_t *d_fbent = nullptr, *d_fbcnt = nullptr, *d_fbover = nullptr;
+    uint64_t *d_fprer = nullptr;
+    uint32_t fband_cap = 0, fbcap = 0;
+    uint64_t fb_segs = 0;
+    double fb_scale = 0;
     int tuned_blocks = 0;      /* defaults for batches created from now on */
     uint32_t tuned_claim = 8;
     /* launch shape for launches issued from now on: waves per workgroup of the persistent
@@ -278,20 +288,45 @@ uint64_t bucket_first_index(const ww_ctx *c)
 
 namespace {
 
+/* The device span of a single chunk, and its pre-sieve words for nseg segments (grown
+   when needed; queued launches may still read the old buffer, so the stream is drained
+   first). */
+int ensure_chunk_span(ww_ctx *c)
+{
+    if (c->d_span1) return WW_OK;
+    hipError_t e;
+    if ((e = hipMalloc(&c->d_prefix1, 2 * sizeof(uint64_t))) != hipSuccess ||
+        (e = hipMalloc(&c->d_span1, sizeof(ww::BatchSpan))) != hipSuccess) { /* span last */
+        if (c->d_prefix1) (void)hipFree(c->d_prefix1);
+        c->d_prefix1 = nullptr;
+        return fail(WW_E_HIP, "chunk span: %s", hipGetErrorString(e));
+    }
+    return WW_OK;
+}
+
+int ensure_pre_plain(ww_ctx *c, uint64_t nseg)
+{
+    if (nseg <= c->pre_plain_segs) return WW_OK;
+    hipError_t e;
+    if ((e = hipStreamSynchronize(c->stream)) != hipSuccess)
+        return fail(WW_E_HIP, "pre-sieve buffer: %s", hipGetErrorString(e));
+    if (c->d_pre_plain) (void)hipFree(c->d_pre_plain);
+    c->d_pre_plain = nullptr;
+    c->pre_plain_segs = 0;
+    if ((e = hipMalloc(&c->d_pre_plain, nseg * ww::kPresieveWordsPerSeg * sizeof(uint64_t))) != hipSuccess)
+        return fail(WW_E_HIP, "pre-sieve buffer: %s", hipGetErrorString(e));
+    c->pre_plain_segs = nseg;
+    return WW_OK;
+}
+
 /* The one-span table of a single chunk [low, high) on the device, and its wide pre-sieve
    when the pre-sieve is on and the chunk is long enough: *pre = the words, else null. */
 int chunk_presieve(ww_ctx *c, uint64_t low, uint64_t high, const uint64_t **pre)
 {
     *pre = nullptr;
     hipError_t e;
-    if (!c->d_span1) {
-        if ((e = hipMalloc(&c->d_prefix1, 2 * sizeof(uint64_t))) != hipSuccess ||
-            (e = hipMalloc(&c->d_span1, sizeof(ww::BatchSpan))) != hipSuccess) { /* span last */
-            if (c->d_prefix1) (void)hipFree(c->d_prefix1);
-            c->d_prefix1 = nullptr;
-            return fail(WW_E_HIP, "chunk span: %s", hipGetErrorString(e));
-        }
-    }
+    int rc = ensure_chunk_span(c);
+    if (rc) return rc;
     uint32_t wfirst, wcount;
     wide_span(c, &wfirst, &wcount);
     const uint64_t nseg = ww::segments_of(low, high);
@@ -300,18 +335,7 @@ int chunk_presieve(ww_ctx *c, uint64_t low, uint64_t high, const uint64_t **pre)
             return fail(WW_E_HIP, "chunk span: %s", hipGetErrorString(e));
         return WW_OK;
     }
-    if (nseg > c->pre_plain_segs) {
-        /* grow the chunk's pre-sieve buffer; queued launches may still read the old one */
-        if ((e = hipStreamSynchronize(c->stream)) != hipSuccess)
-            return fail(WW_E_HIP, "pre-sieve buffer: %s", hipGetErrorString(e));
-        if (c->d_pre_plain) (void)hipFree(c->d_pre_plain);
-        c->d_pre_plain = nullptr;
-        c->pre_plain_segs = 0;
-        if ((e = hipMalloc(&c->d_pre_plain, nseg * ww::kPresieveWordsPerSeg * sizeof(uint64_t))) !=
-            hipSuccess)
-            return fail(WW_E_HIP, "pre-sieve buffer: %s", hipGetErrorString(e));
-        c->pre_plain_segs = nseg;
-    }
+    if ((rc = ensure_pre_plain(c, nseg))) return rc;
     const int pgrid = (int)std::min<uint64_t>((nseg + c->presieve_chunk - 1) / c->presieve_chunk,
                                               (uint64_t)c->cus);
     if ((e = ww::launch_presieve_range(low, high, c->d_span1, c->d_prefix1, c->d_wide + wfirst, wcount,
@@ -319,6 +343,96 @@ int chunk_presieve(ww_ctx *c, uint64_t low, uint64_t high, const uint64_t **pre)
                                        c->presieve_chunk)) != hipSuccess)
         return fail(WW_E_HIP, "pre-sieve: %s", hipGetErrorString(e));
     *pre = c->d_pre_plain;
+    return WW_OK;
+}
+
+void free_fused_buckets(ww_ctx *c)
+{
+    for (uint32_t **p : {&c->d_fband_ent, &c->d_fband_cnt, &c->d_fbent, &c->d_fbcnt, &c->d_fbover}) {
+        if (*p) (void)hipFree(*p);
+        *p = nullptr;
+    }
+    if (c->d_fprer) (void)hipFree(c->d_fprer);
+    if (c->d_fbspan) (void)hipFree(c->d_fbspan);
+    c->d_fprer = nullptr;
+    c->d_fbspan = nullptr;
+    c->fb_segs = 0;
+}
+
+/* The bucket storage of fused chunks of up to nseg segments, sized as a batch's for a top
+   of 2^32 (so any chunk of that length fits) and the context's capacity scale. */
+int ensure_fused_buckets(ww_ctx *c, uint64_t nseg)
+{
+    if (nseg <= c->fb_segs && c->fb_scale == c->bucket_scale) return WW_OK;
+    hipError_t e = hipStreamSynchronize(c->stream); /* queued chunks may use the old ones */
+    if (e != hipSuccess) return fail(WW_E_HIP, "fused buckets: %s", hipGetErrorString(e));
+    free_fused_buckets(c);
+    const uint64_t chunks = ((nseg * 4096 - 1) >> ww::kBucketChunkBits) + 1;
+    const uint64_t bands = (chunks + ww::kBucketBandChunks - 1) / ww::kBucketBandChunks;
+    const double mean = std::ldexp(1.0, 20) * 5760.0 / 15015.0 * std::log(32.0 / 20.0);
+    const double sc = c->bucket_scale;
+    c->fbcap = (uint32_t)std::ceil(sc * (1.15 * mean / ww::kBucketSplitParts + 512));
+    c->fband_cap = (uint32_t)std::ceil(sc * (1.25 * mean * ww::kBucketBandChunks / ww::kBucketGroups + 256));
+    const size_t band_buckets = bands * ww::kBucketGroups, parts = chunks * ww::kBucketSplitParts;
+    e = hipMalloc(&c->d_fband_cnt, band_buckets * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc(&c->d_fband_ent, band_buckets * (size_t)c->fband_cap * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc(&c->d_fbcnt, parts * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc(&c->d_fbent, parts * (size_t)c->fbcap * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc(&c->d_fbover, sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMemsetAsync(c->d_fbover, 0, sizeof(uint32_t), c->stream);
+    if (e == hipSuccess) e = hipMalloc(&c->d_fprer, nseg * ww::kPresieveWordsPerSeg * sizeof(uint64_t));
+    if (e == hipSuccess) e = hipMalloc(&c->d_fbspan, sizeof(ww::BucketSpan)); /* last: all made */
+    if (e != hipSuccess) {
+        free_fused_buckets(c);
+        return fail(WW_E_HIP, "fused buckets: %s", hipGetErrorString(e));
+    }
+    c->fb_segs = nseg;
+    c->fb_scale = sc;
+    return WW_OK;
+}
+
+/* A fused chunk with the bucket sieve, when [low, high) is at least the bucket width, the
+   sieve tiers allow known primes and the chunk is pre-sieved: its span and bucket span,
+   the fill and split, and the pre-sieve with the needs-PRP words; *pre and *prer = the two
+   sets of words. Both stay null when the chunk does not qualify (the plain pre-sieve then
+   runs). */
+int chunk_buckets(ww_ctx *c, uint64_t low, uint64_t high, const uint64_t **pre,
+                  const uint64_t **prer)
+{
+    *pre = *prer = nullptr;
+    uint32_t wfirst, wcount;
+    wide_span(c, &wfirst, &wcount);
+    const uint64_t nseg = ww::segments_of(low, high);
+    if (!wcount || nseg < ww::kPresieveMinSegs || high - low < c->bucket_min_width || !bucket_config_ok(c))
+        return WW_OK;
+    const uint64_t chunks = ((nseg * 4096 - 1) >> ww::kBucketChunkBits) + 1;
+    if (chunks > ww::kBucketMaxChunks) return WW_OK;
+    int rc;
+    if ((rc = ensure_primes32(c)) || (rc = ensure_chunk_span(c)) || (rc = ensure_pre_plain(c, nseg)) ||
+        (rc = ensure_fused_buckets(c, nseg)))
+        return rc;
+    ww::BucketSpan bs{};
+    bs.span = 0;
+    bs.bit0 = 0;
+    bs.chunk_first = 0;
+    bs.nchunks = (uint32_t)chunks;
+    bs.nbands = (uint32_t)((chunks + ww::kBucketBandChunks - 1) / ww::kBucketBandChunks);
+    bs.top = (uint32_t)isqrt64(high - 1);
+    bs.pidx0 = bucket_first_index(c);
+    bs.pidx1 = c->primes32_prefix[std::min<uint64_t>(((uint64_t)bs.top >> 21) + 1,
+                                                     c->primes32_prefix.size() - 1)];
+    if (bs.pidx1 < bs.pidx0) bs.pidx1 = bs.pidx0;
+    HIP_TRY(ww::launch_set_bucket_span(low, high, c->d_span1, c->d_prefix1, c->d_fbspan, bs, c->stream));
+    const ww::BucketStore st{c->d_fband_ent, c->d_fband_cnt, c->fband_cap, c->d_fbent, c->d_fbcnt, c->fbcap};
+    HIP_TRY(ww::launch_bucket_sieve(c->d_span1, c->d_prefix1, c->d_fbspan, 1, bs.nbands, c->d_primes32,
+                                    c->d_orders32, st, c->stream));
+    const ww::BucketArgs bk{c->d_fbspan, c->d_fbent, c->d_fbcnt, c->fbcap, c->d_fbover};
+    const int pgrid = (int)std::min<uint64_t>((nseg + c->presieve_chunk - 1) / c->presieve_chunk,
+                                              (uint64_t)c->cus);
+    HIP_TRY(ww::launch_presieve(c->d_span1, c->d_prefix1, 1, c->d_wide + wfirst, wcount, c->d_pre_plain,
+                                pgrid > 0 ? pgrid : 1, c->stream, c->presieve_chunk, c->d_fprer, &bk));
+    *pre = c->d_pre_plain;
+    *prer = c->d_fprer;
     return WW_OK;
 }
 
@@ -492,6 +606,7 @@ int ww_close(ww_ctx *c)
     if (c->d_fsum) (void)hipFree(c->d_fsum);
     if (c->d_fres) (void)hipFree(c->d_fres);
     if (c->d_pre_plain) (void)hipFree(c->d_pre_plain);
+    free_fused_buckets(c);
     if (c->d_span1) (void)hipFree(c->d_span1);
     if (c->d_prefix1) (void)hipFree(c->d_prefix1);
     if (c->d_results) (void)hipFree(c->d_results);
@@ -746,13 +861,14 @@ int ww_search_fused(ww_ctx *c, uint64_t low, uint64_t high)
     hipEvent_t a, b;
     if ((rc = event_pair(c, c->ev_tests, &a, &b))) return rc;
     HIP_TRY(hipEventRecord(a, c->stream));
-    const uint64_t *pre = nullptr;
-    if ((rc = chunk_presieve(c, low, high, &pre))) return rc;
+    const uint64_t *pre = nullptr, *prer = nullptr;
+    if ((rc = chunk_buckets(c, low, high, &pre, &prer))) return rc;
+    if (!prer && (rc = chunk_presieve(c, low, high, &pre))) return rc;
     int nb = c->tuned_blocks > 0 ? c->tuned_blocks : ww::batch_blocks_per_cu(c->waves);
     if (nb <= 0) nb = 4;
     HIP_TRY(ww::launch_batch(c->d_span1, c->d_prefix1, 1, c->d_fstate, c->d_fsum, 1, c->d_fres,
                              c->result_cap, c->cus * nb, c->tuned_claim, c->d_deep, c->ndeep, pre,
-                             c->stream, c->waves));
+                             c->stream, c->waves, prer));
     HIP_TRY(ww::launch_fold(c->d_fstate, c->d_fsum, c->d_fres, c->result_cap, c->d_state,
                             c->d_results, c->result_cap, c->stream));
     HIP_TRY(hipEventRecord(b, c->stream));

@@ -155,6 +155,11 @@ hipError_t launch_order2(const uint32_t *primes, uint64_t n, const SmallDiv *sma
 hipError_t launch_set_span(uint64_t lo, uint64_t hi, BatchSpan *span, uint64_t *prefix,
                            hipStream_t s);
 
+/* launch_set_span for a fused chunk with the bucket sieve: the span's pad = 1, and bs
+   (bucket span 0 of that one-span table) written to *bspan. */
+hipError_t launch_set_bucket_span(uint64_t lo, uint64_t hi, BatchSpan *span, uint64_t *prefix,
+                                  BucketSpan *bspan, const BucketSpan &bs, hipStream_t s);
+
 /* launch_set_span, then launch_presieve over that one span (a plain-form chunk). */
 hipError_t launch_presieve_range(uint64_t lo, uint64_t hi, BatchSpan *span, uint64_t *prefix,
                                  const WidePrime *tab, uint32_t ntab, uint64_t *out, int grid,

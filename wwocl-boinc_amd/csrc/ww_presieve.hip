@@ -241,11 +241,34 @@ __global__ void k_set_span(BatchSpan *span, u64 *prefix, u64 lo, u64 hi)
     prefix[1] = segments_of(lo, hi);
 }
 
+/* The same for a fused chunk with the bucket sieve: the span refers to bucket span 0
+   (pad = 1), which is written beside it (chunk 0 at the span's first odd number). */
+__global__ void k_set_bucket_span(BatchSpan *span, u64 *prefix, BucketSpan *bspan, u64 lo,
+                                  u64 hi, BucketSpan bs)
+{
+    span->lo = lo;
+    span->hi = hi;
+    span->seg0 = 0;
+    span->slot = 0;
+    span->pad = 1;
+    span->pre_seg = 0;
+    prefix[0] = 0;
+    prefix[1] = segments_of(lo, hi);
+    *bspan = bs;
+}
+
 } // namespace
 
 hipError_t launch_set_span(u64 lo, u64 hi, BatchSpan *span, u64 *prefix, hipStream_t s)
 {
     hipLaunchKernelGGL(k_set_span, dim3(1), dim3(1), 0, s, span, prefix, lo, hi);
+    return hipGetLastError();
+}
+
+hipError_t launch_set_bucket_span(u64 lo, u64 hi, BatchSpan *span, u64 *prefix, BucketSpan *bspan,
+                                  const BucketSpan &bs, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_set_bucket_span, dim3(1), dim3(1), 0, s, span, prefix, bspan, lo, hi, bs);
     return hipGetLastError();
 }
 
