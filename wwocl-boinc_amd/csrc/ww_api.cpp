@@ -360,9 +360,12 @@ void free_fused_buckets(ww_ctx *c)
 }
 
 /* The bucket storage of fused chunks of up to nseg segments, sized as a batch's for a top
-   of 2^32 (so any chunk of that length fits) and the context's capacity scale. */
-int ensure_fused_buckets(ww_ctx *c, uint64_t nseg)
+   of 2^32 (so any chunk of that length fits) and the context's capacity scale. When the
+   device memory is not there, *avail = false and the chunk runs without it (the failed
+   allocation's error is cleared, so later launch checks do not see it). */
+int ensure_fused_buckets(ww_ctx *c, uint64_t nseg, bool *avail)
 {
+    *avail = true;
     if (nseg <= c->fb_segs && c->fb_scale == c->bucket_scale) return WW_OK;
     hipError_t e = hipStreamSynchronize(c->stream); /* queued chunks may use the old ones */
     if (e != hipSuccess) return fail(WW_E_HIP, "fused buckets: %s", hipGetErrorString(e));
@@ -382,6 +385,12 @@ int ensure_fused_buckets(ww_ctx *c, uint64_t nseg)
     if (e == hipSuccess) e = hipMemsetAsync(c->d_fbover, 0, sizeof(uint32_t), c->stream);
     if (e == hipSuccess) e = hipMalloc(&c->d_fprer, nseg * ww::kPresieveWordsPerSeg * sizeof(uint64_t));
     if (e == hipSuccess) e = hipMalloc(&c->d_fbspan, sizeof(ww::BucketSpan)); /* last: all made */
+    if (e == hipErrorOutOfMemory) {
+        (void)hipGetLastError();
+        free_fused_buckets(c);
+        *avail = false;
+        return WW_OK;
+    }
     if (e != hipSuccess) {
         free_fused_buckets(c);
         return fail(WW_E_HIP, "fused buckets: %s", hipGetErrorString(e));
@@ -408,9 +417,11 @@ int chunk_buckets(ww_ctx *c, uint64_t low, uint64_t high, const uint64_t **pre,
     const uint64_t chunks = ((nseg * 4096 - 1) >> ww::kBucketChunkBits) + 1;
     if (chunks > ww::kBucketMaxChunks) return WW_OK;
     int rc;
+    bool avail = false;
     if ((rc = ensure_primes32(c)) || (rc = ensure_chunk_span(c)) || (rc = ensure_pre_plain(c, nseg)) ||
-        (rc = ensure_fused_buckets(c, nseg)))
+        (rc = ensure_fused_buckets(c, nseg, &avail)))
         return rc;
+    if (!avail) return WW_OK;
     ww::BucketSpan bs{};
     bs.span = 0;
     bs.bit0 = 0;
