@@ -274,6 +274,10 @@ bool bucket_config_ok(const ww_ctx *c)
 
 /* Table index of the first prime above the pre-sieve bound: the odd primes up to 113 (29)
    and the wide table's primes up to the bound. */
+/* The share of a sieving prime's odd multiples that the bucket walk visits: cofactors
+   prime to 30030 (5760 of 15015 odd ones), and to 17 and 19. */
+constexpr double kVisited = 5760.0 / 15015.0 * 16.0 / 17.0 * 18.0 / 19.0;
+
 uint64_t bucket_first_index(const ww_ctx *c)
 {
     uint64_t n = 29;
@@ -372,7 +376,7 @@ int ensure_fused_buckets(ww_ctx *c, uint64_t nseg, bool *avail)
     free_fused_buckets(c);
     const uint64_t chunks = ((nseg * 4096 - 1) >> ww::kBucketChunkBits) + 1;
     const uint64_t bands = (chunks + ww::kBucketBandChunks - 1) / ww::kBucketBandChunks;
-    const double mean = std::ldexp(1.0, 20) * 5760.0 / 15015.0 * std::log(32.0 / 20.0);
+    const double mean = std::ldexp(1.0, 20) * kVisited * std::log(32.0 / 20.0);
     const double sc = c->bucket_scale;
     c->fbcap = (uint32_t)std::ceil(sc * (1.15 * mean / ww::kBucketSplitParts + 512));
     c->fband_cap = (uint32_t)std::ceil(sc * (1.25 * mean * ww::kBucketBandChunks / ww::kBucketGroups + 256));
@@ -1233,7 +1237,7 @@ int ww_batch_create(ww_ctx *c, const ww_range *ranges, uint32_t n, uint32_t resu
         e = hipMalloc(&b->d_pre, b->nseg * ww::kPresieveWordsPerSeg * sizeof(uint64_t));
     /* bucket spans: ranges at least bucket_min_width wide (and at most kBucketMaxChunks
        chunks). Buckets are sized for primes from 2^20 to sqrt(hi - 1): per 2^20-bit chunk,
-       2^20 x 5760/15015 (the cofactor wheel) x sum(1/l) hits, sum(1/l) from Mertens'
+       2^20 x kVisited (the odd multiples the walk visits) x sum(1/l) hits, sum(1/l) from Mertens'
        ln ln top - ln ln 2^20 (within 0.005 of the true sum). A chunk part's bucket (one of
        kBucketSplitParts) holds 1.15x its share of that mean plus 512; a band bucket (16
        chunks, one per fill workgroup) 1.25x its mean plus 256. An overflow only costs the
@@ -1260,7 +1264,7 @@ int ww_batch_create(ww_ctx *c, const ww_range *ranges, uint32_t n, uint32_t resu
             chunks += nch;
             bands += bs.nbands;
             const double lt = std::log((double)bs.top), l20 = 20.0 * std::log(2.0);
-            if (lt > l20) mean = std::max(mean, std::ldexp(1.0, 20) * 5760.0 / 15015.0 * std::log(lt / l20));
+            if (lt > l20) mean = std::max(mean, std::ldexp(1.0, 20) * kVisited * std::log(lt / l20));
             b->bspans.push_back(bs);
             spans[i].pad = (uint32_t)b->bspans.size();
         }

@@ -197,21 +197,23 @@ __global__ __launch_bounds__(kFillThreads) void k_bucket_fill(const BatchSpan *s
                 onext = orders[ic];
             }
             if (i >= bs.pidx1 || l > bs.top) continue;
-            BucketStart s = bucket_start(G0, l, inv_below(l), ord, res, wbits, wrank);
+            BucketWalk w = bucket_start(G0, l, inv_below(l), ord, res, wbits, wrank);
             /* walk in global odd-number indices; band buckets of this workgroup at
                wbase + band * bstride */
-            u64 B = bs.bit0 + s.b;
+            w.b += bs.bit0;
             const u64 Bend = bs.bit0 + nbits;
-            u32 idx = s.idx, c = s.c;
-            while (B < Bend) {
-                const u32 j = (u32)(B >> kBucketChunkBits) - bs.chunk_first;
-                const u32 band = j / kBucketBandChunks;
-                const u32 slot = wave_slot(cur, band);
-                if (slot < st.band_cap)
-                    st.band_entries[wbase + (u64)band * bstride + slot] =
-                        ((u32)B & ((1u << kBucketChunkBits) - 1)) |
-                        ((j % kBucketBandChunks) << kBucketChunkBits) | (c == 0u ? kBucketRescue : 0u);
-                bucket_step(B, idx, c, l, ord, gap);
+            while (w.b < Bend) {
+                if (bucket_visit(w)) {
+                    const u32 j = (u32)(w.b >> kBucketChunkBits) - bs.chunk_first;
+                    const u32 band = j / kBucketBandChunks;
+                    const u32 slot = wave_slot(cur, band);
+                    if (slot < st.band_cap)
+                        st.band_entries[wbase + (u64)band * bstride + slot] =
+                            ((u32)w.b & ((1u << kBucketChunkBits) - 1)) |
+                            ((j % kBucketBandChunks) << kBucketChunkBits) |
+                            (w.c == 0u ? kBucketRescue : 0u);
+                }
+                bucket_step(w, l, ord, gap);
             }
         }
         __syncthreads();

@@ -174,14 +174,15 @@ int ww_cpu_bucket_hits(uint64_t g0, uint64_t nbits, uint32_t ell, uint64_t *bits
         return ww_set_error(WW_E_ARG, "g0 must be odd and ell an odd prime above 2^20");
     static constexpr WheelTables wt = make_wheel();
     const u32 ord = order2_u32(ell, small_div_table(), kSmallOddPrimes);
-    BucketStart s = bucket_start(g0, ell, inv_below(ell), ord, wt.res, wt.bits, wt.rank);
+    BucketWalk w = bucket_start(g0, ell, inv_below(ell), ord, wt.res, wt.bits, wt.rank);
     uint64_t n = 0;
-    for (u64 b = s.b; b < nbits; ++n) {
+    for (; w.b < nbits; bucket_step(w, ell, ord, wt.gap)) {
+        if (!bucket_visit(w)) continue;
         if (n < cap) {
-            bits[n] = b;
-            rescue[n] = s.c == 0u;
+            bits[n] = w.b;
+            rescue[n] = w.c == 0u;
         }
-        bucket_step(b, s.idx, s.c, ell, ord, wt.gap);
+        ++n;
     }
     *count = n;
     return WW_OK;
