@@ -9,8 +9,8 @@
  * For a sieving prime l and the odd numbers G0 + 2b of a range (G0 odd), the multiples
  * n = l*k are walked only for k prime to 30030 = 2*3*5*7*11*13: n is odd and prime to
  * 3..13 exactly when k is, and any other n is no candidate (every candidate is odd and
- * prime to 3..113). That skips 9255 of every 15015 odd multiples; those with 17 | k or
- * 19 | k are stepped over without a visit, which leaves 0.342 of them. Each visited n also
+ * prime to 3..113). That skips 9255 of every 15015 odd multiples; those with 17, 19 or 23
+ * dividing k are stepped over without a visit, which leaves 0.327 of them. Each visited n also
  * gets its base-2 pseudoprime test modulo l. A composite n = l*k that is a base-2 Fermat
  * pseudoprime (every strong one is) has 2^(n-1) = 1 (mod l); such an n is not dropped but
  * flagged for the strong PRP test. By Fermat, 2^(n-1) = 2^(lk-1) = (2^(l-1))^k 2^(k-1) =
@@ -224,14 +224,14 @@ WW_HD u32 order2_u32(u32 l, const SmallDiv *tab, u32 ntab)
    G0 + 2b (G0 odd): the multiples n = l k with k prime to 30030, from the first one >= G0
    on. b is the current multiple's bit (the true difference n - G0 is below 22 l at the
    start, so mod-2^64 arithmetic gives it even where n passes 2^64), idx the wheel index of
-   k, c = (k - 1) mod ord, and k mod 17 and k mod 19: a multiple whose cofactor has a factor
-   17 or 19 is no candidate either, so it is stepped over but not visited (bucket_visit),
-   which leaves 16/17 x 18/19 of the wheel's multiples. */
+   k, c = (k - 1) mod ord, and k mod 17, 19 and 23: a multiple whose cofactor has one of
+   these factors is no candidate either, so it is stepped over but not visited
+   (bucket_visit), which leaves 16/17 x 18/19 x 22/23 of the wheel's multiples. */
 struct BucketWalk {
     u64 b;
     u32 idx;
     u32 c;
-    u32 m17, m19;
+    u32 m17, m19, m23;
 };
 
 WW_HD BucketWalk bucket_start(u64 G0, u32 l, double linv, u32 ord, const uint16_t *res,
@@ -257,15 +257,16 @@ WW_HD BucketWalk bucket_start(u64 G0, u32 l, double linv, u32 ord, const uint16_
     divmod_u64_u32(k - 1, ord, inv_below(ord), &q2, &w.c);
     w.m17 = (u32)(k % 17u);
     w.m19 = (u32)(k % 19u);
+    w.m23 = (u32)(k % 23u);
     return w;
 }
 
-/* Whether the walk's current multiple is a candidate position (its cofactor prime to 17
-   and 19; prime to 30030 by the wheel). */
-WW_HD bool bucket_visit(const BucketWalk &w) { return w.m17 != 0u && w.m19 != 0u; }
+/* Whether the walk's current multiple is a candidate position (its cofactor prime to 17,
+   19 and 23; prime to 30030 by the wheel). */
+WW_HD bool bucket_visit(const BucketWalk &w) { return w.m17 != 0u && w.m19 != 0u && w.m23 != 0u; }
 
 /* The next multiple: k -> k + gap, b -> b + gap l / 2, c -> (c + gap) mod ord with one
-   subtraction since gap <= 22 <= ord, and k mod 17, k mod 19 with at most two (gap < 34). */
+   subtraction since gap <= 22 <= ord, and k mod 17, 19, 23 with at most two (gap < 34). */
 WW_HD void bucket_step(BucketWalk &w, u32 l, u32 ord, const uint8_t *gap)
 {
     const u32 g = gap[w.idx];
@@ -276,6 +277,8 @@ WW_HD void bucket_step(BucketWalk &w, u32 l, u32 ord, const uint8_t *gap)
     w.m17 = w.m17 >= 34u ? w.m17 - 34u : w.m17 >= 17u ? w.m17 - 17u : w.m17;
     w.m19 += g;
     w.m19 = w.m19 >= 38u ? w.m19 - 38u : w.m19 >= 19u ? w.m19 - 19u : w.m19;
+    w.m23 += g;
+    w.m23 = w.m23 >= 23u ? w.m23 - 23u : w.m23;
     w.idx = w.idx + 1 == kWheelRes ? 0u : w.idx + 1;
 }
 
