@@ -255,9 +255,11 @@ WW_HD BucketWalk bucket_start(u64 G0, u32 l, double linv, u32 ord, const uint16_
     /* k - 1 < 2^44 + 2^5 and ord >= 22: the quotient is below 2^44 */
     u64 q2;
     divmod_u64_u32(k - 1, ord, inv_below(ord), &q2, &w.c);
-    w.m17 = (u32)(k % 17u);
-    w.m19 = (u32)(k % 19u);
-    w.m23 = (u32)(k % 23u);
+    /* k mod 17, 19, 23 in 32 bits: kq < 2^30 and 30030 = 8, 10, 15 (mod 17, 19, 23) */
+    const u32 kq32 = (u32)kq, rr = res[idx];
+    w.m17 = ((kq32 % 17u) * 8u + rr) % 17u;
+    w.m19 = ((kq32 % 19u) * 10u + rr) % 19u;
+    w.m23 = ((kq32 % 23u) * 15u + rr) % 23u;
     return w;
 }
 
