@@ -80,6 +80,26 @@ __device__ __forceinline__ u32 block_scan_1024(u32 v, u32 *wsum, u32 *total)
     return wsum[wave] + incl - v;
 }
 
+/* Index in a block (odd numbers first + 2 i) of the first odd multiple of the prime l
+   that the block marks: l^2 if that is in or past the block (smaller multiples have a
+   smaller prime factor), else the first odd multiple >= first. */
+__device__ __forceinline__ u64 prime_block_start(u32 l, u32 first)
+{
+    u64 start = (u64)l * l;
+    if (start < first) {
+        const u32 r = first % l;
+        u64 m = (u64)first + (r ? l - r : 0);
+        if ((m & 1) == 0) m += l;
+        start = m;
+    }
+    return (start - first) >> 1;
+}
+
+/* Primes below this many mark a block together (every thread a share of each one's
+   multiples); the larger ones, one per thread, have at most 2^20 / 2048 = 512 marks each,
+   so no thread waits on a long loop of its own. */
+constexpr u32 kPrimeShared = 2048;
+
 /* Block b of the odd numbers below 2^32: n = 2^21 b + 2 i + 1, i < 2^20. Marks the odd
    multiples of the primes 3..65521 from their squares on, and 1. Without `offsets` it
    writes the block's prime count to counts[b]; with them it writes the block's primes in
@@ -91,19 +111,19 @@ __global__ __launch_bounds__(kPrimeThreads) void k_primes32(const u32 *small, u3
     __shared__ u32 bm[kPrimeBlockWords];
     __shared__ u32 wsum[kPrimeThreads / 64 + 1];
     const u32 base = blockIdx.x * (2 * kPrimeBlockBits); /* < 2^32 */
+    const u32 first = base + 1;
     for (u32 w = threadIdx.x; w < kPrimeBlockWords; w += kPrimeThreads) bm[w] = 0;
     __syncthreads();
-    for (u32 j = threadIdx.x; j < nsmall; j += kPrimeThreads) {
+    u32 j = 0;
+    for (; j < nsmall && small[j] < kPrimeShared; ++j) { /* uniform: the whole block */
         const u32 l = small[j];
-        u64 start = (u64)l * l; /* smaller multiples have a smaller prime factor */
-        const u32 first = base + 1;
-        if (start < first) {
-            const u32 r = first % l;
-            u64 m = (u64)first + (r ? l - r : 0);
-            if ((m & 1) == 0) m += l;
-            start = m;
-        }
-        for (u64 i = (start - first) >> 1; i < kPrimeBlockBits; i += l)
+        for (u64 i = prime_block_start(l, first) + (u64)threadIdx.x * l; i < kPrimeBlockBits;
+             i += (u64)kPrimeThreads * l)
+            atomicOr(&bm[i >> 5], 1u << (i & 31));
+    }
+    for (j += threadIdx.x; j < nsmall; j += kPrimeThreads) {
+        const u32 l = small[j];
+        for (u64 i = prime_block_start(l, first); i < kPrimeBlockBits; i += l)
             atomicOr(&bm[i >> 5], 1u << (i & 31));
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(&bm[0], 1u); /* 1 is not prime */
