@@ -153,7 +153,7 @@ int ww_get_presieve_bound(ww_ctx *ctx, uint32_t *bound);
    about 0.85 bytes of device memory per integer of width (7.3 GB for a 2^33-wide range;
    fused chunks keep one such store per context, for the widest chunk so far) and, on
    first use, the odd primes below 2^32 and their base-2 orders, 813 MB each, built on the
-   device in about 0.5 s. Default WW_DEFAULT_BUCKET_MIN_WIDTH. */
+   device in about 0.05 s. Default WW_DEFAULT_BUCKET_MIN_WIDTH. */
 int ww_set_bucket_sieve(ww_ctx *ctx, uint64_t min_width);
 int ww_get_bucket_sieve(ww_ctx *ctx, uint64_t *min_width);
 

@@ -227,19 +227,22 @@ int ensure_primes32(ww_ctx *c)
                                 c->d_primes32, c->stream);
     /* the base-2 orders of the primes above 2^20 */
     ww::SmallDiv *d_div = nullptr;
+    uint64_t *d_bounds = nullptr;
     if (e == hipSuccess) e = hipMalloc(&c->d_orders32, off[kBlocks] * sizeof(uint32_t));
     if (e == hipSuccess) e = hipMalloc(&d_div, ww::kSmallOddPrimes * sizeof(ww::SmallDiv));
+    if (e == hipSuccess) e = hipMalloc(&d_bounds, ((1u << 16) + 1) * sizeof(uint64_t));
     if (e == hipSuccess)
         e = hipMemcpyAsync(d_div, ww::small_div_table(), ww::kSmallOddPrimes * sizeof(ww::SmallDiv),
                            hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess)
         e = ww::launch_order2(c->d_primes32, off[kBlocks], d_div, ww::kSmallOddPrimes, c->d_orders32,
-                              c->stream);
+                              d_bounds, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (d_small) (void)hipFree(d_small);
     if (d_counts) (void)hipFree(d_counts);
     if (d_off) (void)hipFree(d_off);
     if (d_div) (void)hipFree(d_div);
+    if (d_bounds) (void)hipFree(d_bounds);
     if (e != hipSuccess) {
         if (c->d_primes32) (void)hipFree(c->d_primes32);
         if (c->d_orders32) (void)hipFree(c->d_orders32);

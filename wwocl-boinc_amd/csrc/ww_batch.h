@@ -144,11 +144,12 @@ hipError_t launch_bucket_sieve(const BatchSpan *spans, const uint64_t *prefix,
                                const BucketStore &st, hipStream_t s);
 
 /* orders[i] = ord_l(2) for every table prime l = primes[i] above 2^20, 0 for the others
-   (i < n); small = the odd primes below 2^16 with their divisibility data (ww_bucket.h's
-   SmallDiv, nsmall of them). */
+   (i < n, the whole table of odd primes below 2^32); small = the odd primes below 2^16 with
+   their divisibility data (ww_bucket.h's SmallDiv, nsmall of them); bounds = device
+   scratch for 2^16 + 1 table indices. */
 struct SmallDiv;
 hipError_t launch_order2(const uint32_t *primes, uint64_t n, const SmallDiv *small,
-                         uint32_t nsmall, uint32_t *orders, hipStream_t s);
+                         uint32_t nsmall, uint32_t *orders, uint64_t *bounds, hipStream_t s);
 
 /* Writes the one-span table of the range [lo, hi) (segments from lo | 1) into span and
    prefix (device scratch), so that a single range needs no host copy. */

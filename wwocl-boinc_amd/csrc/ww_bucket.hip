@@ -349,62 +349,127 @@ __global__ __launch_bounds__(kFillThreads) void k_bucket_split(const BucketSpan 
 }
 
 /* The order table: orders[i] = ord_l(2) for every table prime l = primes[i] above 2^20 (0
-   for the smaller ones), one lane per prime. The trial division of l - 1 by the odd primes
-   below 2^16 runs as one loop for the whole wave, so the divisors' entries are scalar
-   loads (eight at a time), and ends once q^2 exceeds every lane's unfactored part m. A lane that goes on past
-   its own bound cannot find a new factor: its m is 1 or a prime, which is recorded (and
-   becomes 1) if the loop reaches it, as the host's search records it after its loop. The
-   factors go to register slots, and order2_from_factors (shared with the host) finishes. */
-constexpr int kOrderThreads = 256;
+   for the smaller ones), one workgroup per block of 2^16 integers. The odd prime factors
+   of l - 1 come in two parts:
+   - those above 2^11 by a sieve: for each odd prime q in (2^11, 2^16) the workgroup walks
+     the odd n of its block with q | n - 1 and records q's table index in an LDS slot per
+     odd n. At most two such q divide a number below 2^32 (three would exceed 2^33), so two
+     16-bit halves of one word hold them;
+   - those below 2^11 by trial division, one loop per wave over the 308 odd primes below
+     2^11 (scalar loads, eight at a time), ending once q^2 exceeds every lane's unfactored
+     part m. A lane that goes on past its own bound cannot find a new factor.
+   What is left of m after both is 1 or a prime above 2^16 (every composite below 2^32 has
+   a factor below 2^16), or a prime below 2^11 when the loop ended early. The factors go to
+   register slots, and order2_from_factors (shared with the host) finishes. */
+constexpr int kOrderThreads = 1024;
+constexpr u32 kOrderBlockBits = 16;
+constexpr u32 kOrderBlockOdd = 1u << (kOrderBlockBits - 1);
+constexpr u32 kOrderTrial = 308; /* odd primes below 2^11: the first table entries */
 
-__global__ __launch_bounds__(kOrderThreads) void k_order2(const u32 *primes, u64 n,
+/* bounds[t] = the first table index whose prime is at least t 2^16, t < nbounds. */
+__global__ void k_order_bounds(const u32 *primes, u64 n, u32 nbounds, u64 *bounds)
+{
+    const u32 t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nbounds) return;
+    const u64 v = (u64)t << kOrderBlockBits;
+    u64 a = 0, b = n;
+    while (a < b) {
+        const u64 mid = (a + b) >> 1;
+        if ((u64)primes[mid] < v) a = mid + 1;
+        else b = mid;
+    }
+    bounds[t] = a;
+}
+
+__device__ __forceinline__ void append_factor(u32 *f, u32 &nf, u32 q)
+{
+#pragma unroll
+    for (u32 k = 0; k < kOrderFactors; ++k)
+        if (k == nf) f[k] = q;
+    ++nf;
+}
+
+__global__ __launch_bounds__(kOrderThreads) void k_order2(const u32 *primes, const u64 *bounds,
                                                           const SmallDiv *__restrict__ tab,
                                                           u32 ntab, u32 *orders)
 {
-    const u64 i = (u64)blockIdx.x * kOrderThreads + threadIdx.x;
-    const u32 l = i < n ? primes[i] : 0u;
-    const bool act = l > (1u << kBucketChunkBits);
-    u32 m = act ? (l - 1) >> __builtin_ctz(l - 1) : 1u;
-    u32 f[kOrderFactors];
-    u32 nf = 0;
-#pragma unroll
-    for (u32 k = 0; k < kOrderFactors; ++k) f[k] = 0;
-    /* groups of 8 divisors, loaded together; the exit test takes the group's first */
-    constexpr u32 kGroup = 8;
-    for (u32 j0 = 0; j0 < ntab; j0 += kGroup) {
-        if (__ballot(tab[j0].q * tab[j0].q <= m) == 0) break;
-        SmallDiv g[kGroup];
-#pragma unroll
-        for (u32 u = 0; u < kGroup; ++u) g[u] = tab[j0 + u < ntab ? j0 + u : ntab - 1];
-#pragma unroll
-        for (u32 u = 0; u < kGroup; ++u) {
-            const SmallDiv t = g[u];
-            if (j0 + u < ntab && small_divides(m, t)) {
-#pragma unroll
-                for (u32 k = 0; k < kOrderFactors; ++k)
-                    if (k == nf) f[k] = t.q;
-                ++nf;
-                do m *= t.qinv; while (small_divides(m, t));
-            }
+    __shared__ u32 big[kOrderBlockOdd]; /* per odd n of the block: 1 + two table indices */
+    const u32 blk = blockIdx.x;
+    const u64 i0 = bounds[blk], i1 = bounds[blk + 1];
+    const u32 base = blk << kOrderBlockBits;
+    if (base < (1u << kBucketChunkBits)) { /* no orders below the bucket tier */
+        for (u64 i = i0 + threadIdx.x; i < i1; i += kOrderThreads) orders[i] = 0;
+        return;
+    }
+    for (u32 w = threadIdx.x; w < kOrderBlockOdd; w += kOrderThreads) big[w] = 0;
+    __syncthreads();
+    /* odd n = base + 1 + 2 x: q | n - 1 iff x = -base/2 (mod q) */
+    for (u32 j = kOrderTrial + threadIdx.x; j < ntab; j += kOrderThreads) {
+        const u32 q = tab[j].q;
+        const u32 r = (base >> 1) % q;
+        for (u32 x = r ? q - r : 0u; x < kOrderBlockOdd; x += q) {
+            const u32 v = j + 1;
+            if (atomicCAS(&big[x], 0u, v) != 0u) atomicOr(&big[x], v << 16);
         }
     }
-    if (m > 1u) {
+    __syncthreads();
+    /* the block's primes, one lane each (the lanes of a wave stay together) */
+    const u64 count = i1 - i0;
+    for (u64 t0 = 0; t0 < count; t0 += kOrderThreads) {
+        const u64 i = i0 + t0 + threadIdx.x;
+        const bool act = t0 + threadIdx.x < count;
+        const u32 l = act ? primes[i] : 3u;
+        u32 m = act ? (l - 1) >> __builtin_ctz(l - 1) : 1u;
+        u32 f[kOrderFactors];
+        u32 nf = 0;
 #pragma unroll
-        for (u32 k = 0; k < kOrderFactors; ++k)
-            if (k == nf) f[k] = m;
-        ++nf;
+        for (u32 k = 0; k < kOrderFactors; ++k) f[k] = 0;
+        constexpr u32 kGroup = 8;
+        for (u32 j0 = 0; j0 < kOrderTrial; j0 += kGroup) {
+            if (__ballot(tab[j0].q * tab[j0].q <= m) == 0) break;
+            SmallDiv g[kGroup];
+#pragma unroll
+            for (u32 u = 0; u < kGroup; ++u) g[u] = tab[j0 + u < kOrderTrial ? j0 + u : kOrderTrial - 1];
+#pragma unroll
+            for (u32 u = 0; u < kGroup; ++u) {
+                const SmallDiv d = g[u];
+                if (j0 + u < kOrderTrial && small_divides(m, d)) {
+                    append_factor(f, nf, d.q);
+                    do m *= d.qinv; while (small_divides(m, d));
+                }
+            }
+        }
+        if (act) {
+            const u32 bv = big[(l - base - 1) >> 1];
+#pragma unroll
+            for (u32 h = 0; h < 2; ++h) {
+                const u32 v = (bv >> (16 * h)) & 0xffffu;
+                if (v) {
+                    const SmallDiv d = tab[v - 1];
+                    append_factor(f, nf, d.q);
+                    do m *= d.qinv; while (small_divides(m, d));
+                }
+            }
+        }
+        if (m > 1u) append_factor(f, nf, m);
+        if (act) orders[i] = order2_from_factors(l, inv_below(l), f, nf);
     }
-    if (i < n) orders[i] = act ? order2_from_factors(l, inv_below(l), f, nf) : 0u;
 }
 
 } // namespace
 
 hipError_t launch_order2(const u32 *primes, u64 n, const SmallDiv *small, u32 nsmall,
-                         u32 *orders, hipStream_t s)
+                         u32 *orders, u64 *bounds, hipStream_t s)
 {
-    const u64 blocks = (n + kOrderThreads - 1) / kOrderThreads;
-    if (blocks == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_order2, dim3((u32)blocks), dim3(kOrderThreads), 0, s, primes, n, small,
+    /* blocks of 2^16 integers up to 2^32, and their table bounds (2^16 + 1 of them) */
+    constexpr u32 kBlocks = 1u << (32 - kOrderBlockBits);
+    if (n == 0) return hipSuccess;
+    if (nsmall <= kOrderTrial) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_order_bounds, dim3((kBlocks + 1 + 255) / 256), dim3(256), 0, s, primes, n,
+                       kBlocks + 1, bounds);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_order2, dim3(kBlocks), dim3(kOrderThreads), 0, s, primes, bounds, small,
                        nsmall, orders);
     return hipGetLastError();
 }
