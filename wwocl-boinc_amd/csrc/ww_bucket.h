@@ -10,8 +10,8 @@
  * n = l*k are walked only for k prime to 30030 = 2*3*5*7*11*13: n is odd and prime to
  * 3..13 exactly when k is, and any other n is no candidate (every candidate is odd and
  * prime to 3..113). That skips 9255 of every 15015 odd multiples; those with 17, 19 or 23
- * dividing k are stepped over without a visit, which leaves 0.327 of them. Each visited n also
- * gets its base-2 pseudoprime test modulo l. A composite n = l*k that is a base-2 Fermat
+ * dividing k are stepped over without a visit, which leaves 0.327 of them. Each visited n
+ * also gets its base-2 pseudoprime test modulo l. A composite n = l*k that is a base-2 Fermat
  * pseudoprime (every strong one is) has 2^(n-1) = 1 (mod l); such an n is not dropped but
  * flagged for the strong PRP test. By Fermat, 2^(n-1) = 2^(lk-1) = (2^(l-1))^k 2^(k-1) =
  * 2^(k-1) (mod l), so the test is ord_l(2) | k - 1: the walk carries c = (k - 1) mod ord_l(2)
@@ -60,8 +60,18 @@ constexpr WheelTables make_wheel()
     return t;
 }
 
-/* The wheel's largest gap. */
+/* The wheel's largest gap: bucket_step's single subtraction mod ord (ord >= 22) and its
+   at most two mod 17, 19, 23 rely on it. */
 constexpr u32 kWheelMaxGap = 22;
+
+constexpr u32 wheel_max_gap()
+{
+    constexpr WheelTables t = make_wheel();
+    u32 g = 0;
+    for (u32 i = 0; i < kWheelRes; ++i) g = t.gap[i] > g ? t.gap[i] : g;
+    return g;
+}
+static_assert(wheel_max_gap() == kWheelMaxGap, "the wheel's largest gap");
 
 /* Index of the first residue >= r (r < 30030), kWheelRes when there is none. */
 WW_HD u32 wheel_index(const uint32_t *bits, const uint16_t *rank, u32 r)
