@@ -140,8 +140,8 @@ int ww_get_presieve_bound(ww_ctx *ctx, uint32_t *bound);
 #define WW_DEFAULT_BUCKET_MIN_WIDTH (1ull << 30)
 
 /* The bucket sieve of the persistent form, for ranges at least min_width wide in
-   batches created afterwards and for fused chunks (ww_search_fused) at least that wide
-   (UINT64_MAX: none). No counterpart in the reference. Such a
+   batches created afterwards and for chunks of ww_search_fused and ww_getsegprimes at
+   least that wide (UINT64_MAX: none). No counterpart in the reference. Such a
    range is sieved by every prime up to sqrt(hi - 1): the deep sieve and the wide pre-sieve
    take the primes up to the pre-sieve bound, and one more kernel drops the hits of the
    primes above it, up to 2^32, into per-chunk buckets that the pre-sieve applies. Every
@@ -151,7 +151,7 @@ int ww_get_presieve_bound(ww_ctx *ctx, uint32_t *bound);
    deep bound is at least 1024, the pre-sieve bound at least 2^20 and the pre-sieve segment
    2^20 bits (the defaults); elsewhere such a batch runs as without it. Costs per range
    about 0.85 bytes of device memory per integer of width (7.3 GB for a 2^33-wide range;
-   fused chunks keep one such store per context, for the widest chunk so far) and, on
+   single chunks keep one such store per context, for the widest chunk so far) and, on
    first use, the odd primes below 2^32 and their base-2 orders, 813 MB each, built on the
    device in about 0.05 s. Default WW_DEFAULT_BUCKET_MIN_WIDTH. */
 int ww_set_bucket_sieve(ww_ctx *ctx, uint64_t min_width);
@@ -199,7 +199,9 @@ int ww_clear_prp_count(ww_ctx *ctx);
 
 /* getsegprimes (getsegprimes.cl:228-301, launched at main.cpp:1719-1727): append the
    PRPs of [low, high) to the list. The reference also passes a mod-30 wheel index
-   computed by findWheelOffset (main.cpp:528-587); this sieve needs none.
+   computed by findWheelOffset (main.cpp:528-587); this sieve needs none. A range at
+   least the bucket width (ww_set_bucket_sieve) is sieved up to sqrt(high - 1) first,
+   and its candidates that no sieve tier flags join the list without the PRP test.
    Requires WW_MIN_START <= low < high and high - low <= 2^34. Stream-ordered. */
 int ww_getsegprimes(ww_ctx *ctx, uint64_t low, uint64_t high);
 

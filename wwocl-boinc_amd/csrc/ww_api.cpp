@@ -807,10 +807,11 @@ int ww_getsegprimes(ww_ctx *c, uint64_t low, uint64_t high)
     hipEvent_t a, b;
     if ((rc = event_pair(c, c->ev_sieve, &a, &b))) return rc;
     HIP_TRY(hipEventRecord(a, c->stream));
-    const uint64_t *pre = nullptr;
-    if ((rc = chunk_presieve(c, low, high, &pre))) return rc;
+    const uint64_t *pre = nullptr, *prer = nullptr;
+    if ((rc = chunk_buckets(c, low, high, &pre, &prer))) return rc;
+    if (!prer && (rc = chunk_presieve(c, low, high, &pre))) return rc;
     HIP_TRY(ww::launch_sieve_prp(low, high, c->d_list, c->list_cap, c->d_state, c->d_deep,
-                                 c->ndeep, pre, c->stream));
+                                 c->ndeep, pre, c->stream, prer));
     HIP_TRY(hipEventRecord(b, c->stream));
     c->launches++;
     if (c->ev_sieve.size() >= kMaxPendingEvents) return collect(c->ev_sieve, &c->ms_sieve);

@@ -48,7 +48,7 @@ constexpr uint32_t kMaxDeepBound = 65535;
 
 hipError_t launch_sieve_prp(uint64_t lo, uint64_t hi, uint64_t *list, uint64_t cap,
                             DevState *st, const DeepPrime *deep, uint32_t ndeep,
-                            const uint64_t *pre, hipStream_t s);
+                            const uint64_t *pre, hipStream_t s, const uint64_t *prer = nullptr);
 hipError_t launch_ww_tests(const uint64_t *list, uint64_t cap, DevState *st,
                            ww_result *res, uint32_t res_cap, int grid, hipStream_t s);
 int tests_blocks_per_cu();

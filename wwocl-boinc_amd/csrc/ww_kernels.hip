@@ -5,7 +5,9 @@
  *                 registers, optional deep sieve by 127..B in LDS with exact rescue of
  *                 base-2 pseudoprimes), wave-local compaction into LDS, base-2 strong PRP on the
  *                 compacted candidates, in-place compaction of the survivors and one
- *                 global append per workgroup. Replaces getsegprimes.cl:228-301.
+ *                 global append per workgroup. With the bucket sieve's needs-PRP words,
+ *                 candidates outside them are proven prime and join the list without the
+ *                 test. Replaces getsegprimes.cl:228-301.
  *   k_ww_tests    Wieferich + Wall-Sun-Sun on every list entry, one entry per lane,
  *                 wave-reduced checksum, result records. Replaces wieferich.cl:196-328
  *                 and wallsunsun.cl:221-354.
@@ -19,7 +21,7 @@ namespace ww {
 __global__ __launch_bounds__(kSieveThreads) void k_sieve_prp(u64 base, u64 hi, u64 *list,
                                                               u64 cap, DevState *st,
                                                               const DeepPrime *deep, u32 ndeep,
-                                                              const u64 *pre)
+                                                              const u64 *pre, const u64 *prer)
 {
     __shared__ __align__(16) uint16_t cand[kSieveWaves * kWaveCandCap];
     __shared__ u64 deep_bm[kSieveWaves][64];
@@ -35,9 +37,16 @@ __global__ __launch_bounds__(kSieveThreads) void k_sieve_prp(u64 base, u64 hi, u
 
     /* ---- sieve this lane's 64 odd numbers ---- */
     u64 bits = sieve_word(block_base, lane_pos, hi);
-    /* wide pre-sieve of this wave's segment (segments count from base) */
-    if (pre && wave_base < hi)
-        bits &= ~pre[((u64)blockIdx.x * kSieveWaves + wave) * 64 + lane];
+    /* wide pre-sieve of this wave's segment (segments count from base); with the bucket
+       sieve also the candidates the strong PRP test must decide (the others are proven
+       prime: every prime factor up to sqrt(hi) would have dropped or flagged them) */
+    u64 need = ~0ull;
+    if (pre && wave_base < hi) {
+        const u64 w = ((u64)blockIdx.x * kSieveWaves + wave) * 64 + lane;
+        bits &= ~pre[w];
+        if (prer) need = prer[w];
+    }
+    u64 classes = 0;
     if (ndeep) {
         /* the candidate buffer is free until the compaction below: it holds the deep
            sieve's pseudoprime classes meanwhile */
@@ -48,13 +57,24 @@ __global__ __launch_bounds__(kSieveThreads) void k_sieve_prp(u64 base, u64 hi, u
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        bits &= ~deep_sieve(bm, rm, wave_base, deep, ndeep, lane);
+        bits &= ~deep_sieve(bm, rm, wave_base, deep, ndeep, lane, &classes);
     }
+    need |= classes;
+    const u64 known = bits & ~need;
+    bits &= need;
 
     /* ---- wave-local compaction of candidate positions into LDS, grouped by the
        trailing zeros of n - 1 ---- */
     uint16_t *wc = cand + wave * kWaveCandCap;
     const u32 ncand = compact_by_class(bits, wave_base, lane, wc, 0, 0);
+    /* the known primes' count before this lane (a wave prefix sum) and in all */
+    u32 kpre = (u32)__popcll(known);
+    for (int d = 1; d < 64; d <<= 1) {
+        const u32 t = __shfl_up(kpre, d, 64);
+        if (lane >= (u32)d) kpre += t;
+    }
+    const u32 nknown = __shfl(kpre, 63, 64);
+    kpre -= (u32)__popcll(known);
     __syncthreads();
 
     /* ---- strong PRP over the compacted candidates, survivors compacted in place ---- */
@@ -71,6 +91,16 @@ __global__ __launch_bounds__(kSieveThreads) void k_sieve_prp(u64 base, u64 hi, u
         }
         nout += __popcll(bal);
     }
+    /* the known primes after the PRP survivors (together at most the wave's candidates) */
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (u64 b = known; b; b &= b - 1)
+        wc[nout + kpre++] = (uint16_t)(lane * 64 + (u32)__builtin_ctzll(b));
+    nout += nknown;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 
     /* ---- one global append per workgroup ---- */
     if (lane == 0) s_cnt[wave] = nout;
@@ -159,14 +189,15 @@ int tests_blocks_per_cu()
 }
 
 hipError_t launch_sieve_prp(u64 lo, u64 hi, u64 *list, u64 cap, DevState *st,
-                            const DeepPrime *deep, u32 ndeep, const u64 *pre, hipStream_t s)
+                            const DeepPrime *deep, u32 ndeep, const u64 *pre, hipStream_t s,
+                            const u64 *prer)
 {
     u64 base = lo | 1;
     if (base >= hi) return hipSuccess;
     u64 span = hi - base;
     u64 blocks = (span + kIntsPerBlock - 1) / kIntsPerBlock;
     hipLaunchKernelGGL(k_sieve_prp, dim3((unsigned)blocks), dim3(kSieveThreads), 0, s, base, hi,
-                       list, cap, st, deep, ndeep, pre);
+                       list, cap, st, deep, ndeep, pre, pre ? prer : nullptr);
     return hipGetLastError();
 }
 
