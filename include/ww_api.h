@@ -172,10 +172,11 @@ int ww_prime_table(ww_ctx *ctx, uint64_t first, uint64_t n, uint32_t *out, uint6
 int ww_prime_orders(ww_ctx *ctx, uint64_t first, uint64_t n, uint32_t *out, uint64_t *total);
 
 /* The bucket sieve's walk of one sieving prime, on the host with the code the device runs
-   (verification): the multiples n = ell*k in [g0, g0 + 2 nbits) with k prime to 30030 x 17 x 19 x 23 (g0
-   odd, 2^20 < ell < 2^32 prime), as bit indices (n - g0) / 2 with a 1 in rescue[] where
-   2^(n-1) = 1 (mod ell) (the prime's base-2 pseudoprime class). Writes up to cap of each;
-   *count = the number of hits. */
+   (verification): the multiples n = ell*k in [g0, g0 + 2 nbits) with k prime to
+   30030 x 17 x 19 x 23 (g0 odd, 2^20 < ell < 2^32 prime, and 2 nbits <= 30028 ell: the
+   window holds fewer than 30030 cofactors, as every bucket span's does), as bit indices
+   (n - g0) / 2 with a 1 in rescue[] where 2^(n-1) = 1 (mod ell) (the prime's base-2
+   pseudoprime class). Writes up to cap of each; *count = the number of hits. */
 int ww_cpu_bucket_hits(uint64_t g0, uint64_t nbits, uint32_t ell, uint64_t *bits,
                        uint8_t *rescue, uint64_t cap, uint64_t *count);
 

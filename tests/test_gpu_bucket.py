@@ -1,8 +1,9 @@
 """The bucket sieve on the device (ww_bucket.hip, the pre-sieve's apply, k_batch's known
 primes): the table of odd primes below 2^32, and batch results equal to the oracle's
 wherever the bucket sieve runs, including ranges that hold base-2 strong pseudoprimes whose
-factors lie in each sieve tier (which the tiers must keep for the strong PRP test), mixed
-batches, the configurations that turn it off, graph replays, and overflowing buckets."""
+factors lie in each sieve tier (which the tiers must keep for the strong PRP test), the
+plain form's PRP list and fused chunks with the bucket sieve, mixed batches, the
+configurations that turn it off, graph replays, and overflowing buckets."""
 import random
 
 import numpy as np
@@ -188,6 +189,44 @@ def test_fused_overflowing_buckets_stay_exact(bctx):
     bctx.set_bucket_capacity(0.1)
     lo, hi = (1 << 63) + 1001, (1 << 63) + 1001 + (1 << 23)
     got = bctx.run_range_fused(lo, hi)
+    assert (got.prp_count, got.checksum, got.results) == oracle.range_results(lo, hi)
+
+
+def plain_list(ctx, lo, hi):
+    """ww_getsegprimes over [lo, hi) into an empty list, read back sorted."""
+    ctx.clear_prp_count()
+    ctx.getsegprimes(lo, hi)
+    ctx.finish()
+    return sorted(ctx.prps())
+
+
+@pytest.mark.parametrize("lo,hi", RANGES)
+def test_plain_form_with_buckets_vs_oracle(bctx, lo, hi):
+    """ww_getsegprimes with the bucket sieve: known primes join the list without the strong
+    PRP test and flagged candidates are tested, so the list is the oracle's; the two-kernel
+    form's counters and records equal the oracle's, in one chunk and in uneven ones."""
+    assert plain_list(bctx, lo, hi) == oracle.prp_list(lo, hi), (lo, hi)
+    want = oracle.range_results(lo, hi)
+    for chunk in (hi - lo, (hi - lo) // 3 + 12345):
+        got = bctx.run_range(lo, hi, chunk)
+        assert (got.prp_count, got.checksum, got.results) == want, (lo, hi, chunk)
+
+
+def test_plain_form_keeps_strong_pseudoprimes(bctx):
+    cases = [n for _, n in twin_spsp2(1500, 1)] + [n for _, n in twin_spsp2(70001, 1)]
+    cases += [n for _, n in twin_spsp2(2100000, 1)] + [n for _, n in twin_spsp2(2500000000, 1)]
+    cases += chernick_spsp2()[:1]
+    for n in cases:
+        lo, hi = n - (1 << 21), n + (1 << 21)
+        got = plain_list(bctx, lo, hi)
+        assert n in got and got == oracle.prp_list(lo, hi), n
+
+
+def test_plain_form_overflowing_buckets_stay_exact(bctx):
+    bctx.set_bucket_capacity(0.1)
+    lo, hi = (1 << 63) + 1001, (1 << 63) + 1001 + (1 << 23)
+    assert plain_list(bctx, lo, hi) == oracle.prp_list(lo, hi)
+    got = bctx.run_range(lo, hi)
     assert (got.prp_count, got.checksum, got.results) == oracle.range_results(lo, hi)
 
 

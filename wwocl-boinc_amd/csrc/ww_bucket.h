@@ -10,7 +10,8 @@
  * n = l*k are walked only for k prime to 30030 = 2*3*5*7*11*13: n is odd and prime to
  * 3..13 exactly when k is, and any other n is no candidate (every candidate is odd and
  * prime to 3..113). That skips 9255 of every 15015 odd multiples; those with 17, 19 or 23
- * dividing k are stepped over without a visit, which leaves 0.327 of them. Each visited n
+ * dividing k are stepped over without a visit (k mod 7429 = 17 x 19 x 23 is carried and
+ * looked up in a bitmap), which leaves 0.327 of them. Each visited n
  * also gets its base-2 pseudoprime test modulo l. A composite n = l*k that is a base-2 Fermat
  * pseudoprime (every strong one is) has 2^(n-1) = 1 (mod l); such an n is not dropped but
  * flagged for the strong PRP test. By Fermat, 2^(n-1) = 2^(lk-1) = (2^(l-1))^k 2^(k-1) =
@@ -27,16 +28,27 @@ namespace ww {
 
 /* Cofactor wheel modulo 30030: its 5760 residues prime to 30030, ascending, the gaps
    between them, and the rank structure that gives the index of the first residue >= r
-   (a bitmap of the residues and the count of those below each 32-bit word). */
+   (a bitmap of the residues and the count of those below each 32-bit word). The gaps are
+   stored twice over: a walk covers fewer than 30030 consecutive cofactors (kWalkMaxK), so
+   it passes at most 5760 residues and its index, from below 5760, stays below 2 x 5760
+   without wrapping. Beside them, the bitmap of the residues of k mod 7429 = 17 x 19 x 23
+   prime to 7429: the cofactors that are visited. */
 constexpr u32 kWheelMod = 30030;
 constexpr u32 kWheelRes = 5760;
 constexpr u32 kWheelWords = (kWheelMod + 31) / 32;
+constexpr u32 kVisitMod = 17 * 19 * 23; /* 7429 */
+constexpr u32 kVisitWords = (kVisitMod + 31) / 32;
+constexpr u32 kWheelModVisit = kWheelMod % kVisitMod; /* 314 */
+/* Most cofactors k in one walk: the window [kmin, kmax) is at most this long. */
+constexpr u64 kWalkMaxK = kWheelMod - 1;
 
 struct WheelTables {
-    uint16_t res[kWheelRes];     /* the residues */
-    uint8_t gap[kWheelRes];      /* res[i + 1] - res[i], and 30030 + res[0] - res[5759] */
-    uint32_t bits[kWheelWords];  /* bit r % 32 of word r / 32: r is a residue */
-    uint16_t rank[kWheelWords];  /* residues below 32 w */
+    uint16_t res[kWheelRes];       /* the residues */
+    uint8_t gap[2 * kWheelRes];    /* res[i + 1] - res[i], and 30030 + res[0] - res[5759];
+                                      entry i + 5760 = entry i */
+    uint32_t bits[kWheelWords];    /* bit r % 32 of word r / 32: r is a residue */
+    uint16_t rank[kWheelWords];    /* residues below 32 w */
+    uint32_t visit[kVisitWords];   /* bit m % 32 of word m / 32: m prime to 7429 */
 };
 
 constexpr WheelTables make_wheel()
@@ -50,18 +62,20 @@ constexpr WheelTables make_wheel()
         }
     for (u32 i = 0; i < kWheelRes; ++i) {
         const u32 nx = i + 1 < kWheelRes ? t.res[i + 1] : kWheelMod + t.res[0];
-        t.gap[i] = (uint8_t)(nx - t.res[i]);
+        t.gap[i] = t.gap[i + kWheelRes] = (uint8_t)(nx - t.res[i]);
     }
     u32 below = 0;
     for (u32 w = 0; w < kWheelWords; ++w) {
         t.rank[w] = (uint16_t)below;
         for (u32 b = 0; b < 32; ++b) below += (t.bits[w] >> b) & 1u;
     }
+    for (u32 m = 0; m < kVisitMod; ++m)
+        if (m % 17 && m % 19 && m % 23) t.visit[m / 32] |= 1u << (m % 32);
     return t;
 }
 
-/* The wheel's largest gap: bucket_step's single subtraction mod ord (ord >= 22) and its
-   at most two mod 17, 19, 23 rely on it. */
+/* The wheel's largest gap: bucket_advance's single subtractions mod ord (ord >= 22) and
+   mod 7429 rely on it. */
 constexpr u32 kWheelMaxGap = 22;
 
 constexpr u32 wheel_max_gap()
@@ -232,16 +246,17 @@ WW_HD u32 order2_u32(u32 l, const SmallDiv *tab, u32 ntab)
 
 /* The walk of the sieving prime l (2^20 < l < 2^32, ord = ord_l(2)) over the odd numbers
    G0 + 2b (G0 odd): the multiples n = l k with k prime to 30030, from the first one >= G0
-   on. b is the current multiple's bit (the true difference n - G0 is below 22 l at the
-   start, so mod-2^64 arithmetic gives it even where n passes 2^64), idx the wheel index of
-   k, c = (k - 1) mod ord, and k mod 17, 19 and 23: a multiple whose cofactor has one of
-   these factors is no candidate either, so it is stepped over but not visited
-   (bucket_visit), which leaves 16/17 x 18/19 x 22/23 of the wheel's multiples. */
+   on, for at most kWalkMaxK consecutive cofactors k. d = n - G0 is the current multiple's
+   distance, twice its bit (the true distance is below 22 l at the start, so mod-2^64
+   arithmetic gives it even where n passes 2^64), idx the wheel index of k, c =
+   (k - 1) mod ord, and m = k mod 7429: a multiple whose cofactor has a factor 17, 19 or 23
+   is no candidate either, so it is stepped over but not visited (bucket_visit), which
+   leaves 16/17 x 18/19 x 22/23 of the wheel's multiples. */
 struct BucketWalk {
-    u64 b;
+    u64 d;
     u32 idx;
     u32 c;
-    u32 m17, m19, m23;
+    u32 m;
 };
 
 WW_HD BucketWalk bucket_start(u64 G0, u32 l, double linv, u32 ord, const uint16_t *res,
@@ -260,38 +275,39 @@ WW_HD BucketWalk bucket_start(u64 G0, u32 l, double linv, u32 ord, const uint16_
     }
     const u64 k = kq * kWheelMod + res[idx];
     BucketWalk w;
-    w.b = ((u64)l * k - G0) >> 1;
+    w.d = (u64)l * k - G0;
     w.idx = idx;
     /* k - 1 < 2^44 + 2^5 and ord >= 22: the quotient is below 2^44 */
     u64 q2;
     divmod_u64_u32(k - 1, ord, inv_below(ord), &q2, &w.c);
-    /* k mod 17, 19, 23 in 32 bits: kq < 2^30 and 30030 = 8, 10, 15 (mod 17, 19, 23) */
-    const u32 kq32 = (u32)kq, rr = res[idx];
-    w.m17 = ((kq32 % 17u) * 8u + rr) % 17u;
-    w.m19 = ((kq32 % 19u) * 10u + rr) % 19u;
-    w.m23 = ((kq32 % 23u) * 15u + rr) % 23u;
+    /* k mod 7429 in 32 bits: kq < 2^30, and (kq mod 7429) 314 + res < 2^22 */
+    w.m = ((u32)kq % kVisitMod * kWheelModVisit + res[idx]) % kVisitMod;
     return w;
 }
 
 /* Whether the walk's current multiple is a candidate position (its cofactor prime to 17,
-   19 and 23; prime to 30030 by the wheel). */
-WW_HD bool bucket_visit(const BucketWalk &w) { return w.m17 != 0u && w.m19 != 0u && w.m23 != 0u; }
-
-/* The next multiple: k -> k + gap, b -> b + gap l / 2, c -> (c + gap) mod ord with one
-   subtraction since gap <= 22 <= ord, and k mod 17, 19, 23 with at most two (gap < 34). */
-WW_HD void bucket_step(BucketWalk &w, u32 l, u32 ord, const uint8_t *gap)
+   19 and 23 by the bitmap; prime to 30030 by the wheel). */
+WW_HD bool bucket_visit(const BucketWalk &w, const uint32_t *visit)
 {
-    const u32 g = gap[w.idx];
-    w.b += (u64)(g >> 1) * l;
+    return (visit[w.m >> 5] >> (w.m & 31u)) & 1u;
+}
+
+/* The next multiple, the wheel gap g = gap[idx] ahead: k -> k + g, d -> d + g l,
+   c -> (c + g) mod ord and m -> (m + g) mod 7429, each with one subtraction (g <= 22 <= ord),
+   idx -> idx + 1 (the gaps are stored twice, so no wrap within a walk). */
+WW_HD void bucket_advance(BucketWalk &w, u32 l, u32 ord, u32 g)
+{
+    w.d += (u64)g * l;
     w.c += g;
     w.c = w.c >= ord ? w.c - ord : w.c;
-    w.m17 += g;
-    w.m17 = w.m17 >= 34u ? w.m17 - 34u : w.m17 >= 17u ? w.m17 - 17u : w.m17;
-    w.m19 += g;
-    w.m19 = w.m19 >= 38u ? w.m19 - 38u : w.m19 >= 19u ? w.m19 - 19u : w.m19;
-    w.m23 += g;
-    w.m23 = w.m23 >= 23u ? w.m23 - 23u : w.m23;
-    w.idx = w.idx + 1 == kWheelRes ? 0u : w.idx + 1;
+    w.m += g;
+    w.m = w.m >= kVisitMod ? w.m - kVisitMod : w.m;
+    ++w.idx;
+}
+
+WW_HD void bucket_step(BucketWalk &w, u32 l, u32 ord, const uint8_t *gap)
+{
+    bucket_advance(w, l, ord, gap[w.idx]);
 }
 
 /* Bucket entries: the odd-number index of the hit in its 2^20-bit chunk, and the top bit

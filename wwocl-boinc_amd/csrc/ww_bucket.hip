@@ -33,6 +33,11 @@ namespace ww {
 namespace {
 
 constexpr int kFillThreads = 1024;
+
+/* A bucket span has at most kBucketMaxChunks chunks of 2^20 odd numbers and its sieving
+   primes are above 2^20, so a walk meets at most 2 kBucketMaxChunks + 1 cofactors: its
+   wheel index stays inside the twice-stored gaps. */
+static_assert(2 * (u64)kBucketMaxChunks + 1 <= kWalkMaxK, "a bucket walk's window");
 constexpr int kPrimeThreads = 1024;
 constexpr u32 kPrimeBlockBits = 1u << 20; /* odd numbers per block of the prime sieve */
 constexpr u32 kPrimeBlockWords = kPrimeBlockBits / 32;
@@ -166,19 +171,19 @@ __global__ __launch_bounds__(kFillThreads) void k_bucket_fill(const BatchSpan *s
 {
     __shared__ u32 cur[kBucketMaxBands];
     __shared__ uint16_t res[kWheelRes];
-    __shared__ uint8_t gap[kWheelRes];
+    __shared__ uint8_t gap[2 * kWheelRes];
     __shared__ u32 wbits[kWheelWords];
     __shared__ uint16_t wrank[kWheelWords];
+    __shared__ u32 visit[kVisitWords];
     {
         static constexpr WheelTables wt = make_wheel();
-        for (u32 t = threadIdx.x; t < kWheelRes; t += kFillThreads) {
-            res[t] = wt.res[t];
-            gap[t] = wt.gap[t];
-        }
+        for (u32 t = threadIdx.x; t < kWheelRes; t += kFillThreads) res[t] = wt.res[t];
+        for (u32 t = threadIdx.x; t < 2 * kWheelRes; t += kFillThreads) gap[t] = wt.gap[t];
         for (u32 t = threadIdx.x; t < kWheelWords; t += kFillThreads) {
             wbits[t] = wt.bits[t];
             wrank[t] = wt.rank[t];
         }
+        for (u32 t = threadIdx.x; t < kVisitWords; t += kFillThreads) visit[t] = wt.visit[t];
     }
     const u32 groups = gridDim.x, wg = blockIdx.x;
     const u32 lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -218,22 +223,32 @@ __global__ __launch_bounds__(kFillThreads) void k_bucket_fill(const BatchSpan *s
             }
             if (i >= bs.pidx1 || l > bs.top) continue;
             BucketWalk w = bucket_start(G0, l, inv_below(l), ord, res, wbits, wrank);
-            /* walk in global odd-number indices; band buckets of this workgroup at
-               wbase + band * bstride */
-            w.b += bs.bit0;
-            const u64 Bend = bs.bit0 + nbits;
-            while (w.b < Bend) {
-                if (bucket_visit(w)) {
-                    const u32 j = (u32)(w.b >> kBucketChunkBits) - bs.chunk_first;
+            /* walk in global distances (twice the odd-number index); band buckets of this
+               workgroup at wbase + band * bstride. The next step's gap and visit word are
+               read one step ahead (bucket_step and bucket_visit, pipelined), so their LDS
+               latency hides behind the current step. */
+            w.d += 2 * bs.bit0;
+            const u64 Dend = 2 * (bs.bit0 + nbits);
+            u32 g = gap[w.idx];
+            u32 vw = visit[w.m >> 5];
+            while (w.d < Dend) {
+                const u32 gn = gap[w.idx + 1]; /* below 2 x 5760: the span's window */
+                u32 mn = w.m + g;
+                mn = mn >= kVisitMod ? mn - kVisitMod : mn;
+                const u32 vwn = visit[mn >> 5];
+                if ((vw >> (w.m & 31u)) & 1u) {
+                    const u32 j = (u32)(w.d >> (kBucketChunkBits + 1)) - bs.chunk_first;
                     const u32 band = j / kBucketBandChunks;
                     const u32 slot = wave_slot(cur, band);
                     if (slot < st.band_cap)
                         st.band_entries[wbase + (u64)band * bstride + slot] =
-                            ((u32)w.b & ((1u << kBucketChunkBits) - 1)) |
+                            ((u32)(w.d >> 1) & ((1u << kBucketChunkBits) - 1)) |
                             ((j % kBucketBandChunks) << kBucketChunkBits) |
                             (w.c == 0u ? kBucketRescue : 0u);
                 }
-                bucket_step(w, l, ord, gap);
+                bucket_advance(w, l, ord, g);
+                g = gn;
+                vw = vwn;
             }
         }
         __syncthreads();

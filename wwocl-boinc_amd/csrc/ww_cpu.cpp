@@ -172,14 +172,18 @@ int ww_cpu_bucket_hits(uint64_t g0, uint64_t nbits, uint32_t ell, uint64_t *bits
     if (!count || (cap && (!bits || !rescue))) return ww_set_error(WW_E_ARG, "null buffer");
     if ((g0 & 1) == 0 || ell <= (1u << 20) || (ell & 1) == 0)
         return ww_set_error(WW_E_ARG, "g0 must be odd and ell an odd prime above 2^20");
+    /* the window's cofactors k (l k in [g0, g0 + 2 nbits)) number at most 2 nbits / l + 1 */
+    if (nbits > (kWalkMaxK - 1) * ell / 2)
+        return ww_set_error(WW_E_ARG, "window of %llu odd numbers spans more than %llu multiples",
+                            (unsigned long long)nbits, (unsigned long long)kWalkMaxK);
     static constexpr WheelTables wt = make_wheel();
     const u32 ord = order2_u32(ell, small_div_table(), kSmallOddPrimes);
     BucketWalk w = bucket_start(g0, ell, inv_below(ell), ord, wt.res, wt.bits, wt.rank);
     uint64_t n = 0;
-    for (; w.b < nbits; bucket_step(w, ell, ord, wt.gap)) {
-        if (!bucket_visit(w)) continue;
+    for (; w.d < 2 * nbits; bucket_step(w, ell, ord, wt.gap)) {
+        if (!bucket_visit(w, wt.visit)) continue;
         if (n < cap) {
-            bits[n] = w.b;
+            bits[n] = w.d >> 1;
             rescue[n] = w.c == 0u;
         }
         ++n;
