@@ -143,6 +143,21 @@ def test_order_random():
         check_order(ell, wwlib.cpu_order2(ell))
 
 
+def test_longest_window():
+    """A window of 30029 cofactors, the most a walk may span, starting at the wheel's last
+    residue so that the walk's index runs through both copies of the gaps; one more bit is
+    refused."""
+    ell = 1048583
+    nbits = 30028 * ell // 2
+    for k0 in (30029, 30030 * 7 + 30029, 30030 * 1000 + 1):
+        g0 = ell * k0
+        got = wwlib.cpu_bucket_hits(g0, nbits, ell, cap=1 << 16)
+        assert got == brute(g0, nbits, ell), k0
+        assert len(got) > 4800  # about 5760 x 16/17 x 18/19 x 22/23
+    with pytest.raises(wwlib.WWError):
+        wwlib.cpu_bucket_hits(ell * 30029, nbits + 1, ell)
+
+
 def test_arguments():
     with pytest.raises(wwlib.WWError):
         wwlib.cpu_bucket_hits(2, 100, 2147483647)  # even start
