@@ -173,7 +173,7 @@ int ww_prime_orders(ww_ctx *ctx, uint64_t first, uint64_t n, uint32_t *out, uint
 
 /* The bucket sieve's walk of one sieving prime, on the host with the code the device runs
    (verification): the multiples n = ell*k in [g0, g0 + 2 nbits) with k prime to
-   30030 x 17 x 19 x 23 (g0 odd, 2^20 < ell < 2^32 prime, and 2 nbits <= 30028 ell: the
+   30030 x 17 x 19 x ... x 41 (g0 odd, 2^20 < ell < 2^32 prime, and 2 nbits <= 30028 ell: the
    window holds fewer than 30030 cofactors, as every bucket span's does), as bit indices
    (n - g0) / 2 with a 1 in rescue[] where 2^(n-1) = 1 (mod ell) (the prime's base-2
    pseudoprime class). Writes up to cap of each; *count = the number of hits. */

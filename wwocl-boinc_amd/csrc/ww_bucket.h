@@ -10,8 +10,9 @@
  * n = l*k are walked only for k prime to 30030 = 2*3*5*7*11*13: n is odd and prime to
  * 3..13 exactly when k is, and any other n is no candidate (every candidate is odd and
  * prime to 3..113). That skips 9255 of every 15015 odd multiples; those with 17, 19 or 23
- * dividing k are stepped over without a visit (k mod 7429 = 17 x 19 x 23 is carried and
- * looked up in a bitmap), which leaves 0.327 of them. Each visited n
+ * dividing k, and those with 29, 31, 37 or 41 dividing k, are stepped over without a visit
+ * (k mod 215441 = 17 x 19 x 23 x 29 and k mod 47027 = 31 x 37 x 41 are carried and looked
+ * up in two bitmaps), which leaves 0.290 of them. Each visited n
  * also gets its base-2 pseudoprime test modulo l. A composite n = l*k that is a base-2 Fermat
  * pseudoprime (every strong one is) has 2^(n-1) = 1 (mod l); such an n is not dropped but
  * flagged for the strong PRP test. By Fermat, 2^(n-1) = 2^(lk-1) = (2^(l-1))^k 2^(k-1) =
@@ -31,14 +32,16 @@ namespace ww {
    (a bitmap of the residues and the count of those below each 32-bit word). The gaps are
    stored twice over: a walk covers fewer than 30030 consecutive cofactors (kWalkMaxK), so
    it passes at most 5760 residues and its index, from below 5760, stays below 2 x 5760
-   without wrapping. Beside them, the bitmap of the residues of k mod 7429 = 17 x 19 x 23
-   prime to 7429: the cofactors that are visited. */
+   without wrapping. Beside them, the bitmaps of the residues of k mod 215441 =
+   17 x 19 x 23 x 29 and of k mod 47027 = 31 x 37 x 41 prime to their moduli: a cofactor is
+   visited when both bits are set. */
 constexpr u32 kWheelMod = 30030;
 constexpr u32 kWheelRes = 5760;
 constexpr u32 kWheelWords = (kWheelMod + 31) / 32;
-constexpr u32 kVisitMod = 17 * 19 * 23; /* 7429 */
-constexpr u32 kVisitWords = (kVisitMod + 31) / 32;
-constexpr u32 kWheelModVisit = kWheelMod % kVisitMod; /* 314 */
+constexpr u32 kVisitModA = 17 * 19 * 23 * 29; /* 215441 */
+constexpr u32 kVisitModB = 31 * 37 * 41;      /* 47027 */
+constexpr u32 kVisitWordsA = (kVisitModA + 31) / 32;
+constexpr u32 kVisitWordsB = (kVisitModB + 31) / 32;
 /* Most cofactors k in one walk: the window [kmin, kmax) is at most this long. */
 constexpr u64 kWalkMaxK = kWheelMod - 1;
 
@@ -48,7 +51,8 @@ struct WheelTables {
                                       entry i + 5760 = entry i */
     uint32_t bits[kWheelWords];    /* bit r % 32 of word r / 32: r is a residue */
     uint16_t rank[kWheelWords];    /* residues below 32 w */
-    uint32_t visit[kVisitWords];   /* bit m % 32 of word m / 32: m prime to 7429 */
+    uint32_t visit_a[kVisitWordsA]; /* bit m % 32 of word m / 32: m prime to 215441 */
+    uint32_t visit_b[kVisitWordsB]; /* the same for 47027 */
 };
 
 constexpr WheelTables make_wheel()
@@ -69,8 +73,14 @@ constexpr WheelTables make_wheel()
         t.rank[w] = (uint16_t)below;
         for (u32 b = 0; b < 32; ++b) below += (t.bits[w] >> b) & 1u;
     }
-    for (u32 m = 0; m < kVisitMod; ++m)
-        if (m % 17 && m % 19 && m % 23) t.visit[m / 32] |= 1u << (m % 32);
+    /* the visit bitmaps as sieves: every bit set, then the multiples of each factor
+       cleared (few enough steps for a constant expression) */
+    for (u32 w = 0; w < kVisitWordsA; ++w) t.visit_a[w] = ~0u;
+    for (u32 q : {17u, 19u, 23u, 29u})
+        for (u32 m = 0; m < kVisitModA; m += q) t.visit_a[m / 32] &= ~(1u << (m % 32));
+    for (u32 w = 0; w < kVisitWordsB; ++w) t.visit_b[w] = ~0u;
+    for (u32 q : {31u, 37u, 41u})
+        for (u32 m = 0; m < kVisitModB; m += q) t.visit_b[m / 32] &= ~(1u << (m % 32));
     return t;
 }
 
@@ -249,14 +259,14 @@ WW_HD u32 order2_u32(u32 l, const SmallDiv *tab, u32 ntab)
    on, for at most kWalkMaxK consecutive cofactors k. d = n - G0 is the current multiple's
    distance, twice its bit (the true distance is below 22 l at the start, so mod-2^64
    arithmetic gives it even where n passes 2^64), idx the wheel index of k, c =
-   (k - 1) mod ord, and m = k mod 7429: a multiple whose cofactor has a factor 17, 19 or 23
-   is no candidate either, so it is stepped over but not visited (bucket_visit), which
-   leaves 16/17 x 18/19 x 22/23 of the wheel's multiples. */
+   (k - 1) mod ord, ma = k mod 215441 and mb = k mod 47027: a multiple whose cofactor has a
+   factor from 17 to 41 is no candidate either, so it is stepped over but not visited
+   (bucket_visit), which leaves 16/17 x 18/19 x ... x 40/41 of the wheel's multiples. */
 struct BucketWalk {
     u64 d;
     u32 idx;
     u32 c;
-    u32 m;
+    u32 ma, mb;
 };
 
 WW_HD BucketWalk bucket_start(u64 G0, u32 l, double linv, u32 ord, const uint16_t *res,
@@ -280,28 +290,32 @@ WW_HD BucketWalk bucket_start(u64 G0, u32 l, double linv, u32 ord, const uint16_
     /* k - 1 < 2^44 + 2^5 and ord >= 22: the quotient is below 2^44 */
     u64 q2;
     divmod_u64_u32(k - 1, ord, inv_below(ord), &q2, &w.c);
-    /* k mod 7429 in 32 bits: kq < 2^30, and (kq mod 7429) 314 + res < 2^22 */
-    w.m = ((u32)kq % kVisitMod * kWheelModVisit + res[idx]) % kVisitMod;
+    /* k mod 215441 and mod 47027 (k < 2^45; 30030 is below both moduli) */
+    w.ma = (u32)(((u64)((u32)kq % kVisitModA) * kWheelMod + res[idx]) % kVisitModA);
+    w.mb = ((u32)kq % kVisitModB * kWheelMod + res[idx]) % kVisitModB;
     return w;
 }
 
-/* Whether the walk's current multiple is a candidate position (its cofactor prime to 17,
-   19 and 23 by the bitmap; prime to 30030 by the wheel). */
-WW_HD bool bucket_visit(const BucketWalk &w, const uint32_t *visit)
+/* Whether the walk's current multiple is a candidate position (its cofactor prime to 17
+   .. 41 by the bitmaps; prime to 30030 by the wheel). */
+WW_HD bool bucket_visit(const BucketWalk &w, const uint32_t *visit_a, const uint32_t *visit_b)
 {
-    return (visit[w.m >> 5] >> (w.m & 31u)) & 1u;
+    return (visit_a[w.ma >> 5] >> (w.ma & 31u)) & (visit_b[w.mb >> 5] >> (w.mb & 31u)) & 1u;
 }
 
 /* The next multiple, the wheel gap g = gap[idx] ahead: k -> k + g, d -> d + g l,
-   c -> (c + g) mod ord and m -> (m + g) mod 7429, each with one subtraction (g <= 22 <= ord),
+   c -> (c + g) mod ord, ma and mb -> (ma + g, mb + g) mod their moduli, each with one
+   subtraction (g <= 22 <= ord),
    idx -> idx + 1 (the gaps are stored twice, so no wrap within a walk). */
 WW_HD void bucket_advance(BucketWalk &w, u32 l, u32 ord, u32 g)
 {
     w.d += (u64)g * l;
     w.c += g;
     w.c = w.c >= ord ? w.c - ord : w.c;
-    w.m += g;
-    w.m = w.m >= kVisitMod ? w.m - kVisitMod : w.m;
+    w.ma += g;
+    w.ma = w.ma >= kVisitModA ? w.ma - kVisitModA : w.ma;
+    w.mb += g;
+    w.mb = w.mb >= kVisitModB ? w.mb - kVisitModB : w.mb;
     ++w.idx;
 }
 

@@ -174,7 +174,8 @@ __global__ __launch_bounds__(kFillThreads) void k_bucket_fill(const BatchSpan *s
     __shared__ uint8_t gap[2 * kWheelRes];
     __shared__ u32 wbits[kWheelWords];
     __shared__ uint16_t wrank[kWheelWords];
-    __shared__ u32 visit[kVisitWords];
+    __shared__ u32 visit_a[kVisitWordsA];
+    __shared__ u32 visit_b[kVisitWordsB];
     {
         static constexpr WheelTables wt = make_wheel();
         for (u32 t = threadIdx.x; t < kWheelRes; t += kFillThreads) res[t] = wt.res[t];
@@ -183,7 +184,8 @@ __global__ __launch_bounds__(kFillThreads) void k_bucket_fill(const BatchSpan *s
             wbits[t] = wt.bits[t];
             wrank[t] = wt.rank[t];
         }
-        for (u32 t = threadIdx.x; t < kVisitWords; t += kFillThreads) visit[t] = wt.visit[t];
+        for (u32 t = threadIdx.x; t < kVisitWordsA; t += kFillThreads) visit_a[t] = wt.visit_a[t];
+        for (u32 t = threadIdx.x; t < kVisitWordsB; t += kFillThreads) visit_b[t] = wt.visit_b[t];
     }
     const u32 groups = gridDim.x, wg = blockIdx.x;
     const u32 lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -224,19 +226,20 @@ __global__ __launch_bounds__(kFillThreads) void k_bucket_fill(const BatchSpan *s
             if (i >= bs.pidx1 || l > bs.top) continue;
             BucketWalk w = bucket_start(G0, l, inv_below(l), ord, res, wbits, wrank);
             /* walk in global distances (twice the odd-number index); band buckets of this
-               workgroup at wbase + band * bstride. The next step's gap and visit word are
+               workgroup at wbase + band * bstride. The next step's gap and visit words are
                read one step ahead (bucket_step and bucket_visit, pipelined), so their LDS
                latency hides behind the current step. */
             w.d += 2 * bs.bit0;
             const u64 Dend = 2 * (bs.bit0 + nbits);
             u32 g = gap[w.idx];
-            u32 vw = visit[w.m >> 5];
+            u32 va = visit_a[w.ma >> 5], vb = visit_b[w.mb >> 5];
             while (w.d < Dend) {
                 const u32 gn = gap[w.idx + 1]; /* below 2 x 5760: the span's window */
-                u32 mn = w.m + g;
-                mn = mn >= kVisitMod ? mn - kVisitMod : mn;
-                const u32 vwn = visit[mn >> 5];
-                if ((vw >> (w.m & 31u)) & 1u) {
+                u32 man = w.ma + g, mbn = w.mb + g;
+                man = man >= kVisitModA ? man - kVisitModA : man;
+                mbn = mbn >= kVisitModB ? mbn - kVisitModB : mbn;
+                const u32 van = visit_a[man >> 5], vbn = visit_b[mbn >> 5];
+                if ((va >> (w.ma & 31u)) & (vb >> (w.mb & 31u)) & 1u) {
                     const u32 j = (u32)(w.d >> (kBucketChunkBits + 1)) - bs.chunk_first;
                     const u32 band = j / kBucketBandChunks;
                     const u32 slot = wave_slot(cur, band);
@@ -248,7 +251,8 @@ __global__ __launch_bounds__(kFillThreads) void k_bucket_fill(const BatchSpan *s
                 }
                 bucket_advance(w, l, ord, g);
                 g = gn;
-                vw = vwn;
+                va = van;
+                vb = vbn;
             }
         }
         __syncthreads();
