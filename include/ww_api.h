@@ -150,7 +150,7 @@ int ww_get_presieve_bound(ww_ctx *ctx, uint32_t *bound);
    as before. The PRP list, checksums and records are unchanged. It takes effect where the
    deep bound is at least 1024, the pre-sieve bound at least 2^20 and the pre-sieve segment
    2^20 bits (the defaults); elsewhere such a batch runs as without it. Costs per range
-   about 0.85 bytes of device memory per integer of width (7.3 GB for a 2^33-wide range;
+   about 0.71 bytes of device memory per integer of width (6.1 GB for a 2^33-wide range;
    single chunks keep one such store per context, for the widest chunk so far) and, on
    first use, the odd primes below 2^32 and their base-2 orders, 813 MB each, built on the
    device in about 0.05 s. Default WW_DEFAULT_BUCKET_MIN_WIDTH. */

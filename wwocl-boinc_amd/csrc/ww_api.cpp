@@ -64,7 +64,8 @@ struct ww_ctx {
     ww::BatchState *d_fstate = nullptr;
     SlotSum *d_fsum = nullptr;
     ww_result *d_fres = nullptr;
-    /* fused chunks with the bucket sieve: band and chunk buckets for up to fb_segs segments
+    /* single chunks with the bucket sieve (ww_search_fused, ww_getsegprimes): band and
+       chunk buckets for up to fb_segs segments
        (sized for sieving primes up to 2^32 at capacity scale fb_scale), the bucket span,
        the overflow count and the needs-PRP words */
     ww::BucketSpan *d_fbspan = nullptr;
@@ -408,11 +409,11 @@ int ensure_fused_buckets(ww_ctx *c, uint64_t nseg, bool *avail)
     return WW_OK;
 }
 
-/* A fused chunk with the bucket sieve, when [low, high) is at least the bucket width, the
-   sieve tiers allow known primes and the chunk is pre-sieved: its span and bucket span,
-   the fill and split, and the pre-sieve with the needs-PRP words; *pre and *prer = the two
-   sets of words. Both stay null when the chunk does not qualify (the plain pre-sieve then
-   runs). */
+/* A single chunk (of ww_search_fused or ww_getsegprimes) with the bucket sieve, when
+   [low, high) is at least the bucket width, the sieve tiers allow known primes and the chunk
+   is pre-sieved: its span and bucket span, the fill and split, and the pre-sieve with the
+   needs-PRP words; *pre and *prer = the two sets of words. Both stay null when the chunk
+   does not qualify (the plain pre-sieve then runs). */
 int chunk_buckets(ww_ctx *c, uint64_t low, uint64_t high, const uint64_t **pre,
                   const uint64_t **prer)
 {

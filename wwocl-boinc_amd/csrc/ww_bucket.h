@@ -85,7 +85,7 @@ constexpr WheelTables make_wheel()
 }
 
 /* The wheel's largest gap: bucket_advance's single subtractions mod ord (ord >= 22) and
-   mod 7429 rely on it. */
+   mod the two visit moduli rely on it. */
 constexpr u32 kWheelMaxGap = 22;
 
 constexpr u32 wheel_max_gap()
