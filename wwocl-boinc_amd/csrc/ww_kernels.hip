@@ -124,7 +124,10 @@ __global__ __launch_bounds__(kSieveThreads) void k_sieve_prp(u64 base, u64 hi, u
  * ------------------------------------------------------------------------- */
 
 #ifndef WW_TESTS_WAVES_PER_EU
-#define WW_TESTS_WAVES_PER_EU 6 /* 80 VGPRs; 4% faster than 4 waves at 108 */
+/* 80 VGPRs and 18 spilled; 4% faster than 4 waves at 108. 8 waves (64 VGPRs, 36 spilled)
+   are 1.4% faster still, but their spill stores leave the L2: 3.4 GB written per 2^32 chunk
+   at 2^63 instead of 18 MB */
+#define WW_TESTS_WAVES_PER_EU 6
 #endif
 
 __global__ __launch_bounds__(kTestThreads)
