@@ -150,7 +150,7 @@ int ww_get_presieve_bound(ww_ctx *ctx, uint32_t *bound);
    as before. The PRP list, checksums and records are unchanged. It takes effect where the
    deep bound is at least 1024, the pre-sieve bound at least 2^20 and the pre-sieve segment
    2^20 bits (the defaults); elsewhere such a batch runs as without it. Costs per range
-   about 0.71 bytes of device memory per integer of width (6.1 GB for a 2^33-wide range;
+   about 0.67 bytes of device memory per integer of width (5.7 GB for a 2^33-wide range;
    single chunks keep one such store per context, for the widest chunk so far) and, on
    first use, the odd primes below 2^32 and their base-2 orders, 813 MB each, built on the
    device in about 0.05 s. Default WW_DEFAULT_BUCKET_MIN_WIDTH. */
@@ -173,7 +173,7 @@ int ww_prime_orders(ww_ctx *ctx, uint64_t first, uint64_t n, uint32_t *out, uint
 
 /* The bucket sieve's walk of one sieving prime, on the host with the code the device runs
    (verification): the multiples n = ell*k in [g0, g0 + 2 nbits) with k prime to
-   30030 x 17 x 19 x ... x 41 (g0 odd, 2^20 < ell < 2^32 prime, and 2 nbits <= 30028 ell: the
+   30030 x 17 x 19 x ... x 53 (g0 odd, 2^20 < ell < 2^32 prime, and 2 nbits <= 30028 ell: the
    window holds fewer than 30030 cofactors, as every bucket span's does), as bit indices
    (n - g0) / 2 with a 1 in rescue[] where 2^(n-1) = 1 (mod ell) (the prime's base-2
    pseudoprime class). Writes up to cap of each; *count = the number of hits. */

@@ -1,9 +1,9 @@
 """The bucket sieve's walk of one sieving prime (csrc/ww_bucket.h), built for the host
 through ww_cpu_bucket_hits: for a prime l above 2^20 and the odd numbers g0 + 2b, it must
-visit exactly the multiples n = l*k with k prime to 30030 x 17 x 19 x ... x 41 (in order,
+visit exactly the multiples n = l*k with k prime to 30030 x 17 x 19 x ... x 53 (in order,
 as bit indices (n - g0) / 2), and flag exactly those with 2^(n-1) = 1 (mod l), the prime's
 base-2 pseudoprime class. The expected lists come from Python's exact integers: every
-multiple of l in the window, a gcd with 30030 x 17 x 19 x ... x 41, and pow(2, n - 1, l).
+multiple of l in the window, a gcd with 30030 x 17 x 19 x ... x 53, and pow(2, n - 1, l).
 Primes of small multiplicative order (factors of 2^31 - 1, 2^37 - 1, 2^41 - 1, ...) make the
 class frequent, windows near 2^64 make the multiples pass 2^64. The walk's class test uses the prime's base-2 order, which
 ww_cpu_order2 computes with the order table's arithmetic; it is checked against the
@@ -21,7 +21,7 @@ def brute(g0, nbits, ell):
     out = []
     k = -(-g0 // ell)
     while ell * k < g0 + 2 * nbits:
-        if math.gcd(k, 30030 * 17 * 19 * 23 * 29 * 31 * 37 * 41) == 1:
+        if math.gcd(k, 30030 * 17 * 19 * 23 * 29 * 31 * 37 * 41 * 43 * 47 * 53) == 1:
             n = ell * k
             out.append(((n - g0) // 2, pow(2, n - 1, ell) == 1))
         k += 1
@@ -71,7 +71,7 @@ def test_extreme_primes():
     g0 = (1 << 63) + 1
     got = wwlib.cpu_bucket_hits(g0, 1 << 32, 1048583, cap=1 << 16)
     assert got == brute(g0, 1 << 32, 1048583)
-    assert len(got) > 1150  # 2^32 / l odd multiples, 0.290 of them visited
+    assert len(got) > 1050  # 2^32 / l odd multiples, 0.272 of them visited
 
 
 def prime_factors(n):
@@ -153,7 +153,7 @@ def test_longest_window():
         g0 = ell * k0
         got = wwlib.cpu_bucket_hits(g0, nbits, ell, cap=1 << 16)
         assert got == brute(g0, nbits, ell), k0
-        assert len(got) > 4300  # about 5760 x 16/17 x 18/19 x ... x 40/41
+        assert len(got) > 3900  # about 5760 x 16/17 x 18/19 x ... x 52/53
     with pytest.raises(wwlib.WWError):
         wwlib.cpu_bucket_hits(ell * 30029, nbits + 1, ell)
 

@@ -279,9 +279,10 @@ bool bucket_config_ok(const ww_ctx *c)
 /* Table index of the first prime above the pre-sieve bound: the odd primes up to 113 (29)
    and the wide table's primes up to the bound. */
 /* The share of a sieving prime's odd multiples that the bucket walk visits: cofactors
-   prime to 30030 (5760 of 15015 odd ones), and to the primes from 17 to 41. */
+   prime to 30030 (5760 of 15015 odd ones), and to the primes from 17 to 53. */
 constexpr double kVisited = 5760.0 / 15015.0 * 16.0 / 17.0 * 18.0 / 19.0 * 22.0 / 23.0 *
-                            28.0 / 29.0 * 30.0 / 31.0 * 36.0 / 37.0 * 40.0 / 41.0;
+                            28.0 / 29.0 * 30.0 / 31.0 * 36.0 / 37.0 * 40.0 / 41.0 *
+                            42.0 / 43.0 * 46.0 / 47.0 * 52.0 / 53.0;
 
 uint64_t bucket_first_index(const ww_ctx *c)
 {

@@ -10,9 +10,10 @@
  * n = l*k are walked only for k prime to 30030 = 2*3*5*7*11*13: n is odd and prime to
  * 3..13 exactly when k is, and any other n is no candidate (every candidate is odd and
  * prime to 3..113). That skips 9255 of every 15015 odd multiples; those with 17, 19 or 23
- * dividing k, and those with 29, 31, 37 or 41 dividing k, are stepped over without a visit
- * (k mod 215441 = 17 x 19 x 23 x 29 and k mod 47027 = 31 x 37 x 41 are carried and looked
- * up in two bitmaps), which leaves 0.290 of them. Each visited n
+ * dividing k, and those with a prime factor from 29 to 53 dividing k, are stepped over
+ * without a visit (k mod 215441 = 17 x 19 x 23 x 29, k mod 47027 = 31 x 37 x 41 and
+ * k mod 107113 = 43 x 47 x 53 are carried and looked up in three bitmaps), which leaves
+ * 0.272 of them. Each visited n
  * also gets its base-2 pseudoprime test modulo l. A composite n = l*k that is a base-2 Fermat
  * pseudoprime (every strong one is) has 2^(n-1) = 1 (mod l); such an n is not dropped but
  * flagged for the strong PRP test. By Fermat, 2^(n-1) = 2^(lk-1) = (2^(l-1))^k 2^(k-1) =
@@ -33,15 +34,17 @@ namespace ww {
    stored twice over: a walk covers fewer than 30030 consecutive cofactors (kWalkMaxK), so
    it passes at most 5760 residues and its index, from below 5760, stays below 2 x 5760
    without wrapping. Beside them, the bitmaps of the residues of k mod 215441 =
-   17 x 19 x 23 x 29 and of k mod 47027 = 31 x 37 x 41 prime to their moduli: a cofactor is
-   visited when both bits are set. */
+   17 x 19 x 23 x 29, k mod 47027 = 31 x 37 x 41 and k mod 107113 = 43 x 47 x 53 prime to
+   their moduli: a cofactor is visited when all three bits are set. */
 constexpr u32 kWheelMod = 30030;
 constexpr u32 kWheelRes = 5760;
 constexpr u32 kWheelWords = (kWheelMod + 31) / 32;
 constexpr u32 kVisitModA = 17 * 19 * 23 * 29; /* 215441 */
 constexpr u32 kVisitModB = 31 * 37 * 41;      /* 47027 */
+constexpr u32 kVisitModC = 43 * 47 * 53;      /* 107113 */
 constexpr u32 kVisitWordsA = (kVisitModA + 31) / 32;
 constexpr u32 kVisitWordsB = (kVisitModB + 31) / 32;
+constexpr u32 kVisitWordsC = (kVisitModC + 31) / 32;
 /* Most cofactors k in one walk: the window [kmin, kmax) is at most this long. */
 constexpr u64 kWalkMaxK = kWheelMod - 1;
 
@@ -53,6 +56,7 @@ struct WheelTables {
     uint16_t rank[kWheelWords];    /* residues below 32 w */
     uint32_t visit_a[kVisitWordsA]; /* bit m % 32 of word m / 32: m prime to 215441 */
     uint32_t visit_b[kVisitWordsB]; /* the same for 47027 */
+    uint32_t visit_c[kVisitWordsC]; /* and for 107113 */
 };
 
 constexpr WheelTables make_wheel()
@@ -81,6 +85,9 @@ constexpr WheelTables make_wheel()
     for (u32 w = 0; w < kVisitWordsB; ++w) t.visit_b[w] = ~0u;
     for (u32 q : {31u, 37u, 41u})
         for (u32 m = 0; m < kVisitModB; m += q) t.visit_b[m / 32] &= ~(1u << (m % 32));
+    for (u32 w = 0; w < kVisitWordsC; ++w) t.visit_c[w] = ~0u;
+    for (u32 q : {43u, 47u, 53u})
+        for (u32 m = 0; m < kVisitModC; m += q) t.visit_c[m / 32] &= ~(1u << (m % 32));
     return t;
 }
 
@@ -259,14 +266,15 @@ WW_HD u32 order2_u32(u32 l, const SmallDiv *tab, u32 ntab)
    on, for at most kWalkMaxK consecutive cofactors k. d = n - G0 is the current multiple's
    distance, twice its bit (the true distance is below 22 l at the start, so mod-2^64
    arithmetic gives it even where n passes 2^64), idx the wheel index of k, c =
-   (k - 1) mod ord, ma = k mod 215441 and mb = k mod 47027: a multiple whose cofactor has a
-   factor from 17 to 41 is no candidate either, so it is stepped over but not visited
-   (bucket_visit), which leaves 16/17 x 18/19 x ... x 40/41 of the wheel's multiples. */
+   (k - 1) mod ord, ma = k mod 215441, mb = k mod 47027 and mc = k mod 107113: a multiple
+   whose cofactor has a factor from 17 to 53 is no candidate either, so it is stepped over
+   but not visited (bucket_visit), which leaves 16/17 x 18/19 x ... x 52/53 of the wheel's
+   multiples. */
 struct BucketWalk {
     u64 d;
     u32 idx;
     u32 c;
-    u32 ma, mb;
+    u32 ma, mb, mc;
 };
 
 WW_HD BucketWalk bucket_start(u64 G0, u32 l, double linv, u32 ord, const uint16_t *res,
@@ -290,22 +298,25 @@ WW_HD BucketWalk bucket_start(u64 G0, u32 l, double linv, u32 ord, const uint16_
     /* k - 1 < 2^44 + 2^5 and ord >= 22: the quotient is below 2^44 */
     u64 q2;
     divmod_u64_u32(k - 1, ord, inv_below(ord), &q2, &w.c);
-    /* k mod 215441 and mod 47027 (k < 2^45; 30030 is below both moduli) */
+    /* k mod 215441, 47027 and 107113 (k < 2^45; 30030 is below the moduli) */
     w.ma = (u32)(((u64)((u32)kq % kVisitModA) * kWheelMod + res[idx]) % kVisitModA);
     w.mb = ((u32)kq % kVisitModB * kWheelMod + res[idx]) % kVisitModB;
+    w.mc = (u32)(((u64)((u32)kq % kVisitModC) * kWheelMod + res[idx]) % kVisitModC);
     return w;
 }
 
 /* Whether the walk's current multiple is a candidate position (its cofactor prime to 17
-   .. 41 by the bitmaps; prime to 30030 by the wheel). */
-WW_HD bool bucket_visit(const BucketWalk &w, const uint32_t *visit_a, const uint32_t *visit_b)
+   .. 53 by the bitmaps; prime to 30030 by the wheel). */
+WW_HD bool bucket_visit(const BucketWalk &w, const uint32_t *visit_a, const uint32_t *visit_b,
+                        const uint32_t *visit_c)
 {
-    return (visit_a[w.ma >> 5] >> (w.ma & 31u)) & (visit_b[w.mb >> 5] >> (w.mb & 31u)) & 1u;
+    return (visit_a[w.ma >> 5] >> (w.ma & 31u)) & (visit_b[w.mb >> 5] >> (w.mb & 31u)) &
+           (visit_c[w.mc >> 5] >> (w.mc & 31u)) & 1u;
 }
 
 /* The next multiple, the wheel gap g = gap[idx] ahead: k -> k + g, d -> d + g l,
-   c -> (c + g) mod ord, ma and mb -> (ma + g, mb + g) mod their moduli, each with one
-   subtraction (g <= 22 <= ord),
+   c -> (c + g) mod ord, ma, mb and mc -> (ma + g, mb + g, mc + g) mod their moduli, each
+   with one subtraction (g <= 22 <= ord),
    idx -> idx + 1 (the gaps are stored twice, so no wrap within a walk). */
 WW_HD void bucket_advance(BucketWalk &w, u32 l, u32 ord, u32 g)
 {
@@ -316,6 +327,8 @@ WW_HD void bucket_advance(BucketWalk &w, u32 l, u32 ord, u32 g)
     w.ma = w.ma >= kVisitModA ? w.ma - kVisitModA : w.ma;
     w.mb += g;
     w.mb = w.mb >= kVisitModB ? w.mb - kVisitModB : w.mb;
+    w.mc += g;
+    w.mc = w.mc >= kVisitModC ? w.mc - kVisitModC : w.mc;
     ++w.idx;
 }
 

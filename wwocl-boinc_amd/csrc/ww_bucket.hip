@@ -176,6 +176,7 @@ __global__ __launch_bounds__(kFillThreads) void k_bucket_fill(const BatchSpan *s
     __shared__ uint16_t wrank[kWheelWords];
     __shared__ u32 visit_a[kVisitWordsA];
     __shared__ u32 visit_b[kVisitWordsB];
+    __shared__ u32 visit_c[kVisitWordsC];
     {
         static constexpr WheelTables wt = make_wheel();
         for (u32 t = threadIdx.x; t < kWheelRes; t += kFillThreads) res[t] = wt.res[t];
@@ -186,6 +187,7 @@ __global__ __launch_bounds__(kFillThreads) void k_bucket_fill(const BatchSpan *s
         }
         for (u32 t = threadIdx.x; t < kVisitWordsA; t += kFillThreads) visit_a[t] = wt.visit_a[t];
         for (u32 t = threadIdx.x; t < kVisitWordsB; t += kFillThreads) visit_b[t] = wt.visit_b[t];
+        for (u32 t = threadIdx.x; t < kVisitWordsC; t += kFillThreads) visit_c[t] = wt.visit_c[t];
     }
     const u32 groups = gridDim.x, wg = blockIdx.x;
     const u32 lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -232,14 +234,15 @@ __global__ __launch_bounds__(kFillThreads) void k_bucket_fill(const BatchSpan *s
             w.d += 2 * bs.bit0;
             const u64 Dend = 2 * (bs.bit0 + nbits);
             u32 g = gap[w.idx];
-            u32 va = visit_a[w.ma >> 5], vb = visit_b[w.mb >> 5];
+            u32 va = visit_a[w.ma >> 5], vb = visit_b[w.mb >> 5], vc = visit_c[w.mc >> 5];
             while (w.d < Dend) {
                 const u32 gn = gap[w.idx + 1]; /* below 2 x 5760: the span's window */
-                u32 man = w.ma + g, mbn = w.mb + g;
+                u32 man = w.ma + g, mbn = w.mb + g, mcn = w.mc + g;
                 man = man >= kVisitModA ? man - kVisitModA : man;
                 mbn = mbn >= kVisitModB ? mbn - kVisitModB : mbn;
-                const u32 van = visit_a[man >> 5], vbn = visit_b[mbn >> 5];
-                if ((va >> (w.ma & 31u)) & (vb >> (w.mb & 31u)) & 1u) {
+                mcn = mcn >= kVisitModC ? mcn - kVisitModC : mcn;
+                const u32 van = visit_a[man >> 5], vbn = visit_b[mbn >> 5], vcn = visit_c[mcn >> 5];
+                if ((va >> (w.ma & 31u)) & (vb >> (w.mb & 31u)) & (vc >> (w.mc & 31u)) & 1u) {
                     const u32 j = (u32)(w.d >> (kBucketChunkBits + 1)) - bs.chunk_first;
                     const u32 band = j / kBucketBandChunks;
                     const u32 slot = wave_slot(cur, band);
@@ -253,6 +256,7 @@ __global__ __launch_bounds__(kFillThreads) void k_bucket_fill(const BatchSpan *s
                 g = gn;
                 va = van;
                 vb = vbn;
+                vc = vcn;
             }
         }
         __syncthreads();

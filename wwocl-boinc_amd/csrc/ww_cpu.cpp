@@ -181,7 +181,7 @@ int ww_cpu_bucket_hits(uint64_t g0, uint64_t nbits, uint32_t ell, uint64_t *bits
     BucketWalk w = bucket_start(g0, ell, inv_below(ell), ord, wt.res, wt.bits, wt.rank);
     uint64_t n = 0;
     for (; w.d < 2 * nbits; bucket_step(w, ell, ord, wt.gap)) {
-        if (!bucket_visit(w, wt.visit_a, wt.visit_b)) continue;
+        if (!bucket_visit(w, wt.visit_a, wt.visit_b, wt.visit_c)) continue;
         if (n < cap) {
             bits[n] = w.d >> 1;
             rescue[n] = w.c == 0u;

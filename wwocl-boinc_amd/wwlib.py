@@ -315,7 +315,7 @@ def cpu_spsp2(n: int, top: int = -1, tmax: int = -1) -> bool:
 
 def cpu_bucket_hits(g0: int, nbits: int, ell: int, cap: int = 1 << 16):
     """Host build of the bucket sieve's walk of one prime: [(bit, in pseudoprime class)]
-    for the multiples ell*k of [g0, g0 + 2 nbits) with k prime to 30030 x 17 x 19 x ... x 41."""
+    for the multiples ell*k of [g0, g0 + 2 nbits) with k prime to 30030 x 17 x 19 x ... x 53."""
     bits = (U64 * max(cap, 1))()
     resc = (ctypes.c_uint8 * max(cap, 1))()
     n = U64()
